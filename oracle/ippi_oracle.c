@@ -1,0 +1,712 @@
+/*
+ * IPPI-LBM D2Q9 time step -- CPU oracle, FP64 (TEST INFRASTRUCTURE ONLY).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this file's library, and only as the checker. The HIP library never
+ * links it.
+ *
+ * Each function cites the reference lines it restates. Floating-point
+ * expressions keep the reference's evaluation order (the file is built
+ * with -ffp-contract=off) so the FP64 HIP path can be compared bit for bit.
+ *
+ * Pins (tests/test_oracle_*.py):
+ *   or_feq, or_feq_init, or_macro, or_collide_bgk/mrt/rr, or_guo
+ *       bitwise against the reference's own Lattice methods, compiled from
+ *       IPPI-LBM/NodeAndLattice.cpp by `make -C oracle ref` (golden fixtures
+ *       in tests/golden/), plus moment identities.
+ *   or_les_tau   the quadratic it solves, tau (tau - tau0) = 4.5 Cs^2 |S|/rho,
+ *                with |S| from Pi_neq computed independently; Cs = 0 and
+ *                Pi_neq = 0 give tau0.
+ *   or_delta     Peskin identities (sum = 1, first moment 0, sum of
+ *                squares 3/8).
+ *   streaming / BC / IB composition: mass conservation, permutation check
+ *                by brute force and numpy.roll, decay of a shear wave between
+ *                halfway bounce-back walls, interpolation of linear
+ *                fields, spread = transpose of interpolation.
+ *   MLP          torch autograd (FP64) and central finite differences.
+ */
+#include "ippi_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* NodeAndLattice.cpp:3-14 */
+static const int EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+static const int EY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+static const int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6}; /* mesh.cpp:236 */
+#define W0 (4. / 9.)
+#define W1 (1. / 9.)
+#define W2 (1. / 36.)
+static const double WEIGHT[9] = {W0, W1, W1, W1, W1, W2, W2, W2, W2};
+
+/* NodeAndLattice.cpp:18-29 (MRT transform and its inverse) */
+static const double TM[9][9] = {
+    {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0},
+    {-4.0, -1.0, -1.0, -1.0, -1.0, 2.0, 2.0, 2.0, 2.0},
+    {4.0, -2.0, -2.0, -2.0, -2.0, 1.0, 1.0, 1.0, 1.0},
+    {0.0, 1.0, 0.0, -1.0, 0.0, 1.0, -1.0, -1.0, 1.0},
+    {0.0, -2.0, 0.0, 2.0, 0.0, 1.0, -1.0, -1.0, 1.0},
+    {0.0, 0.0, 1.0, 0.0, -1.0, 1.0, 1.0, -1.0, -1.0},
+    {0.0, 0.0, -2.0, 0.0, 2.0, 1.0, 1.0, -1.0, -1.0},
+    {0.0, 1.0, -1.0, 1.0, -1.0, 0.0, 0.0, 0.0, 0.0},
+    {0.0, 0.0, 0.0, 0.0, 0.0, 1.0, -1.0, 1.0, -1.0}};
+/* entries are integer multiples of a1 = 1/36, each product rounded once */
+static const int TMINV_K[9][9] = {
+    {4, -4, 4, 0, 0, 0, 0, 0, 0},    {4, -1, -2, 6, -6, 0, 0, 9, 0},
+    {4, -1, -2, 0, 0, 6, -6, -9, 0}, {4, -1, -2, -6, 6, 0, 0, 9, 0},
+    {4, -1, -2, 0, 0, -6, 6, -9, 0}, {4, 2, 1, 6, 3, 6, 3, 0, 9},
+    {4, 2, 1, -6, -3, 6, 3, 0, -9},  {4, 2, 1, -6, -3, -6, -3, 0, 9},
+    {4, 2, 1, 6, 3, -6, -3, 0, -9}};
+
+static double tminv(int i, int j) {
+    const double a1 = 1.0 / 36.0;
+    return (double)TMINV_K[i][j] * a1;
+}
+
+/* Compute_feq, NodeAndLattice.cpp:124-149 */
+void or_feq(double rho, double ux, double uy, double feq[9]) {
+    double w0 = rho * (4.0 / 9.0);
+    double w1 = rho / 9.0;
+    double w2 = rho / 36.0;
+    double uu = 1 - 1.5 * (ux * ux + uy * uy);
+    feq[0] = w0 * (uu);
+    feq[1] = w1 * (uu + 3.0 * (ux) + 4.5 * (ux) * (ux));
+    feq[2] = w1 * (uu + 3.0 * (uy) + 4.5 * (uy) * (uy));
+    feq[3] = w1 * (uu - 3.0 * (ux) + 4.5 * (ux) * (ux));
+    feq[4] = w1 * (uu - 3.0 * (uy) + 4.5 * (uy) * (uy));
+    feq[5] = w2 * (uu + 3.0 * (ux + uy) + 4.5 * (ux + uy) * (ux + uy));
+    feq[6] = w2 * (uu - 3.0 * (ux - uy) + 4.5 * (ux - uy) * (ux - uy));
+    feq[7] = w2 * (uu - 3.0 * (ux + uy) + 4.5 * (ux + uy) * (ux + uy));
+    feq[8] = w2 * (uu + 3.0 * (ux - uy) + 4.5 * (ux - uy) * (ux - uy));
+}
+
+/* InitialLatticeFeq, NodeAndLattice.cpp:104-119 (note rho*4.0/9.0) */
+void or_feq_init(double rho, double ux, double uy, double f[9]) {
+    double w0 = rho * 4.0 / 9.0;
+    double w1 = rho / 9.0;
+    double w2 = rho / 36.0;
+    double uu = 1 - 1.5 * (ux * ux + uy * uy);
+    f[0] = w0 * (uu);
+    f[1] = w1 * (uu + 3.0 * (ux) + 4.5 * (ux) * (ux));
+    f[2] = w1 * (uu + 3.0 * (uy) + 4.5 * (uy) * (uy));
+    f[3] = w1 * (uu - 3.0 * (ux) + 4.5 * (ux) * (ux));
+    f[4] = w1 * (uu - 3.0 * (uy) + 4.5 * (uy) * (uy));
+    f[5] = w2 * (uu + 3.0 * (ux + uy) + 4.5 * (ux + uy) * (ux + uy));
+    f[6] = w2 * (uu - 3.0 * (ux - uy) + 4.5 * (ux - uy) * (ux - uy));
+    f[7] = w2 * (uu - 3.0 * (ux + uy) + 4.5 * (ux + uy) * (ux + uy));
+    f[8] = w2 * (uu + 3.0 * (ux - uy) + 4.5 * (ux - uy) * (ux - uy));
+}
+
+/* CalMacro, NodeAndLattice.cpp:196-226 (half-force shift commented out there) */
+void or_macro(const double f[9], double out[3]) {
+    double trho = 0.0, tx = 0.0, ty = 0.0;
+    for (int k = 0; k < 9; k++) {
+        trho += f[k];
+        tx += f[k] * EX[k];
+        ty += f[k] * EY[k];
+    }
+    if (trho < 1.e-6) {
+        tx = 0.0;
+        ty = 0.0;
+    } else {
+        tx /= trho;
+        ty /= trho;
+    }
+    out[0] = trho;
+    out[1] = tx;
+    out[2] = ty;
+}
+
+/* ComputeBGKCollision, NodeAndLattice.cpp:150-161 */
+void or_collide_bgk(const double f[9], double rho, double ux, double uy, double tau, double out[9]) {
+    double feq[9];
+    or_feq(rho, ux, uy, feq);
+    for (int k = 0; k < 9; k++) out[k] = (1.0 - 1.0 / tau) * f[k] + feq[k] / tau;
+}
+
+/* ComputeMRTCollision, NodeAndLattice.cpp:290-360 (relaxation rates :306) */
+void or_collide_mrt(const double f[9], double rho, double ux, double uy, double tau, double out[9]) {
+    double sm[9] = {0, 1.2, 1.1, 0, 1.2, 0, 1.2, 1. / tau, 1. / tau};
+    double fmom[9], fmeq[9];
+    double t1 = ux * ux + uy * uy;
+    double t2 = ux * ux - uy * uy;
+    fmeq[0] = rho;
+    fmeq[1] = rho * (-2.0 + 3.0 * t1);
+    fmeq[2] = rho * (1.0 - 3.0 * t1);
+    fmeq[3] = rho * ux;
+    fmeq[4] = -rho * ux;
+    fmeq[5] = rho * uy;
+    fmeq[6] = -rho * uy;
+    fmeq[7] = rho * t2;
+    fmeq[8] = rho * ux * uy;
+    for (int k = 0; k < 9; k++) {
+        double s = 0.0;
+        for (int l = 0; l < 9; l++) s += TM[k][l] * f[l];
+        fmom[k] = s;
+    }
+    for (int k = 0; k < 9; k++) {
+        double s = 0.;
+        for (int l = 0; l < 9; l++) s += (tminv(k, l) * sm[l]) * (fmom[l] - fmeq[l]);
+        out[k] = f[k] - s;
+    }
+}
+
+/* ComputeRRCollision, NodeAndLattice.cpp:228-286 */
+void or_collide_rr(const double f[9], double rho, double ux, double uy, double tau, double out[9]) {
+    double feq[9];
+    or_feq(rho, ux, uy, feq);
+    double xx = 0, txx = 0, xy = 0, txy = 0, yx = 0, tyx = 0, yy = 0, tyy = 0;
+    for (int k = 0; k < 9; k++) {
+        xx += f[k] * EX[k] * EX[k];
+        txx += feq[k] * EX[k] * EX[k];
+        xy += f[k] * EX[k] * EY[k];
+        txy += feq[k] * EX[k] * EY[k];
+        yx += f[k] * EY[k] * EX[k];
+        tyx += feq[k] * EY[k] * EX[k];
+        yy += f[k] * EY[k] * EY[k];
+        tyy += feq[k] * EY[k] * EY[k];
+    }
+    double nxx = xx - txx, nxy = xy - txy, nyx = yx - tyx, nyy = yy - tyy;
+    for (int k = 0; k < 9; k++) {
+        double q0 = EX[k] * EX[k] - 1. / 3;
+        double q1 = EX[k] * EY[k];
+        double q2 = EY[k] * EX[k];
+        double q3 = EY[k] * EY[k] - 1. / 3;
+        double fone = 4.5 * WEIGHT[k] * (q0 * nxx + q1 * nxy + q2 * nyx + q3 * nyy);
+        out[k] = feq[k] + (1.0 - 1.0 / tau) * fone;
+    }
+}
+
+/* Compute_Disforce (Guo forcing term), NodeAndLattice.cpp:162-195 */
+void or_guo(double ux, double uy, double fx, double fy, double tau_begin, double out[9]) {
+    double omega = 1.0 - 0.5 * (1.0 / tau_begin);
+    for (int k = 0; k < 9; k++) {
+        double eu = (EX[k] * ux + EY[k] * uy);
+        double gx = WEIGHT[k] * omega * (3.0 * (EX[k] - ux) + 9.0 * eu * EX[k]) * fx;
+        double gy = WEIGHT[k] * omega * (3.0 * (EY[k] - uy) + 9.0 * eu * EY[k]) * fy;
+        out[k] = gx + gy;
+    }
+}
+
+/* CaculateLESTau live body (hybrid Smagorinsky), mesh.cpp:877-910 */
+double or_les_tau(const double f[9], double rho, double ux, double uy, double tau0, double cs) {
+    double rhoTemp = 1.0;
+    rhoTemp *= rho;
+    double feq[9];
+    or_feq(rho, ux, uy, feq);
+    double Sxx = 0.0, Sxy = 0.0, Syx, Syy = 0.0;
+    for (int k = 0; k < 9; k++) {
+        Sxx += EX[k] * EX[k] * (f[k] - feq[k]);
+        Sxy += EX[k] * EY[k] * (f[k] - feq[k]);
+        Syy += EY[k] * EY[k] * (f[k] - feq[k]);
+    }
+    Syx = Sxy;
+    Sxx *= -3.0 / (2 * rhoTemp * tau0);
+    Sxy *= -3.0 / (2 * rhoTemp * tau0);
+    Syx *= -3.0 / (2 * rhoTemp * tau0);
+    Syy *= -3.0 / (2 * rhoTemp * tau0);
+    double S2 = Sxx * Sxx + Sxy * Sxy + Syx * Syx + Syy * Syy;
+    double Q = sqrt(2.0 * S2);
+    double g1 = tau0 * tau0 + 18.0 * cs * cs * Q / (rhoTemp);
+    return (sqrt(g1) + tau0) / 2.0;
+}
+
+/* DiracDelta (Peskin 4-point), mesh.cpp:1354-1385 */
+double or_delta(double x) {
+    if (fabs(x) < 1) {
+        double R = 1.0 + 4.0 * fabs(x) - 4 * (x * x);
+        return (1.0 / 8.0) * (3.0 - 2.0 * fabs(x) + sqrt(R));
+    } else if (fabs(x) < 2) {
+        double R = -7.0 + 12.0 * fabs(x) - 4 * (x * x);
+        return (1.0 / 8.0) * (5.0 - 2.0 * fabs(x) - sqrt(R));
+    }
+    return 0;
+}
+
+/* Seeded uniform noise in [-1, 1): splitmix64 finaliser keyed by
+ * (seed, node, component). Identical formula in the HIP library. */
+double or_noise(uint64_t seed, int64_t node, int comp) {
+    uint64_t z = seed + (uint64_t)(2 * node + comp + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    double r = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    return 2.0 * r - 1.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* MLP relaxation-time model (6 -> 32 -> 32 -> 1, SiLU, softplus + 0.5)  */
+/* Parameter blob: W1[32][6] b1[32] W2[32][32] b2[32] W3[32] b3          */
+/*                 then feature normalisation mu[6] isig[6].             */
+/* ------------------------------------------------------------------ */
+#define NIN OR_MLP_NIN
+#define NH OR_MLP_NH
+#define OFF_W1 0
+#define OFF_B1 (OFF_W1 + NH * NIN)
+#define OFF_W2 (OFF_B1 + NH)
+#define OFF_B2 (OFF_W2 + NH * NH)
+#define OFF_W3 (OFF_B2 + NH)
+#define OFF_B3 (OFF_W3 + NH)
+#define OFF_MU (OFF_B3 + 1)
+#define OFF_IS (OFF_MU + NIN)
+
+/* Features: rho, ux, uy, |S| (the LES strain magnitude Q of mesh.cpp:886-906,
+ * built on tau0 = 1 so it is the raw |Pi_neq| scale), Pi_xx - Pi_yy, Pi_xy. */
+void or_mlp_features(const double f[9], double rho, double ux, double uy, double feat[6]) {
+    double feq[9];
+    or_feq(rho, ux, uy, feq);
+    double pxx = 0, pxy = 0, pyy = 0;
+    for (int k = 0; k < 9; k++) {
+        double d = f[k] - feq[k];
+        pxx += EX[k] * EX[k] * d;
+        pxy += EX[k] * EY[k] * d;
+        pyy += EY[k] * EY[k] * d;
+    }
+    double c = -3.0 / (2 * rho);
+    double sxx = pxx * c, sxy = pxy * c, syy = pyy * c;
+    feat[0] = rho;
+    feat[1] = ux;
+    feat[2] = uy;
+    feat[3] = sqrt(2.0 * (sxx * sxx + 2.0 * sxy * sxy + syy * syy));
+    feat[4] = pxx - pyy;
+    feat[5] = pxy;
+}
+
+static double silu(double z) { return z / (1.0 + exp(-z)); }
+static double dsilu(double z) {
+    double s = 1.0 / (1.0 + exp(-z));
+    return s * (1.0 + z * (1.0 - s));
+}
+static double softplus(double z) { return z > 30.0 ? z : log1p(exp(z)); }
+static double sigm(double z) { return 1.0 / (1.0 + exp(-z)); }
+
+static double mlp_fwd(const double* p, const double in[6], double* z1, double* h1, double* z2,
+                      double* h2, double* xo, double* o_pre) {
+    double x[NIN];
+    for (int i = 0; i < NIN; i++) x[i] = (in[i] - p[OFF_MU + i]) * p[OFF_IS + i];
+    for (int j = 0; j < NH; j++) {
+        double s = p[OFF_B1 + j];
+        for (int i = 0; i < NIN; i++) s += p[OFF_W1 + j * NIN + i] * x[i];
+        z1[j] = s;
+        h1[j] = silu(s);
+    }
+    for (int j = 0; j < NH; j++) {
+        double s = p[OFF_B2 + j];
+        for (int i = 0; i < NH; i++) s += p[OFF_W2 + j * NH + i] * h1[i];
+        z2[j] = s;
+        h2[j] = silu(s);
+    }
+    double o = p[OFF_B3];
+    for (int i = 0; i < NH; i++) o += p[OFF_W3 + i] * h2[i];
+    if (xo)
+        for (int i = 0; i < NIN; i++) xo[i] = x[i];
+    *o_pre = o;
+    return softplus(o) + 0.5;
+}
+
+double or_mlp_forward(const double* params, const double in[6]) {
+    double z1[NH], h1[NH], z2[NH], h2[NH], o;
+    return mlp_fwd(params, in, z1, h1, z2, h2, NULL, &o);
+}
+
+double or_mlp_loss_grad(const double* p, const double* xs, const double* ys, int n, double* g) {
+    memset(g, 0, sizeof(double) * OR_MLP_NPARAM);
+    double loss = 0.0;
+    for (int s = 0; s < n; s++) {
+        double z1[NH], h1[NH], z2[NH], h2[NH], x[NIN], o;
+        double t = mlp_fwd(p, xs + (size_t)s * NIN, z1, h1, z2, h2, x, &o);
+        double r = t - ys[s];
+        loss += 0.5 * r * r;
+        double dt = r / n;
+        double dout = dt * sigm(o);
+        g[OFF_B3] += dout;
+        double dz2[NH];
+        for (int i = 0; i < NH; i++) {
+            g[OFF_W3 + i] += dout * h2[i];
+            dz2[i] = dout * p[OFF_W3 + i] * dsilu(z2[i]);
+        }
+        double dh1[NH];
+        for (int i = 0; i < NH; i++) dh1[i] = 0.0;
+        for (int j = 0; j < NH; j++) {
+            g[OFF_B2 + j] += dz2[j];
+            for (int i = 0; i < NH; i++) {
+                g[OFF_W2 + j * NH + i] += dz2[j] * h1[i];
+                dh1[i] += dz2[j] * p[OFF_W2 + j * NH + i];
+            }
+        }
+        for (int j = 0; j < NH; j++) {
+            double dz1 = dh1[j] * dsilu(z1[j]);
+            g[OFF_B1 + j] += dz1;
+            for (int i = 0; i < NIN; i++) g[OFF_W1 + j * NIN + i] += dz1 * x[i];
+        }
+    }
+    return loss / n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Whole-domain solver                                                 */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double x, y;      /* position (IBParticle.position) */
+    double udx, udy;  /* desired velocity (IBParticle.vel) */
+    double uix, uiy;  /* interpolated velocity (IBParticle.vel_ref) */
+    double fx, fy;    /* marker force */
+} or_marker;
+
+struct or_sim {
+    or_config c;
+    int N;
+    double *fnew, *fcol, *rho, *ux, *uy, *tau, *tau_base, *Fx, *Fy;
+    int nbody;
+    int body_first[64], body_count[64];
+    or_marker* mk;
+    int nmk, cap_mk;
+    double* forces; /* [rec][nbody][2] */
+    int nrec, cap_rec;
+    double mlp[OR_MLP_NPARAM + 2 * OR_MLP_NIN];
+    int has_mlp;
+};
+
+static int g_threads = 0;
+void or_set_threads(int n) { g_threads = n; }
+
+or_sim* or_create(const or_config* cfg) {
+    or_sim* s = (or_sim*)calloc(1, sizeof(or_sim));
+    s->c = *cfg;
+    s->N = cfg->nx * cfg->ny;
+    size_t N = (size_t)s->N;
+    s->fnew = (double*)calloc(9 * N, sizeof(double));
+    s->fcol = (double*)calloc(9 * N, sizeof(double));
+    s->rho = (double*)calloc(N, sizeof(double));
+    s->ux = (double*)calloc(N, sizeof(double));
+    s->uy = (double*)calloc(N, sizeof(double));
+    s->tau = (double*)calloc(N, sizeof(double));
+    s->tau_base = (double*)calloc(N, sizeof(double));
+    s->Fx = (double*)calloc(N, sizeof(double));
+    s->Fy = (double*)calloc(N, sizeof(double));
+    for (size_t i = 0; i < N; i++) s->tau[i] = s->tau_base[i] = cfg->tau0;
+    return s;
+}
+
+void or_destroy(or_sim* s) {
+    if (!s) return;
+    free(s->fnew); free(s->fcol); free(s->rho); free(s->ux); free(s->uy);
+    free(s->tau); free(s->tau_base); free(s->Fx); free(s->Fy);
+    free(s->mk); free(s->forces);
+    free(s);
+}
+
+/* InitialLatMesh + InitialLatticeFeq (mesh.cpp:88-167): rho, u, f = feq.
+ * The initial velocity is U0 plus seeded uniform noise of amplitude `noise`. */
+void or_init_fields(or_sim* s, uint64_t seed, double rho0, double ux0, double uy0, double noise) {
+    int N = s->N;
+    for (int n = 0; n < N; n++) {
+        double ux = ux0 + noise * or_noise(seed, n, 0);
+        double uy = uy0 + noise * or_noise(seed, n, 1);
+        double f[9];
+        or_feq_init(rho0, ux, uy, f);
+        for (int k = 0; k < 9; k++) s->fnew[(size_t)k * N + n] = s->fcol[(size_t)k * N + n] = f[k];
+        s->rho[n] = rho0;
+        s->ux[n] = ux;
+        s->uy[n] = uy;
+    }
+    s->nrec = 0;
+}
+
+void or_set_state(or_sim* s, const double* fnew, const double* rho, const double* ux, const double* uy) {
+    size_t N = (size_t)s->N;
+    memcpy(s->fnew, fnew, 9 * N * sizeof(double));
+    memcpy(s->rho, rho, N * sizeof(double));
+    memcpy(s->ux, ux, N * sizeof(double));
+    memcpy(s->uy, uy, N * sizeof(double));
+    s->nrec = 0;
+}
+
+/* InitialIBParticle, mesh.cpp:173-191 */
+int or_add_body(or_sim* s, double cx, double cy, double diameter, int M, double udx, double udy) {
+    if (s->nbody >= 64 || M <= 0) return -1;
+    double R = 0.5 * diameter;
+    if (s->nmk + M > s->cap_mk) {
+        s->cap_mk = (s->nmk + M) * 2;
+        s->mk = (or_marker*)realloc(s->mk, sizeof(or_marker) * s->cap_mk);
+    }
+    s->body_first[s->nbody] = s->nmk;
+    s->body_count[s->nbody] = M;
+    for (int i = 0; i < M; i++) {
+        or_marker* m = &s->mk[s->nmk + i];
+        m->x = cx + R * sin(2. * M_PI * (double)i / M);
+        m->y = cy + R * cos(2. * M_PI * (double)i / M);
+        m->udx = udx;
+        m->udy = udy;
+        m->uix = m->uiy = m->fx = m->fy = 0.0;
+    }
+    s->nmk += M;
+    s->nbody++;
+    return s->nbody - 1;
+}
+
+void or_set_tau_field(or_sim* s, const double* tau) {
+    memcpy(s->tau_base, tau, sizeof(double) * s->N);
+    memcpy(s->tau, tau, sizeof(double) * s->N);
+}
+
+void or_set_mlp(or_sim* s, const double* params) {
+    memcpy(s->mlp, params, sizeof(s->mlp));
+    s->has_mlp = 1;
+}
+
+static void record_forces(or_sim* s) {
+    if (s->nrec + 1 > s->cap_rec) {
+        s->cap_rec = s->cap_rec ? s->cap_rec * 2 : 1024;
+        s->forces = (double*)realloc(s->forces, sizeof(double) * 2 * (size_t)s->cap_rec * (s->nbody ? s->nbody : 1));
+    }
+    for (int b = 0; b < s->nbody; b++) {
+        double fx = 0.0, fy = 0.0;
+        for (int m = s->body_first[b]; m < s->body_first[b] + s->body_count[b]; m++) {
+            fx += s->mk[m].fx;
+            fy += s->mk[m].fy;
+        }
+        s->forces[((size_t)s->nrec * s->nbody + b) * 2 + 0] = -fx;
+        s->forces[((size_t)s->nrec * s->nbody + b) * 2 + 1] = -fy;
+    }
+    s->nrec++;
+}
+
+/* Immersed boundary, wired before collision:
+ *   InterpolateVel (4-point delta)  mesh.cpp:555-605
+ *   CalParticleForce (DirectForce)  mesh.cpp:514-532
+ *   SpreadForce (4-point delta)     mesh.cpp:659-741
+ *   UpdateVelocity                  mesh.cpp:1244-1281 */
+static void ib_phase(or_sim* s) {
+    const int nx = s->c.nx;
+    for (int i = 0; i < s->nmk; i++) {
+        or_marker* m = &s->mk[i];
+        m->uix = 0;
+        m->uiy = 0;
+        int XI = (int)(m->x), YI = (int)(m->y);
+        for (int X = XI - 1; X <= XI + 2; X++)
+            for (int Y = YI - 1; Y <= YI + 2; Y++) {
+                double dx = m->x - X, dy = m->y - Y;
+                double D = or_delta(dx) * or_delta(dy);
+                int n = Y * nx + X;
+                m->uix += (s->ux[n] * D);
+                m->uiy += (s->uy[n] * D);
+            }
+    }
+    for (int i = 0; i < s->nmk; i++) {
+        or_marker* m = &s->mk[i];
+        m->fx = 2 * (m->udx - m->uix);
+        m->fy = 2 * (m->udy - m->uiy);
+    }
+    memset(s->Fx, 0, sizeof(double) * s->N);
+    memset(s->Fy, 0, sizeof(double) * s->N);
+    for (int i = 0; i < s->nmk; i++) {
+        or_marker* m = &s->mk[i];
+        int XI = (int)(m->x), YI = (int)(m->y);
+        for (int X = XI - 1; X <= XI + 2; X++)
+            for (int Y = YI - 1; Y <= YI + 2; Y++) {
+                double dx = m->x - X, dy = m->y - Y;
+                double D = or_delta(dx) * or_delta(dy);
+                int n = Y * nx + X;
+                s->Fx[n] += (m->fx * D);
+                s->Fy[n] += (m->fy * D);
+            }
+    }
+    for (int n = 0; n < s->N; n++) {
+        s->ux[n] = s->ux[n] + 0.5 * 1 * s->Fx[n] / s->rho[n];
+        s->uy[n] = s->uy[n] + 0.5 * 1 * s->Fy[n] / s->rho[n];
+    }
+    record_forces(s);
+}
+
+/* Collision at every node (mesh.cpp:386-438): tau from the NN guard
+ * (mesh.cpp:2821-2827, :393), the LES model or the base value, then the
+ * collision operator, plus the Guo term where IB force acts. */
+static void collide_phase(or_sim* s) {
+    const int N = s->N;
+    const or_config* c = &s->c;
+    const int ib = s->nbody > 0;
+#pragma omp parallel for schedule(static) if (N > 4096) num_threads(g_threads > 0 ? g_threads : 1)
+    for (int n = 0; n < N; n++) {
+        double f[9], out[9];
+        for (int k = 0; k < 9; k++) f[k] = s->fnew[(size_t)k * N + n];
+        double rho = s->rho[n], ux = s->ux[n], uy = s->uy[n];
+        double tb = s->tau_base[n];
+        int nn_ok = 0;
+        double tau = tb;
+        if (c->tau_mode == OR_TAU_NN && s->has_mlp) {
+            double feat[6];
+            or_mlp_features(f, rho, ux, uy, feat);
+            double t = or_mlp_forward(s->mlp, feat);
+            if (t >= 0.501) {
+                tau = t;
+                nn_ok = 1;
+            }
+        }
+        if (!nn_ok && c->les) tau = or_les_tau(f, rho, ux, uy, tb, c->cs);
+        s->tau[n] = tau;
+        if (c->collision == OR_COLL_MRT)
+            or_collide_mrt(f, rho, ux, uy, tau, out);
+        else if (c->collision == OR_COLL_RR)
+            or_collide_rr(f, rho, ux, uy, tau, out);
+        else
+            or_collide_bgk(f, rho, ux, uy, tau, out);
+        if (ib) {
+            double g[9];
+            or_guo(ux, uy, s->Fx[n], s->Fy[n], tb, g);
+            for (int k = 0; k < 9; k++) out[k] = out[k] + g[k];
+        }
+        for (int k = 0; k < 9; k++) s->fcol[(size_t)k * N + n] = out[k];
+    }
+}
+
+/* Pull streaming (mesh.cpp:192-226) with the boundary kinds:
+ *  OR_BC_PERIODIC_BB: periodic x (the reference's ring halo, mesh.cpp:467-484,
+ *     with inlet/outlet off) and halfway bounce-back walls below row 0 and
+ *     above row ny-1 (the reference declares inv[] at mesh.cpp:236 but has no
+ *     bounce-back; this kind is the channel of the plain-method workload).
+ *  OR_BC_INFLOW: columns 0 / nx-1 are non-equilibrium-extrapolation inlet /
+ *     outlet (mesh.cpp:239-286: rho = 1, inlet u = u_in, outlet u copied from
+ *     column nx-2); rows 0 / ny-1 (1 <= x <= nx-2) are specular free-slip
+ *     walls (the 'm' rule, mesh.cpp:325-337, mirrored for the top row). */
+static inline double fc(const or_sim* s, int k, int x, int y) {
+    return s->fcol[(size_t)k * s->N + (size_t)y * s->c.nx + x];
+}
+
+static void pull_node(const or_sim* s, int x, int y, double f[9]) {
+    const int nx = s->c.nx, ny = s->c.ny;
+    if (s->c.bc == OR_BC_PERIODIC_BB) {
+        for (int k = 0; k < 9; k++) {
+            int sx = x - EX[k];
+            int sy = y - EY[k];
+            if (sx < 0) sx += nx;
+            if (sx >= nx) sx -= nx;
+            if (sy < 0 || sy >= ny)
+                f[k] = fc(s, OPP[k], x, y);
+            else
+                f[k] = fc(s, k, sx, sy);
+        }
+        return;
+    }
+    /* OR_BC_INFLOW, non-NEE nodes */
+    if (y == 0) { /* mesh.cpp:327-335 */
+        f[0] = fc(s, 0, x, y);
+        f[1] = fc(s, 1, x - 1, y);
+        f[2] = fc(s, 4, x, y + 1);
+        f[3] = fc(s, 3, x + 1, y);
+        f[4] = fc(s, 4, x, y + 1);
+        f[5] = fc(s, 8, x - 1, y + 1);
+        f[6] = fc(s, 7, x + 1, y + 1);
+        f[7] = fc(s, 7, x + 1, y + 1);
+        f[8] = fc(s, 8, x - 1, y + 1);
+    } else if (y == ny - 1) { /* mirror image of the 'm' rule */
+        f[0] = fc(s, 0, x, y);
+        f[1] = fc(s, 1, x - 1, y);
+        f[2] = fc(s, 2, x, y - 1);
+        f[3] = fc(s, 3, x + 1, y);
+        f[4] = fc(s, 2, x, y - 1);
+        f[5] = fc(s, 5, x - 1, y - 1);
+        f[6] = fc(s, 6, x + 1, y - 1);
+        f[7] = fc(s, 6, x + 1, y - 1);
+        f[8] = fc(s, 5, x - 1, y - 1);
+    } else {
+        for (int k = 0; k < 9; k++) f[k] = fc(s, k, x - EX[k], y - EY[k]);
+    }
+}
+
+static void stream_phase(or_sim* s) {
+    const int nx = s->c.nx, ny = s->c.ny, N = s->N;
+    const int inflow = s->c.bc == OR_BC_INFLOW;
+#pragma omp parallel for schedule(static) if (N > 4096) num_threads(g_threads > 0 ? g_threads : 1)
+    for (int y = 0; y < ny; y++) {
+        for (int x = 0; x < nx; x++) {
+            if (inflow && (x == 0 || x == nx - 1)) continue;
+            double f[9], m[3];
+            pull_node(s, x, y, f);
+            or_macro(f, m);
+            int n = y * nx + x;
+            for (int k = 0; k < 9; k++) s->fnew[(size_t)k * N + n] = f[k];
+            s->rho[n] = m[0];
+            s->ux[n] = m[1];
+            s->uy[n] = m[2];
+        }
+    }
+    if (!inflow) return;
+    /* inlet / outlet NEE, mesh.cpp:239-286 */
+    for (int y = 0; y < ny; y++) {
+        for (int side = 0; side < 2; side++) {
+            int xb = side == 0 ? 0 : nx - 1;
+            int xn = side == 0 ? 1 : nx - 2;
+            int nb = y * nx + xb, nn = y * nx + xn;
+            double rb = 1.0, uxb, uyb;
+            if (side == 0) {
+                uxb = s->c.u_in_x;
+                uyb = s->c.u_in_y;
+            } else {
+                uxb = s->ux[nn];
+                uyb = s->uy[nn];
+            }
+            double fb[9], fq[9];
+            or_feq(rb, uxb, uyb, fb);
+            or_feq(s->rho[nn], s->ux[nn], s->uy[nn], fq);
+            for (int k = 0; k < 9; k++)
+                s->fnew[(size_t)k * N + nb] = fb[k] + s->fnew[(size_t)k * N + nn] - fq[k];
+            s->rho[nb] = rb;
+            s->ux[nb] = uxb;
+            s->uy[nb] = uyb;
+        }
+    }
+}
+
+/* One time step in the reference's OneLBMStep order (mesh.cpp:357-513),
+ * with the immersed boundary wired before the collision. */
+int or_step(or_sim* s, int nsteps) {
+    for (int t = 0; t < nsteps; t++) {
+        if (s->nbody > 0) ib_phase(s);
+        collide_phase(s);
+        stream_phase(s);
+        /* ErrorCatch, mesh.cpp:2101-2111 (NaN also caught here) */
+        for (int n = 0; n < s->N; n++) {
+            double r = s->rho[n];
+            if (!(r >= 0.0 && r <= 10.0)) return 1;
+        }
+    }
+    return 0;
+}
+
+void or_get_fields(const or_sim* s, double* rho, double* ux, double* uy) {
+    size_t N = (size_t)s->N;
+    if (rho) memcpy(rho, s->rho, N * sizeof(double));
+    if (ux) memcpy(ux, s->ux, N * sizeof(double));
+    if (uy) memcpy(uy, s->uy, N * sizeof(double));
+}
+void or_get_fnew(const or_sim* s, double* f) { memcpy(f, s->fnew, 9 * (size_t)s->N * sizeof(double)); }
+void or_get_fcol(const or_sim* s, double* f) { memcpy(f, s->fcol, 9 * (size_t)s->N * sizeof(double)); }
+void or_get_tau(const or_sim* s, double* t) { memcpy(t, s->tau, (size_t)s->N * sizeof(double)); }
+int or_num_force_records(const or_sim* s) { return s->nrec; }
+void or_get_forces(const or_sim* s, double* out) {
+    memcpy(out, s->forces, sizeof(double) * 2 * (size_t)s->nrec * s->nbody);
+}
+int or_num_markers(const or_sim* s) { return s->nmk; }
+void or_get_markers(const or_sim* s, double* out) {
+    for (int i = 0; i < s->nmk; i++) {
+        out[6 * i + 0] = s->mk[i].x;
+        out[6 * i + 1] = s->mk[i].y;
+        out[6 * i + 2] = s->mk[i].uix;
+        out[6 * i + 3] = s->mk[i].uiy;
+        out[6 * i + 4] = s->mk[i].fx;
+        out[6 * i + 5] = s->mk[i].fy;
+    }
+}
+double or_mass(const or_sim* s) {
+    double m = 0.0;
+    for (int n = 0; n < s->N; n++) m += s->rho[n];
+    return m;
+}
+void or_get_force_field(const or_sim* s, double* fx, double* fy) {
+    memcpy(fx, s->Fx, sizeof(double) * s->N);
+    memcpy(fy, s->Fy, sizeof(double) * s->N);
+}

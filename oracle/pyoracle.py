@@ -1,0 +1,257 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+always as the checker or the CPU baseline, never as the product path.
+The library is oracle/_build/libippi_oracle.so, built by `make -C oracle`.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libippi_oracle.so")
+REF_LIB_PATH = os.path.join(_HERE, "_ref", "libippi_refnode.so")
+
+COLL = {"bgk": 0, "mrt": 1, "rr": 2}
+BC = {"periodic_bb": 0, "inflow": 1}
+TAU = {"const": 0, "field": 1, "nn": 2}
+MLP_NPARAM = 6 * 32 + 32 + 32 * 32 + 32 + 32 + 1
+MLP_BLOB = MLP_NPARAM + 12
+
+
+class OrConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("collision", ctypes.c_int),
+        ("bc", ctypes.c_int), ("les", ctypes.c_int), ("tau_mode", ctypes.c_int),
+        ("tau0", ctypes.c_double), ("cs", ctypes.c_double),
+        ("u_in_x", ctypes.c_double), ("u_in_y", ctypes.c_double),
+    ]
+
+
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        L.or_feq.argtypes = [d, d, d, _D]
+        L.or_feq_init.argtypes = [d, d, d, _D]
+        L.or_macro.argtypes = [_D, _D]
+        for n in ("or_collide_bgk", "or_collide_mrt", "or_collide_rr"):
+            getattr(L, n).argtypes = [_D, d, d, d, d, _D]
+        L.or_guo.argtypes = [d, d, d, d, d, _D]
+        L.or_les_tau.argtypes = [_D, d, d, d, d, d]
+        L.or_les_tau.restype = d
+        L.or_delta.argtypes = [d]
+        L.or_delta.restype = d
+        L.or_noise.argtypes = [ctypes.c_uint64, ctypes.c_int64, i]
+        L.or_noise.restype = d
+        L.or_mlp_features.argtypes = [_D, d, d, d, _D]
+        L.or_mlp_forward.argtypes = [_D, _D]
+        L.or_mlp_forward.restype = d
+        L.or_mlp_loss_grad.argtypes = [_D, _D, _D, i, _D]
+        L.or_mlp_loss_grad.restype = d
+        L.or_create.argtypes = [ctypes.POINTER(OrConfig)]
+        L.or_create.restype = vp
+        L.or_destroy.argtypes = [vp]
+        L.or_set_threads.argtypes = [i]
+        L.or_init_fields.argtypes = [vp, ctypes.c_uint64, d, d, d, d]
+        L.or_set_state.argtypes = [vp, _D, _D, _D, _D]
+        L.or_add_body.argtypes = [vp, d, d, d, i, d, d]
+        L.or_add_body.restype = i
+        L.or_set_tau_field.argtypes = [vp, _D]
+        L.or_set_mlp.argtypes = [vp, _D]
+        L.or_step.argtypes = [vp, i]
+        L.or_step.restype = i
+        L.or_get_fields.argtypes = [vp, _D, _D, _D]
+        L.or_get_fnew.argtypes = [vp, _D]
+        L.or_get_fcol.argtypes = [vp, _D]
+        L.or_get_tau.argtypes = [vp, _D]
+        L.or_num_force_records.argtypes = [vp]
+        L.or_num_force_records.restype = i
+        L.or_get_forces.argtypes = [vp, _D]
+        L.or_num_markers.argtypes = [vp]
+        L.or_num_markers.restype = i
+        L.or_get_markers.argtypes = [vp, _D]
+        L.or_mass.argtypes = [vp]
+        L.or_mass.restype = d
+        L.or_get_force_field.argtypes = [vp, _D, _D]
+        _lib = L
+    return _lib
+
+
+# ---- per-node functions -------------------------------------------------
+def feq(rho, ux, uy):
+    out = np.zeros(9)
+    lib().or_feq(rho, ux, uy, _p(out))
+    return out
+
+
+def feq_init(rho, ux, uy):
+    out = np.zeros(9)
+    lib().or_feq_init(rho, ux, uy, _p(out))
+    return out
+
+
+def macro(f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(3)
+    lib().or_macro(_p(f), _p(out))
+    return out
+
+
+def collide(kind, f, rho, ux, uy, tau):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(9)
+    getattr(lib(), "or_collide_" + kind)(_p(f), rho, ux, uy, tau, _p(out))
+    return out
+
+
+def guo(ux, uy, fx, fy, tau_begin):
+    out = np.zeros(9)
+    lib().or_guo(ux, uy, fx, fy, tau_begin, _p(out))
+    return out
+
+
+def les_tau(f, rho, ux, uy, tau0, cs):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().or_les_tau(_p(f), rho, ux, uy, tau0, cs)
+
+
+def delta(r):
+    return lib().or_delta(r)
+
+
+def noise(seed, node, comp):
+    return lib().or_noise(seed, node, comp)
+
+
+def mlp_features(f, rho, ux, uy):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(6)
+    lib().or_mlp_features(_p(f), rho, ux, uy, _p(out))
+    return out
+
+
+def mlp_forward(params, x):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().or_mlp_forward(_p(params), _p(x))
+
+
+def mlp_loss_grad(params, xs, ys):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    g = np.zeros(MLP_NPARAM)
+    loss = lib().or_mlp_loss_grad(_p(params), _p(xs), _p(ys), len(ys), _p(g))
+    return loss, g
+
+
+# ---- whole-domain solver ------------------------------------------------
+class Sim:
+    """FP64 CPU solver; same configuration vocabulary as the HIP library."""
+
+    def __init__(self, nx, ny, collision="bgk", bc="periodic_bb", les=False, tau_mode="const",
+                 tau0=0.6, cs=0.0, u_in=(0.0, 0.0), threads=1):
+        self.nx, self.ny = nx, ny
+        self.N = nx * ny
+        cfg = OrConfig(nx, ny, COLL[collision], BC[bc], int(bool(les)), TAU[tau_mode],
+                       tau0, cs, u_in[0], u_in[1])
+        lib().or_set_threads(int(threads))
+        self.h = lib().or_create(ctypes.byref(cfg))
+
+    def close(self):
+        if self.h:
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_fields(self, seed, rho0=1.0, u0=(0.0, 0.0), noise_amp=0.0):
+        lib().or_init_fields(self.h, seed, rho0, u0[0], u0[1], noise_amp)
+
+    def set_state(self, fnew, rho, ux, uy):
+        a = [np.ascontiguousarray(x, dtype=np.float64).ravel() for x in (fnew, rho, ux, uy)]
+        lib().or_set_state(self.h, *[_p(x) for x in a])
+
+    def add_body(self, cx, cy, diameter, nmarkers, ud=(0.0, 0.0)):
+        b = lib().or_add_body(self.h, cx, cy, diameter, nmarkers, ud[0], ud[1])
+        if b < 0:
+            raise ValueError("oracle rejected body")
+        self._nbodies = b + 1
+        return b
+
+    def set_tau_field(self, tau):
+        t = np.ascontiguousarray(tau, dtype=np.float64).ravel()
+        lib().or_set_tau_field(self.h, _p(t))
+
+    def set_mlp(self, blob):
+        b = np.ascontiguousarray(blob, dtype=np.float64)
+        assert b.size == MLP_BLOB
+        lib().or_set_mlp(self.h, _p(b))
+
+    def step(self, n):
+        return lib().or_step(self.h, n)
+
+    def fields(self):
+        r, ux, uy = (np.zeros(self.N) for _ in range(3))
+        lib().or_get_fields(self.h, _p(r), _p(ux), _p(uy))
+        s = (self.ny, self.nx)
+        return r.reshape(s), ux.reshape(s), uy.reshape(s)
+
+    def fnew(self):
+        f = np.zeros(9 * self.N)
+        lib().or_get_fnew(self.h, _p(f))
+        return f.reshape(9, self.ny, self.nx)
+
+    def fcol(self):
+        f = np.zeros(9 * self.N)
+        lib().or_get_fcol(self.h, _p(f))
+        return f.reshape(9, self.ny, self.nx)
+
+    def tau(self):
+        t = np.zeros(self.N)
+        lib().or_get_tau(self.h, _p(t))
+        return t.reshape(self.ny, self.nx)
+
+    def forces(self):
+        n = lib().or_num_force_records(self.h)
+        nm = self.num_bodies()
+        out = np.zeros(max(1, 2 * n * nm))
+        if n and nm:
+            lib().or_get_forces(self.h, _p(out))
+        return out[: 2 * n * nm].reshape(n, nm, 2)
+
+    def num_bodies(self):
+        return getattr(self, "_nbodies", 0)
+
+    def markers(self):
+        n = lib().or_num_markers(self.h)
+        out = np.zeros(max(1, 6 * n))
+        if n:
+            lib().or_get_markers(self.h, _p(out))
+        return out[: 6 * n].reshape(n, 6)
+
+    def force_field(self):
+        fx, fy = np.zeros(self.N), np.zeros(self.N)
+        lib().or_get_force_field(self.h, _p(fx), _p(fy))
+        s = (self.ny, self.nx)
+        return fx.reshape(s), fy.reshape(s)
+
+    def mass(self):
+        return lib().or_mass(self.h)
