@@ -1,0 +1,132 @@
+/*
+ * ippi_lbm.h -- C ABI of the MI355X (gfx950) D2Q9 time step of IPPI-LBM.
+ *
+ * The library (libippi_lbm.so) owns the whole domain on one GPU: SoA
+ * populations f[k][y][x] in HBM (A/B buffers), immersed-boundary markers,
+ * optional per-node relaxation-time field and the fused relaxation-time
+ * MLP. Every call returns an int status (IPPI_OK = 0); no C++ exception
+ * crosses this boundary. A handle is not thread-safe; all work of a handle
+ * is ordered on one HIP stream. ippi_step() only enqueues work: errors that
+ * happen on the device (divergence) are reported by the next synchronising
+ * call (ippi_get_*, ippi_mass, ippi_synchronize).
+ *
+ * Host arrays are copied, never retained. Field arrays are row-major
+ * [y][x] with nx*ny doubles; population arrays are [k][y][x] with
+ * 9*nx*ny doubles, k in the reference's velocity order
+ * (IPPI-LBM/NodeAndLattice.cpp:3-13): (0,0),(1,0),(0,1),(-1,0),(0,-1),
+ * (1,1),(-1,1),(-1,-1),(1,-1).
+ *
+ * Reference interface each entry point replaces (IPPI-LBM/ paths):
+ *   ippi_create        Mesh::InitialLatMesh          mesh.h:56, mesh.cpp:60-171
+ *                      + JudgeStyle boundary kinds   mesh.cpp:1391-1454
+ *   ippi_init_fields   InitialLatMesh's rho/u/feq    mesh.cpp:88-167
+ *   ippi_set_fields    (direct writes of LatMesh[i][j].rho/u + InitialLatticeFeq)
+ *   ippi_add_body      Mesh::InitialIBParticle       mesh.h:57, mesh.cpp:173-191
+ *   ippi_set_tau_field (writes of LatMesh[i][j].tau / TauAtBeginning, mesh.cpp:90-91)
+ *   ippi_set_mlp       NNmodel_Type00..07 model load mesh.cpp:2767-3260
+ *   ippi_step          Mesh::OneLBMStep              mesh.h:60, mesh.cpp:357-513
+ *                      (+ InterpolateVel / CalParticleForce / SpreadForce /
+ *                       UpdateVelocity, mesh.cpp:514-832,1244-1281)
+ *   ippi_get_fields    reads of LatMesh[i][j].rho/u  (WriteDatFluid, mesh.cpp:1180-1243)
+ *   ippi_get_body_forces  OutputCd                   mesh.cpp:1461-1526
+ *   ippi_mass          ErrorCatch / global checks    mesh.cpp:2101-2111
+ *   ippi_destroy       Mesh::~Mesh                   mesh.cpp:52-55
+ */
+#ifndef IPPI_LBM_H
+#define IPPI_LBM_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPPI_OK 0
+#define IPPI_ERR_ARG 1       /* invalid argument or configuration */
+#define IPPI_ERR_HIP 2       /* HIP runtime error (message via ippi_last_error) */
+#define IPPI_ERR_DIVERGED 3  /* a node's density left [0, 10] or became NaN (ErrorCatch) */
+#define IPPI_ERR_STATE 4     /* call not valid in the handle's current state */
+#define IPPI_ERR_NOMEM 5     /* device allocation failed */
+
+enum ippi_collision { IPPI_COLL_BGK = 0, IPPI_COLL_MRT = 1, IPPI_COLL_RR = 2 };
+enum ippi_precision { IPPI_FP64 = 0, IPPI_FP32_SHIFTED = 1 };
+enum ippi_bc { IPPI_BC_PERIODIC_BB = 0, IPPI_BC_INFLOW = 1 };
+enum ippi_tau_mode { IPPI_TAU_CONST = 0, IPPI_TAU_FIELD = 1, IPPI_TAU_NN = 2 };
+
+typedef struct ippi_config {
+    int nx, ny;          /* whole domain (no ghost columns) */
+    int collision;       /* ippi_collision */
+    int precision;       /* ippi_precision: FP64 populations, or FP32 f - w */
+    int bc;              /* ippi_bc */
+    int les;             /* 1: hybrid Smagorinsky tau (mesh.cpp:877-910) */
+    int tau_mode;        /* ippi_tau_mode */
+    double tau0;         /* startTau = TauAtBeginning */
+    double cs;           /* Smagorinsky constant Cs */
+    double u_in_x;       /* inlet velocity for IPPI_BC_INFLOW */
+    double u_in_y;
+    int device;          /* HIP device ordinal */
+    int force_history;   /* body-force records kept (ring); 0 -> 65536 */
+} ippi_config;
+
+typedef struct ippi_handle ippi_handle;
+
+int ippi_create(const ippi_config* cfg, ippi_handle** out);
+int ippi_destroy(ippi_handle* h);
+const char* ippi_last_error(const ippi_handle* h);
+int ippi_version(void);
+
+/* rho = rho0, u = (ux0, uy0) + noise * U(-1,1) per node and component
+ * (splitmix64 keyed by seed, node, component), f = feq(rho, u). */
+int ippi_init_fields(ippi_handle* h, uint64_t seed, double rho0, double ux0, double uy0, double noise);
+/* rho/u given per node ([y][x]); f = feq(rho, u). */
+int ippi_set_fields(ippi_handle* h, const double* rho, const double* ux, const double* uy);
+
+/* A circular body of nmarkers Lagrangian markers at x = cx + R sin(2 pi i/M),
+ * y = cy + R cos(2 pi i/M), R = diameter/2, desired velocity (udx, udy).
+ * The 4-point support of every marker must lie in the interior
+ * (1 <= x <= nx-2, 1 <= y <= ny-2). Returns the body index in *body. */
+int ippi_add_body(ippi_handle* h, double cx, double cy, double diameter, int nmarkers,
+                  double udx, double udy, int* body);
+int ippi_num_bodies(const ippi_handle* h, int* n);
+
+/* Per-node base relaxation time (IPPI_TAU_FIELD), nx*ny doubles. */
+int ippi_set_tau_field(ippi_handle* h, const double* tau);
+
+/* Relaxation-time MLP 6 -> 32 -> 32 -> 1 (IPPI_TAU_NN); blob layout:
+ * W1[32][6] b1[32] W2[32][32] b2[32] W3[32] b3 mu[6] isig[6] (n = 1325). */
+int ippi_set_mlp(ippi_handle* h, const double* blob, int n);
+
+/* Advance nsteps time steps (asynchronous). */
+int ippi_step(ippi_handle* h, int nsteps);
+int ippi_synchronize(ippi_handle* h);
+int ippi_steps_done(const ippi_handle* h, int64_t* steps);
+
+/* Macroscopic fields after the last completed step (synchronising). */
+int ippi_get_fields(ippi_handle* h, double* rho, double* ux, double* uy);
+/* Post-streaming populations f (FP32 path: w + stored deviation), 9*nx*ny. */
+int ippi_get_populations(ippi_handle* h, double* f);
+/* Per-step force on each body (minus the sum of its marker forces, FP64),
+ * records oldest first: out[rec][body][2]; *nrec <= max_records. */
+int ippi_get_body_forces(ippi_handle* h, double* out, int max_records, int* nrec);
+/* Marker state of the last step: out[m][6] = x, y, Ux_interp, Uy_interp, Fx, Fy. */
+int ippi_get_markers(ippi_handle* h, double* out, int max_markers, int* n);
+/* Global mass sum(rho) in FP64, fixed-order reduction. */
+int ippi_mass(ippi_handle* h, double* mass);
+
+/* Kernel timing (no counterpart in the reference: measurement only). While
+ * enabled, every fused stream-collide launch and every immersed-boundary
+ * kernel pair is bracketed by HIP events on the handle's stream;
+ * ippi_get_timing synchronises, returns the summed durations (ms) and
+ * launch counts since the last call, and clears them. */
+int ippi_set_timing(ippi_handle* h, int enable);
+int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, double* ib_ms,
+                    int64_t* ib_launches);
+
+/* Checkpoint: the full device state as a flat byte image. *size gives the
+ * required size when buf is NULL. */
+int ippi_save(ippi_handle* h, void* buf, size_t* size);
+int ippi_load(ippi_handle* h, const void* buf, size_t size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,236 @@
+"""MI355X-native D2Q9 time step of IPPI-LBM -- Python binding of libippi_lbm.so.
+
+The compute path is the HIP library built from csrc/ (gfx950); this module
+only marshals host arrays across the C ABI declared in include/ippi_lbm.h.
+There is no CPU fallback: if the library is missing, or no GPU is visible,
+constructing a Solver raises.
+
+Reference counterpart: the `Mesh` object of IPPI-LBM (mesh.h:33-115), driven
+by main.cpp:52-120; see facade/ for the C++ `Mesh` mirror over the same ABI.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libippi_lbm.so")
+
+IPPI_OK, IPPI_ERR_ARG, IPPI_ERR_HIP, IPPI_ERR_DIVERGED, IPPI_ERR_STATE, IPPI_ERR_NOMEM = range(6)
+COLLISION = {"bgk": 0, "mrt": 1, "rr": 2}
+PRECISION = {"fp64": 0, "fp32": 1}
+BC = {"periodic_bb": 0, "inflow": 1}
+TAU_MODE = {"const": 0, "field": 1, "nn": 2}
+MLP_BLOB = 6 * 32 + 32 + 32 * 32 + 32 + 32 + 1 + 12
+
+
+class IppiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ippi error {code}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    """Mirror of struct ippi_config (include/ippi_lbm.h)."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("collision", ctypes.c_int), ("precision", ctypes.c_int), ("bc", ctypes.c_int),
+        ("les", ctypes.c_int), ("tau_mode", ctypes.c_int),
+        ("tau0", ctypes.c_double), ("cs", ctypes.c_double),
+        ("u_in_x", ctypes.c_double), ("u_in_y", ctypes.c_double),
+        ("device", ctypes.c_int), ("force_history", ctypes.c_int),
+    ]
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+# every entry point of include/ippi_lbm.h with its ctypes signature
+_SIGS = {
+    "ippi_version": ([], ctypes.c_int),
+    "ippi_create": ([ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "ippi_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "ippi_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
+    "ippi_init_fields": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                          ctypes.c_double], ctypes.c_int),
+    "ippi_set_fields": ([ctypes.c_void_p, _D, _D, _D], ctypes.c_int),
+    "ippi_add_body": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "ippi_num_bodies": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "ippi_set_tau_field": ([ctypes.c_void_p, _D], ctypes.c_int),
+    "ippi_set_mlp": ([ctypes.c_void_p, _D, ctypes.c_int], ctypes.c_int),
+    "ippi_step": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "ippi_synchronize": ([ctypes.c_void_p], ctypes.c_int),
+    "ippi_steps_done": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "ippi_get_fields": ([ctypes.c_void_p, _D, _D, _D], ctypes.c_int),
+    "ippi_get_populations": ([ctypes.c_void_p, _D], ctypes.c_int),
+    "ippi_get_body_forces": ([ctypes.c_void_p, _D, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "ippi_get_markers": ([ctypes.c_void_p, _D, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "ippi_mass": ([ctypes.c_void_p, _D], ctypes.c_int),
+    "ippi_set_timing": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "ippi_get_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64), _D, ctypes.POINTER(ctypes.c_int64)],
+                        ctypes.c_int),
+    "ippi_save": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "ippi_load": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
+}
+
+
+def library():
+    """Load libippi_lbm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run build_lib.py)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+class Solver:
+    """One whole-domain D2Q9 solver on one GPU (a handle of the C ABI)."""
+
+    def __init__(self, nx, ny, collision="bgk", precision="fp64", bc="periodic_bb", les=False,
+                 tau_mode="const", tau0=0.6, cs=0.0, u_in=(0.0, 0.0), device=0, force_history=0):
+        L = library()
+        self.nx, self.ny, self.N = int(nx), int(ny), int(nx) * int(ny)
+        self.precision = precision
+        cfg = Config(self.nx, self.ny, COLLISION[collision], PRECISION[precision], BC[bc], int(bool(les)),
+                     TAU_MODE[tau_mode], float(tau0), float(cs), float(u_in[0]), float(u_in[1]), int(device),
+                     int(force_history))
+        h = ctypes.c_void_p()
+        rc = L.ippi_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != IPPI_OK:
+            raise IppiError(rc, "ippi_create failed (bad configuration or no usable GPU)")
+        self.h = h
+        self.nbodies = 0
+
+    # -- helpers --------------------------------------------------------------
+    def _check(self, rc):
+        if rc != IPPI_OK:
+            msg = library().ippi_last_error(self.h)
+            raise IppiError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().ippi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state ---------------------------------------------------------------
+    def init_fields(self, seed, rho0=1.0, u0=(0.0, 0.0), noise_amp=0.0):
+        self._check(library().ippi_init_fields(self.h, int(seed), float(rho0), float(u0[0]), float(u0[1]),
+                                               float(noise_amp)))
+
+    def set_fields(self, rho, ux, uy):
+        a = [_f64(v, self.N) for v in (rho, ux, uy)]
+        self._check(library().ippi_set_fields(self.h, *[_p(v) for v in a]))
+
+    def add_body(self, cx, cy, diameter, nmarkers, ud=(0.0, 0.0)):
+        b = ctypes.c_int()
+        self._check(library().ippi_add_body(self.h, float(cx), float(cy), float(diameter), int(nmarkers),
+                                            float(ud[0]), float(ud[1]), ctypes.byref(b)))
+        self.nbodies = b.value + 1
+        return b.value
+
+    def set_tau_field(self, tau):
+        t = _f64(tau, self.N)
+        self._check(library().ippi_set_tau_field(self.h, _p(t)))
+
+    def set_mlp(self, blob):
+        b = _f64(blob, MLP_BLOB)
+        self._check(library().ippi_set_mlp(self.h, _p(b), b.size))
+
+    def step(self, n=1):
+        self._check(library().ippi_step(self.h, int(n)))
+
+    def synchronize(self):
+        self._check(library().ippi_synchronize(self.h))
+
+    def steps_done(self):
+        s = ctypes.c_int64()
+        self._check(library().ippi_steps_done(self.h, ctypes.byref(s)))
+        return s.value
+
+    # -- readback -------------------------------------------------------------
+    def fields(self):
+        r, ux, uy = (np.empty(self.N) for _ in range(3))
+        self._check(library().ippi_get_fields(self.h, _p(r), _p(ux), _p(uy)))
+        s = (self.ny, self.nx)
+        return r.reshape(s), ux.reshape(s), uy.reshape(s)
+
+    def populations(self):
+        f = np.empty(9 * self.N)
+        self._check(library().ippi_get_populations(self.h, _p(f)))
+        return f.reshape(9, self.ny, self.nx)
+
+    def body_forces(self, max_records=1 << 30):
+        n = ctypes.c_int()
+        self._check(library().ippi_get_body_forces(self.h, None, int(max_records), ctypes.byref(n)))
+        out = np.empty(max(1, 2 * n.value * self.nbodies))
+        self._check(library().ippi_get_body_forces(self.h, _p(out), n.value, ctypes.byref(n)))
+        return out[: 2 * n.value * self.nbodies].reshape(n.value, self.nbodies, 2)
+
+    def markers(self):
+        n = ctypes.c_int()
+        self._check(library().ippi_get_markers(self.h, None, 1 << 30, ctypes.byref(n)))
+        out = np.empty(max(1, 6 * n.value))
+        self._check(library().ippi_get_markers(self.h, _p(out), n.value, ctypes.byref(n)))
+        return out[: 6 * n.value].reshape(n.value, 6)
+
+    def mass(self):
+        m = ctypes.c_double()
+        self._check(library().ippi_mass(self.h, ctypes.byref(m)))
+        return m.value
+
+    def set_timing(self, enable):
+        self._check(library().ippi_set_timing(self.h, int(bool(enable))))
+
+    def timing(self):
+        """(fused-kernel ms, launches, IB-kernels ms, IB launches) since the last call."""
+        sm, im = ctypes.c_double(), ctypes.c_double()
+        sc, ic = ctypes.c_int64(), ctypes.c_int64()
+        self._check(library().ippi_get_timing(self.h, ctypes.byref(sm), ctypes.byref(sc), ctypes.byref(im),
+                                              ctypes.byref(ic)))
+        return sm.value, sc.value, im.value, ic.value
+
+    def save(self):
+        n = ctypes.c_size_t()
+        self._check(library().ippi_save(self.h, None, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(library().ippi_save(self.h, buf, ctypes.byref(n)))
+        return buf.raw[: n.value]
+
+    def load(self, blob):
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        self._check(library().ippi_load(self.h, buf, len(blob)))
+
+
+def drag_coefficient(force_x, u_ref, radius):
+    """C = F / (0.5 U^2 2R), the normalisation of OutputCd (mesh.cpp:1501-1502)."""
+    return force_x / (0.5 * u_ref * u_ref * 2 * radius)

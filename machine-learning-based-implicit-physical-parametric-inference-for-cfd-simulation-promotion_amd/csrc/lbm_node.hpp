@@ -1,0 +1,394 @@
+// Per-node D2Q9 arithmetic for the gfx950 stream-collide kernels.
+//
+// Two families:
+//   ref::   FP64 on full populations f, in the evaluation order of the
+//           reference's Lattice methods so the FP64 GPU path reproduces the
+//           reference arithmetic bit for bit (the translation unit turns FP
+//           contraction off for these functions).
+//   sh::    FP32 on shifted populations g = f - w (w = lattice weights), the
+//           bandwidth path. Same physics, FMA-friendly forms.
+//
+// Velocity order (IPPI-LBM/NodeAndLattice.cpp:3-13):
+//   k : 0  1  2  3  4  5  6  7  8
+//   ex: 0  1  0 -1  0  1 -1 -1  1
+//   ey: 0  0  1  0 -1  1  1 -1 -1
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define IPPI_HD __host__ __device__ __forceinline__
+
+namespace lbm {
+
+constexpr int EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int EY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+constexpr double WD[9] = {4. / 9., 1. / 9., 1. / 9., 1. / 9., 1. / 9., 1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+constexpr float WF[9] = {4.f / 9.f, 1.f / 9.f, 1.f / 9.f, 1.f / 9.f, 1.f / 9.f,
+                         1.f / 36.f, 1.f / 36.f, 1.f / 36.f, 1.f / 36.f};
+// MRT transform (NodeAndLattice.cpp:18-27) and 36 x its inverse (:28-29)
+constexpr double TM[9][9] = {
+    {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0},
+    {-4.0, -1.0, -1.0, -1.0, -1.0, 2.0, 2.0, 2.0, 2.0},
+    {4.0, -2.0, -2.0, -2.0, -2.0, 1.0, 1.0, 1.0, 1.0},
+    {0.0, 1.0, 0.0, -1.0, 0.0, 1.0, -1.0, -1.0, 1.0},
+    {0.0, -2.0, 0.0, 2.0, 0.0, 1.0, -1.0, -1.0, 1.0},
+    {0.0, 0.0, 1.0, 0.0, -1.0, 1.0, 1.0, -1.0, -1.0},
+    {0.0, 0.0, -2.0, 0.0, 2.0, 1.0, 1.0, -1.0, -1.0},
+    {0.0, 1.0, -1.0, 1.0, -1.0, 0.0, 0.0, 0.0, 0.0},
+    {0.0, 0.0, 0.0, 0.0, 0.0, 1.0, -1.0, 1.0, -1.0}};
+constexpr int TMINV_K[9][9] = {
+    {4, -4, 4, 0, 0, 0, 0, 0, 0},    {4, -1, -2, 6, -6, 0, 0, 9, 0},
+    {4, -1, -2, 0, 0, 6, -6, -9, 0}, {4, -1, -2, -6, 6, 0, 0, 9, 0},
+    {4, -1, -2, 0, 0, -6, 6, -9, 0}, {4, 2, 1, 6, 3, 6, 3, 0, 9},
+    {4, 2, 1, -6, -3, 6, 3, 0, -9},  {4, 2, 1, -6, -3, -6, -3, 0, 9},
+    {4, 2, 1, 6, 3, -6, -3, 0, -9}};
+// fixed MRT rates of the non-hydrodynamic moments (NodeAndLattice.cpp:306);
+// entries 7 and 8 are 1/tau at run time, 0, 3, 5 are conserved.
+constexpr double SM_FIXED[9] = {0.0, 1.2, 1.1, 0.0, 1.2, 0.0, 1.2, -1.0, -1.0};
+
+// Peskin 4-point kernel, mesh.cpp:1354-1385 (host: marker weights; device: adjoint)
+IPPI_HD double delta4(double x) {
+#pragma clang fp contract(off)
+    double ax = x < 0 ? -x : x;
+    if (ax < 1) {
+        double R = 1.0 + 4.0 * ax - 4 * (x * x);
+        return (1.0 / 8.0) * (3.0 - 2.0 * ax + sqrt(R));
+    } else if (ax < 2) {
+        double R = -7.0 + 12.0 * ax - 4 * (x * x);
+        return (1.0 / 8.0) * (5.0 - 2.0 * ax - sqrt(R));
+    }
+    return 0;
+}
+
+IPPI_HD double noise(unsigned long long seed, long long node, int comp) {
+    unsigned long long z = seed + (unsigned long long)(2 * node + comp + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    double r = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    return 2.0 * r - 1.0;
+}
+
+// ------------------------------------------------------------------ FP64
+namespace ref {
+
+// Compute_feq, NodeAndLattice.cpp:124-149
+IPPI_HD void feq(double rho, double ux, double uy, double* q) {
+#pragma clang fp contract(off)
+    double w0 = rho * (4.0 / 9.0);
+    double w1 = rho / 9.0;
+    double w2 = rho / 36.0;
+    double uu = 1 - 1.5 * (ux * ux + uy * uy);
+    double s = ux + uy, d = ux - uy;
+    q[0] = w0 * (uu);
+    q[1] = w1 * (uu + 3.0 * (ux) + 4.5 * (ux) * (ux));
+    q[2] = w1 * (uu + 3.0 * (uy) + 4.5 * (uy) * (uy));
+    q[3] = w1 * (uu - 3.0 * (ux) + 4.5 * (ux) * (ux));
+    q[4] = w1 * (uu - 3.0 * (uy) + 4.5 * (uy) * (uy));
+    q[5] = w2 * (uu + 3.0 * s + 4.5 * s * s);
+    q[6] = w2 * (uu - 3.0 * d + 4.5 * d * d);
+    q[7] = w2 * (uu - 3.0 * s + 4.5 * s * s);
+    q[8] = w2 * (uu + 3.0 * d + 4.5 * d * d);
+}
+
+// InitialLatticeFeq, NodeAndLattice.cpp:104-119 (w0 = rho*4.0/9.0)
+IPPI_HD void feq_init(double rho, double ux, double uy, double* q) {
+#pragma clang fp contract(off)
+    double w0 = rho * 4.0 / 9.0;
+    double w1 = rho / 9.0;
+    double w2 = rho / 36.0;
+    double uu = 1 - 1.5 * (ux * ux + uy * uy);
+    double s = ux + uy, d = ux - uy;
+    q[0] = w0 * (uu);
+    q[1] = w1 * (uu + 3.0 * (ux) + 4.5 * (ux) * (ux));
+    q[2] = w1 * (uu + 3.0 * (uy) + 4.5 * (uy) * (uy));
+    q[3] = w1 * (uu - 3.0 * (ux) + 4.5 * (ux) * (ux));
+    q[4] = w1 * (uu - 3.0 * (uy) + 4.5 * (uy) * (uy));
+    q[5] = w2 * (uu + 3.0 * s + 4.5 * s * s);
+    q[6] = w2 * (uu - 3.0 * d + 4.5 * d * d);
+    q[7] = w2 * (uu - 3.0 * s + 4.5 * s * s);
+    q[8] = w2 * (uu + 3.0 * d + 4.5 * d * d);
+}
+
+// CalMacro, NodeAndLattice.cpp:196-226. The reference loops over all nine
+// e components; terms with e = 0 add a signed zero, so the sums below are
+// the same values in the same order.
+IPPI_HD void macro(const double* f, double& rho, double& ux, double& uy) {
+#pragma clang fp contract(off)
+    double r = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r += f[k];
+    double tx = ((((f[1] - f[3]) + f[5]) - f[6]) - f[7]) + f[8];
+    double ty = ((((f[2] - f[4]) + f[5]) + f[6]) - f[7]) - f[8];
+    if (r < 1.e-6) {
+        tx = 0.0;
+        ty = 0.0;
+    } else {
+        tx /= r;
+        ty /= r;
+    }
+    rho = r;
+    ux = tx;
+    uy = ty;
+}
+
+// ComputeBGKCollision, NodeAndLattice.cpp:150-161
+IPPI_HD void bgk(const double* f, double rho, double ux, double uy, double tau, double* o) {
+#pragma clang fp contract(off)
+    double q[9];
+    feq(rho, ux, uy, q);
+    double a = (1.0 - 1.0 / tau);
+#pragma unroll
+    for (int k = 0; k < 9; k++) o[k] = a * f[k] + q[k] / tau;
+}
+
+// ComputeMRTCollision, NodeAndLattice.cpp:290-360. Zero coefficients are
+// skipped: each skipped term is a signed zero added to a running sum.
+IPPI_HD void mrt(const double* f, double rho, double ux, double uy, double tau, double* o) {
+#pragma clang fp contract(off)
+    const double a1 = 1.0 / 36.0;
+    double s78 = 1. / tau;
+    double t1 = ux * ux + uy * uy;
+    double t2 = ux * ux - uy * uy;
+    double meq[9];
+    meq[0] = rho;
+    meq[1] = rho * (-2.0 + 3.0 * t1);
+    meq[2] = rho * (1.0 - 3.0 * t1);
+    meq[3] = rho * ux;
+    meq[4] = -rho * ux;
+    meq[5] = rho * uy;
+    meq[6] = -rho * uy;
+    meq[7] = rho * t2;
+    meq[8] = rho * ux * uy;
+    double dm[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        double s = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int l = 0; l < 9; l++) {
+            if (TM[k][l] != 0.0) {
+                double t = TM[k][l] * f[l];
+                s = first ? t : s + t;
+                first = false;
+            }
+        }
+        dm[k] = s - meq[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        double s = 0.;
+        bool first = true;
+#pragma unroll
+        for (int l = 0; l < 9; l++) {
+            if (TMINV_K[k][l] != 0 && SM_FIXED[l] != 0.0) {
+                double sl = SM_FIXED[l] < 0 ? s78 : SM_FIXED[l];
+                double t = (((double)TMINV_K[k][l] * a1) * sl) * dm[l];
+                s = first ? t : s + t;
+                first = false;
+            }
+        }
+        o[k] = f[k] - s;
+    }
+}
+
+// ComputeRRCollision, NodeAndLattice.cpp:228-286
+IPPI_HD void rr(const double* f, double rho, double ux, double uy, double tau, double* o) {
+#pragma clang fp contract(off)
+    double q[9];
+    feq(rho, ux, uy, q);
+    // second moments of f and feq, reference loop order with e = 0 terms skipped
+    double xx = ((((f[1] + f[3]) + f[5]) + f[6]) + f[7]) + f[8];
+    double txx = ((((q[1] + q[3]) + q[5]) + q[6]) + q[7]) + q[8];
+    double xy = ((f[5] - f[6]) + f[7]) - f[8];
+    double txy = ((q[5] - q[6]) + q[7]) - q[8];
+    double yy = ((((f[2] + f[4]) + f[5]) + f[6]) + f[7]) + f[8];
+    double tyy = ((((q[2] + q[4]) + q[5]) + q[6]) + q[7]) + q[8];
+    double nxx = xx - txx, nxy = xy - txy, nyy = yy - tyy;
+    double a = (1.0 - 1.0 / tau);
+    const double third = 1. / 3;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        double q0 = EX[k] * EX[k] - third;
+        double q3 = EY[k] * EY[k] - third;
+        double s = q0 * nxx;
+        if (EX[k] * EY[k] != 0) {
+            double q1 = EX[k] * EY[k];
+            s = s + q1 * nxy;
+            s = s + q1 * nxy;
+        }
+        s = s + q3 * nyy;
+        double fone = 4.5 * WD[k] * s;
+        o[k] = q[k] + a * fone;
+    }
+}
+
+// Compute_Disforce (Guo term), NodeAndLattice.cpp:162-195
+IPPI_HD void guo(double ux, double uy, double fx, double fy, double tau_begin, double* o) {
+#pragma clang fp contract(off)
+    double omega = 1.0 - 0.5 * (1.0 / tau_begin);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        double ex = EX[k], ey = EY[k];
+        double eu = (ex * ux + ey * uy);
+        double gx = WD[k] * omega * (3.0 * (ex - ux) + 9.0 * eu * ex) * fx;
+        double gy = WD[k] * omega * (3.0 * (ey - uy) + 9.0 * eu * ey) * fy;
+        o[k] = gx + gy;
+    }
+}
+
+// CaculateLESTau live body, mesh.cpp:877-910
+IPPI_HD double les_tau(const double* f, double rho, double ux, double uy, double tau0, double cs) {
+#pragma clang fp contract(off)
+    double q[9];
+    feq(rho, ux, uy, q);
+    double d[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = f[k] - q[k];
+    double Sxx = ((((d[1] + d[3]) + d[5]) + d[6]) + d[7]) + d[8];
+    double Sxy = ((d[5] - d[6]) + d[7]) - d[8];
+    double Syy = ((((d[2] + d[4]) + d[5]) + d[6]) + d[7]) + d[8];
+    double c = -3.0 / (2 * rho * tau0);
+    Sxx *= c;
+    Sxy *= c;
+    double Syx = Sxy;
+    Syy *= c;
+    double S2 = Sxx * Sxx + Sxy * Sxy + Syx * Syx + Syy * Syy;
+    double Q = sqrt(2.0 * S2);
+    double g1 = tau0 * tau0 + 18.0 * cs * cs * Q / (rho);
+    return (sqrt(g1) + tau0) / 2.0;
+}
+
+}  // namespace ref
+
+// ------------------------------------------------------------------ FP32
+// g_k = f_k - w_k; rho = 1 + sum g; rho u = sum g e (sum w e = 0).
+namespace sh {
+
+IPPI_HD void macro(const float* g, float& drho, float& rho, float& ux, float& uy) {
+    float a = (g[1] + g[3]) + (g[2] + g[4]);
+    float b = (g[5] + g[7]) + (g[6] + g[8]);
+    drho = g[0] + (a + b);
+    float jx = (g[1] - g[3]) + ((g[5] - g[7]) + (g[8] - g[6]));
+    float jy = (g[2] - g[4]) + ((g[5] - g[7]) + (g[6] - g[8]));
+    rho = 1.0f + drho;
+    if (rho < 1e-6f) {
+        ux = 0.f;
+        uy = 0.f;
+    } else {
+        float inv = 1.0f / rho;
+        ux = jx * inv;
+        uy = jy * inv;
+    }
+}
+
+// shifted equilibrium feq - w = w (drho + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u))
+IPPI_HD void geq(float drho, float rho, float ux, float uy, float* q) {
+    float uu = 1.5f * (ux * ux + uy * uy);
+    float s = ux + uy, d = ux - uy;
+    q[0] = WF[0] * (drho - rho * uu);
+    float ax = rho * (4.5f * ux * ux - uu), ay = rho * (4.5f * uy * uy - uu);
+    float as = rho * (4.5f * s * s - uu), ad = rho * (4.5f * d * d - uu);
+    float bx = 3.0f * rho * ux, by = 3.0f * rho * uy, bs = 3.0f * rho * s, bd = 3.0f * rho * d;
+    q[1] = WF[1] * (drho + ax + bx);
+    q[3] = WF[3] * (drho + ax - bx);
+    q[2] = WF[2] * (drho + ay + by);
+    q[4] = WF[4] * (drho + ay - by);
+    q[5] = WF[5] * (drho + as + bs);
+    q[7] = WF[7] * (drho + as - bs);
+    q[8] = WF[8] * (drho + ad + bd);
+    q[6] = WF[6] * (drho + ad - bd);
+}
+
+IPPI_HD void neq_stress(const float* g, const float* q, float& pxx, float& pxy, float& pyy) {
+    float d[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = g[k] - q[k];
+    float c = (d[5] + d[6]) + (d[7] + d[8]);
+    pxx = (d[1] + d[3]) + c;
+    pyy = (d[2] + d[4]) + c;
+    pxy = (d[5] - d[6]) + (d[7] - d[8]);
+}
+
+IPPI_HD float les_tau(float pxx, float pxy, float pyy, float rho, float tau0, float cs) {
+    float c = 3.0f / (2.0f * rho * tau0);
+    float S2 = c * c * (pxx * pxx + 2.0f * pxy * pxy + pyy * pyy);
+    float Q = sqrtf(2.0f * S2);
+    float g1 = tau0 * tau0 + 18.0f * cs * cs * Q / rho;
+    return 0.5f * (sqrtf(g1) + tau0);
+}
+
+IPPI_HD void bgk(float* g, const float* q, float tau) {
+    float om = 1.0f / tau;
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = g[k] + om * (q[k] - g[k]);
+}
+
+IPPI_HD void rr(float* g, const float* q, float pxx, float pxy, float pyy, float tau) {
+    float a = 1.0f - 1.0f / tau;
+    const float third = 1.0f / 3.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        float q0 = (float)(EX[k] * EX[k]) - third;
+        float q3 = (float)(EY[k] * EY[k]) - third;
+        float s = q0 * pxx + q3 * pyy;
+        if (EX[k] * EY[k] != 0) s += 2.0f * (float)(EX[k] * EY[k]) * pxy;
+        g[k] = q[k] + a * (4.5f * WF[k]) * s;
+    }
+}
+
+// MRT on shifted populations: M w = (1, -2, 1, 0, ...) so the shifted
+// moment equilibria are meq(rho, u) - M w.
+IPPI_HD void mrt(float* g, float drho, float rho, float ux, float uy, float tau) {
+    float m[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        float s = 0.f;
+#pragma unroll
+        for (int l = 0; l < 9; l++)
+            if (TM[k][l] != 0.0) s += (float)TM[k][l] * g[l];
+        m[k] = s;
+    }
+    float t1 = ux * ux + uy * uy, t2 = ux * ux - uy * uy;
+    float eq[9];
+    eq[1] = -2.0f * drho + 3.0f * rho * t1;
+    eq[2] = drho - 3.0f * rho * t1;
+    eq[4] = -rho * ux;
+    eq[6] = -rho * uy;
+    eq[7] = rho * t2;
+    eq[8] = rho * ux * uy;
+    float s78 = 1.0f / tau;
+    float dm[9];
+#pragma unroll
+    for (int l = 0; l < 9; l++) {
+        if (SM_FIXED[l] == 0.0) {
+            dm[l] = 0.f;
+        } else {
+            float sl = SM_FIXED[l] < 0 ? s78 : (float)SM_FIXED[l];
+            dm[l] = sl * (m[l] - eq[l]) * (1.0f / 36.0f);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        float s = 0.f;
+#pragma unroll
+        for (int l = 0; l < 9; l++)
+            if (TMINV_K[k][l] != 0 && SM_FIXED[l] != 0.0) s += (float)TMINV_K[k][l] * dm[l];
+        g[k] -= s;
+    }
+}
+
+IPPI_HD void guo(float ux, float uy, float fx, float fy, float tau_begin, float* g) {
+    float omega = 1.0f - 0.5f / tau_begin;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        float ex = (float)EX[k], ey = (float)EY[k];
+        float eu = ex * ux + ey * uy;
+        float ef = ex * fx + ey * fy;
+        float uf = ux * fx + uy * fy;
+        g[k] += WF[k] * omega * (3.0f * (ef - uf) + 9.0f * eu * ef);
+    }
+}
+
+}  // namespace sh
+}  // namespace lbm

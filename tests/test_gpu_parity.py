@@ -1,0 +1,217 @@
+"""HIP library (through the C ABI) against the FP64 CPU oracle.
+
+FP64 path: the kernels keep the reference's evaluation order, so fields are
+expected to agree bit for bit; the asserted bar is the relative L2 error
+<= 1e-12 on density and velocity. FP32 shifted-population path: relative L2
+error of the velocity against the FP64 oracle <= 1e-5 (the stated budget of
+the reduced-precision path), density <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+from tests import _pkg
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_U_TOL = 1e-5
+FP32_RHO_TOL = 1e-6
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def vel_err(ux, uy, rx, ry):
+    num = np.sqrt(np.sum((ux - rx) ** 2 + (uy - ry) ** 2))
+    return float(num / np.sqrt(np.sum(rx ** 2 + ry ** 2)))
+
+
+@pytest.fixture(scope="module")
+def ippi():
+    m = _pkg.load()
+    m.library()
+    return m
+
+
+CASES = {
+    # name: (nx, ny, collision, bc, les, tau_mode, tau0, cs, u0, body, steps)
+    "bgk_channel_cyl": (256, 128, "bgk", "periodic_bb", False, "const", 0.6, 0.0, (0.05, 0.0), (64, 64, 20, 63), 1000),
+    "mrt_les_inflow_cyl": (160, 80, "mrt", "inflow", True, "const", 0.52, 0.1, (0.08, 0.0), (48, 40, 20, 63), 400),
+    "rr_les_channel_cyl": (128, 64, "rr", "periodic_bb", True, "const", 0.51, 0.15, (0.06, 0.0), (40, 32, 16, 51), 400),
+    "bgk_field_channel_cyl": (128, 64, "bgk", "periodic_bb", False, "field", 0.6, 0.0, (0.05, 0.0), (40, 30, 16, 51), 400),
+    "mrt_les_channel": (96, 64, "mrt", "periodic_bb", True, "const", 0.505, 0.12, (0.08, 0.0), None, 300),
+    "bgk_inflow": (96, 48, "bgk", "inflow", False, "const", 0.7, 0.0, (0.05, 0.0), None, 300),
+    "rr_les_inflow_cyl": (128, 64, "rr", "inflow", True, "const", 0.53, 0.1, (0.07, 0.0), (40, 32, 16, 51), 300),
+    "bgk_les_channel_cyl": (128, 64, "bgk", "periodic_bb", True, "const", 0.51, 0.173, (0.08, 0.0), (40, 32, 20, 63), 300),
+}
+
+
+def tau_field(nx, ny, seed=7):
+    rng = np.random.default_rng(seed)
+    return 0.6 + rng.uniform(-0.02, 0.02, (ny, nx))
+
+
+def run_pair(oracle, ippi, case, precision):
+    nx, ny, coll, bc, les, tm, tau0, cs, u0, body, steps = CASES[case]
+    o = oracle.Sim(nx, ny, coll, bc, les=les, tau_mode=tm, tau0=tau0, cs=cs, u_in=u0, threads=4)
+    g = ippi.Solver(nx, ny, coll, precision, bc, les=les, tau_mode=tm, tau0=tau0, cs=cs, u_in=u0)
+    o.init_fields(42, 1.0, u0, 1e-3)
+    g.init_fields(42, 1.0, u0, 1e-3)
+    if tm == "field":
+        t = tau_field(nx, ny)
+        o.set_tau_field(t)
+        g.set_tau_field(t)
+    if body:
+        o.add_body(*body)
+        g.add_body(*body)
+    print(f"case {case} precision {precision} grid {nx}x{ny} steps {steps}", flush=True)
+    assert o.step(steps) == 0
+    g.step(steps)
+    return o, g, steps
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fp64_matches_oracle(oracle, ippi, case):
+    o, g, steps = run_pair(oracle, ippi, case, "fp64")
+    r0, x0, y0 = o.fields()
+    r1, x1, y1 = g.fields()
+    assert rel_l2(r1, r0) <= FP64_TOL
+    assert vel_err(x1, y1, x0, y0) <= FP64_TOL
+    f0, f1 = o.fnew(), g.populations()
+    assert rel_l2(f1, f0) <= FP64_TOL
+    ndiff = int(np.count_nonzero(f0 != f1))
+    print(f"{case}: populations differing bitwise: {ndiff} of {f0.size}")
+    if CASES[case][9] is not None:
+        d0, d1 = o.forces(), g.body_forces()
+        assert d1.shape == d0.shape == (steps, 1, 2)
+        assert rel_l2(d1, d0) <= FP64_TOL
+        m0, m1 = o.markers(), g.markers()
+        assert np.array_equal(m0[:, :2], m1[:, :2])
+        assert rel_l2(m1[:, 2:], m0[:, 2:]) <= FP64_TOL
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fp32_shifted_within_budget(oracle, ippi, case):
+    o, g, steps = run_pair(oracle, ippi, case, "fp32")
+    r0, x0, y0 = o.fields()
+    r1, x1, y1 = g.fields()
+    eu = vel_err(x1, y1, x0, y0)
+    er = rel_l2(r1, r0)
+    print(f"{case}: FP32 vs FP64 oracle: rel L2 u {eu:.3e}, rho {er:.3e}")
+    assert eu <= FP32_U_TOL
+    assert er <= FP32_RHO_TOL
+    if CASES[case][9] is not None:
+        d0, d1 = o.forces(), g.body_forces()
+        fd = np.abs(d1 - d0).max() / np.abs(d0).max()
+        print(f"{case}: FP32 body-force max rel error {fd:.3e}")
+        assert fd <= 1e-3
+    g.close()
+    o.close()
+
+
+def test_initial_fields_exact(oracle, ippi):
+    nx, ny = 64, 32
+    o = oracle.Sim(nx, ny)
+    g = ippi.Solver(nx, ny)
+    o.init_fields(123, 1.0, (0.05, -0.01), 1e-3)
+    g.init_fields(123, 1.0, (0.05, -0.01), 1e-3)
+    for a, b in zip(o.fields(), g.fields()):
+        assert np.array_equal(a, b)
+    assert np.array_equal(o.fnew(), g.populations())
+
+
+def test_set_fields_matches_init(oracle, ippi):
+    nx, ny = 64, 32
+    rng = np.random.default_rng(3)
+    rho = 1 + rng.uniform(-1e-3, 1e-3, (ny, nx))
+    ux = rng.uniform(-0.05, 0.05, (ny, nx))
+    uy = rng.uniform(-0.05, 0.05, (ny, nx))
+    o = oracle.Sim(nx, ny, "mrt", "periodic_bb", les=True, tau0=0.55, cs=0.1)
+    g = ippi.Solver(nx, ny, "mrt", "fp64", "periodic_bb", les=True, tau0=0.55, cs=0.1)
+    f = np.stack([np.array([oracle.feq_init(a, b, c)[k] for a, b, c in zip(rho.ravel(), ux.ravel(), uy.ravel())])
+                  for k in range(9)])
+    o.set_state(f, rho, ux, uy)
+    g.set_fields(rho, ux, uy)
+    o.step(50)
+    g.step(50)
+    r0, x0, y0 = o.fields()
+    r1, x1, y1 = g.fields()
+    assert rel_l2(r1, r0) <= FP64_TOL and vel_err(x1, y1, x0, y0) <= FP64_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_checkpoint_roundtrip(ippi, precision):
+    g = ippi.Solver(96, 48, "mrt", precision, "inflow", les=True, tau0=0.52, cs=0.1, u_in=(0.06, 0))
+    g.init_fields(9, 1.0, (0.06, 0), 1e-3)
+    g.step(40)
+    blob = g.save()
+    g.step(40)
+    a = g.populations()
+    g.load(blob)
+    g.step(40)
+    b = g.populations()
+    assert np.array_equal(a, b)
+    g.close()
+
+
+def test_divergence_is_reported(ippi):
+    nx, ny = 32, 16
+    g = ippi.Solver(nx, ny)
+    rho = np.ones((ny, nx))
+    rho[5, 7] = -1.0  # the first collision sees rho < 0 (ErrorCatch)
+    g.set_fields(rho, np.zeros((ny, nx)), np.zeros((ny, nx)))
+    g.step(1)
+    with pytest.raises(ippi.IppiError) as e:
+        g.fields()
+    assert e.value.code == ippi.IPPI_ERR_DIVERGED
+    g.close()
+
+
+def test_rejects_bad_bodies(ippi):
+    g = ippi.Solver(64, 32)
+    with pytest.raises(ippi.IppiError):
+        g.add_body(3.0, 16.0, 8.0, 25)  # support leaves the interior
+    g.add_body(20.0, 16.0, 8.0, 25)
+    with pytest.raises(ippi.IppiError):
+        g.add_body(24.0, 16.0, 8.0, 25)  # overlapping supports
+    g.close()
+
+
+# ---- full-size, size-independent properties (2048 x 1024) --------------------
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_full_size_mass_conservation(ippi, precision):
+    nx, ny = 2048, 1024
+    g = ippi.Solver(nx, ny, "mrt", precision, "periodic_bb", les=True, tau0=0.50192, cs=0.1)
+    g.init_fields(5, 1.0, (0.08, 0.0), 1e-3)
+    g.add_body(512, 512, 80, 251)
+    m0 = g.mass()
+    g.step(200)
+    m1 = g.mass()
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    print(f"{precision}: relative mass drift {abs(m1 - m0) / m0:.3e}")
+    assert abs(m1 - m0) / m0 < tol
+    g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_full_size_translation_equivariance(ippi, precision):
+    """Periodic in x: shifting the initial field by s columns shifts the
+    solution by s columns, exactly (same arithmetic on every node)."""
+    nx, ny, s = 2048, 1024, 333
+    rng = np.random.default_rng(11)
+    rho = 1 + rng.uniform(-1e-3, 1e-3, (ny, nx))
+    ux = 0.08 + rng.uniform(-1e-3, 1e-3, (ny, nx))
+    uy = rng.uniform(-1e-3, 1e-3, (ny, nx))
+    out = []
+    for shift in (0, s):
+        g = ippi.Solver(nx, ny, "mrt", precision, "periodic_bb", les=True, tau0=0.50192, cs=0.1)
+        g.set_fields(*(np.roll(a, shift, axis=1) for a in (rho, ux, uy)))
+        g.step(100)
+        out.append(g.fields())
+        g.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(np.roll(a, s, axis=1), b)
