@@ -1,0 +1,79 @@
+// Mesh facade: the reference's Mesh interface (IPPI-LBM/mesh.h:33-115) over
+// the C ABI of libippi_lbm.so, so a driver written like IPPI-LBM/main.cpp
+// runs the whole domain on one GPU. Public fields keep the reference's names
+// and meaning; RankID must be 0 and CoreNum 1 (one GPU holds every column).
+// The compile-time switches of IPPI-LBM/Parameter.h are runtime fields here
+// (Collision, Precision, Boundary, UseLES).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/ippi_lbm.h"
+
+typedef double REAL;
+struct Point2D {
+    REAL x;
+    REAL y;
+};
+
+class Mesh {
+public:
+    Mesh();
+    ~Mesh();
+    // reference fields (mesh.h:37-52)
+    int LatmeshSizeNx = 0;
+    int LatmeshSizeNy = 0;
+    Point2D center{0, 0};
+    double radius = 0;
+    double startTau = 0.6;
+    Point2D startU{0, 0};
+    int MyrankID = 0;
+    int MyCoreNum = 1;
+    int stepH = 0, stepL = 0;
+    int startAvg = 0;
+    double StartCs = 0.0;
+    int errorflag = 0;
+    int ParallelSize = 0;
+    // runtime equivalents of Parameter.h:8-16
+    int Collision = IPPI_COLL_BGK;   // SRTLBM / MRT / RRLBM
+    int Precision = IPPI_FP64;       // REAL = double, or FP32 shifted populations
+    int Boundary = IPPI_BC_INFLOW;   // inlet/outlet NEE + free-slip walls, or periodic + bounce-back
+    int UseLES = 1;                  // SmagorinskyByHybrid
+    uint64_t Seed = 0;
+    double NoiseAmplitude = 0.0;
+
+    void InitialLatMesh(int Nx_P, int Ny, double tau, Point2D StartVel, double Rho, int RankID, int CoreNum,
+                        int PartitionRemainderFlag);
+    void InitialIBParticle(int NumNodes, double Radius, double ParticlePositionX, double ParticlePositionY,
+                           Point2D ParticleVel);
+    void OneLBMStep(int RankID, int CoreNum, REAL Cs, REAL Pr, int t);
+    void JudgeStyle(Point2D CylinderCenter1, REAL Radius, int RankID, int CoreNum);
+    void InitialFile();
+    void WriteDatFluid(int t, int RankID, int NX, int CoreNum);
+    void MergeDatFluid(int t, int CoreNum);
+    void OutputCd(int RankID, int CoreNum, int time, double Radius);
+    void ComputeAvgVelocity(int t);
+    void ErrorCatch(int i, int j);
+
+    // direct access for callers that read the fields back
+    int Fields(std::vector<double>& rho, std::vector<double>& ux, std::vector<double>& uy);
+    ippi_handle* Handle() { return h_; }
+
+private:
+    void Check(int rc, const char* what);
+    void EnsureHandle(REAL Cs);
+    ippi_handle* h_ = nullptr;
+    int nx_ = 0, ny_ = 0;
+    double rho0_ = 1.0;
+    Point2D u0_{0, 0};
+    double cs_ = -1.0;
+    bool fields_set_ = false;
+    struct PendingBody {
+        double cx, cy, d;
+        int m;
+        Point2D v;
+    };
+    std::vector<PendingBody> pending_;
+    std::vector<double> avg_u_, avg_v_, avg_rho_;
+    int last_force_rec_ = 0;
+};
