@@ -38,6 +38,7 @@ struct Patch {
 struct StepArgs {
     int nx, ny;
     double tau0, cs, uinx, uiny;
+    float tau0f, csf, uinxf, uinyf;
     int les;
     int tau_field;          // 1: per-node base tau from tau_ptr
     const void* tau_ptr;    // double* (FP64) / float* (FP32)
@@ -144,8 +145,9 @@ __device__ __forceinline__ void node_state(const double* __restrict__ s, int x, 
 }
 
 __device__ __forceinline__ void node_state(const float* __restrict__ s, int x, int y, int nx, int ny, unsigned N,
-                                           int bc, double uinx, double uiny, float* g, float& drho, float& rho,
+                                           int bc, float uinx, float uiny, float* g, float& drho, float& rho,
                                            float& ux, float& uy) {
+#pragma clang fp contract(fast)
     if (bc == IPPI_BC_PERIODIC_BB) {
         pull<float, IPPI_BC_PERIODIC_BB>(s, x, y, nx, ny, N, g);
         sh::macro(g, drho, rho, ux, uy);
@@ -156,7 +158,7 @@ __device__ __forceinline__ void node_state(const float* __restrict__ s, int x, i
     pull<float, IPPI_BC_INFLOW>(s, xs, y, nx, ny, N, g);
     sh::macro(g, drho, rho, ux, uy);
     if (nee) {
-        float bx = x == 0 ? (float)uinx : ux, by = x == 0 ? (float)uiny : uy;
+        float bx = x == 0 ? uinx : ux, by = x == 0 ? uiny : uy;
         float qb[9], qn[9];
         sh::geq(0.f, 1.f, bx, by, qb);
         sh::geq(drho, rho, ux, uy, qn);
@@ -246,6 +248,7 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
 template <int COLL, int BC, bool PULL>
 __global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
+#pragma clang fp contract(fast)
     const int x = blockIdx.x * BX + threadIdx.x;
     const int y = blockIdx.y;
     const int nx = a.nx, ny = a.ny;
@@ -254,7 +257,7 @@ __global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, 
     const unsigned n = (unsigned)y * nx + x;
     float g[9], drho, rho, ux, uy;
     if (PULL) {
-        node_state(src, x, y, nx, ny, N, BC, a.uinx, a.uiny, g, drho, rho, ux, uy);
+        node_state(src, x, y, nx, ny, N, BC, a.uinxf, a.uinyf, g, drho, rho, ux, uy);
     } else {
 #pragma unroll
         for (int k = 0; k < 9; k++) g[k] = src[k * N + n];
@@ -273,12 +276,12 @@ __global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, 
         if (forced) {
             fx = (float)fxd;
             fy = (float)fyd;
-            float ir = 0.5f / rho;
+            float ir = 0.5f * sh::rcp(rho);
             ux += fx * ir;
             uy += fy * ir;
         }
     }
-    const float tb = a.tau_field ? static_cast<const float*>(a.tau_ptr)[n] : (float)a.tau0;
+    const float tb = a.tau_field ? static_cast<const float*>(a.tau_ptr)[n] : a.tau0f;
     float tau = tb;
     float q[9];
     sh::geq(drho, rho, ux, uy, q);
@@ -286,21 +289,21 @@ __global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, 
         if (a.les) {
             float pxx, pxy, pyy;
             sh::neq_stress(g, q, pxx, pxy, pyy);
-            tau = sh::les_tau(pxx, pxy, pyy, rho, tb, (float)a.cs);
+            tau = sh::les_tau(pxx, pxy, pyy, rho, tb, a.csf);
         }
         sh::mrt(g, drho, rho, ux, uy, tau);
     } else if (COLL == IPPI_COLL_RR) {
         float pxx, pxy, pyy;
         sh::neq_stress(g, q, pxx, pxy, pyy);
-        if (a.les) tau = sh::les_tau(pxx, pxy, pyy, rho, tb, (float)a.cs);
-        sh::rr(g, q, pxx, pxy, pyy, tau);
+        if (a.les) tau = sh::les_tau(pxx, pxy, pyy, rho, tb, a.csf);
+        sh::rr(g, q, pxx, pxy, pyy, tau, fx, fy);
     } else {
         if (a.les) {
             float pxx, pxy, pyy;
             sh::neq_stress(g, q, pxx, pxy, pyy);
-            tau = sh::les_tau(pxx, pxy, pyy, rho, tb, (float)a.cs);
+            tau = sh::les_tau(pxx, pxy, pyy, rho, tb, a.csf);
         }
-        sh::bgk(g, q, tau);
+        sh::bgk(g, q, tau, fx, fy);
     }
     if (forced) sh::guo(ux, uy, fx, fy, tb, g);
 #pragma unroll
@@ -357,7 +360,7 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
         node_state(src, x, y, nx, ny, N, BC, uinx, uiny, f, rho, ux, uy);
     } else {
         float g[9], dr, r, vx, vy;
-        node_state(src, x, y, nx, ny, N, BC, uinx, uiny, g, dr, r, vx, vy);
+        node_state(src, x, y, nx, ny, N, BC, (float)uinx, (float)uiny, g, dr, r, vx, vy);
         double s = 0.0, jx, jy;
         double gd[9];
 #pragma unroll
@@ -396,58 +399,75 @@ struct MarkerDev {
     int nm;
 };
 
+// One thread per (marker, support node): 16 markers per 256-thread block.
+// Contributions u(X,Y) phi phi go to LDS and one thread per marker adds
+// them in the reference's X-outer / Y-inner order.
+constexpr int IB_MPB = 16;
+
 template <typename T, int BC, bool PULL>
-__global__ void k_ib_markers(const T* __restrict__ src, const double* __restrict__ mac, MarkerDev md, int nx,
-                             int ny) {
+__global__ void __launch_bounds__(256) k_ib_markers(const T* __restrict__ src, const double* __restrict__ mac,
+                                                    MarkerDev md, int nx, int ny) {
 #pragma clang fp contract(off)
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= md.nm) return;
+    __shared__ double cx[IB_MPB][16], cy[IB_MPB][16];
+    const int lm = threadIdx.x >> 4, sidx = threadIdx.x & 15;
+    const int m = blockIdx.x * IB_MPB + lm;
     const unsigned N = (unsigned)nx * ny;
-    const int X0 = md.base[2 * m], Y0 = md.base[2 * m + 1];
-    double uix = 0, uiy = 0;
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) {
-            const int X = X0 + i, Y = Y0 + j;
-            const unsigned n = (unsigned)Y * nx + X;
-            double ux, uy;
-            if (PULL) {
-                if constexpr (sizeof(T) == 8) {
-                    double f[9], r;
-                    pull<double, BC>(src, X, Y, nx, ny, N, f);
-                    ref::macro(f, r, ux, uy);
-                } else {
-                    float g[9], dr, r, vx, vy;
-                    pull<float, BC>(src, X, Y, nx, ny, N, g);
-                    sh::macro(g, dr, r, vx, vy);
-                    ux = vx;
-                    uy = vy;
-                }
+    if (m < md.nm) {
+        const int X = md.base[2 * m] + (sidx >> 2), Y = md.base[2 * m + 1] + (sidx & 3);
+        const unsigned n = (unsigned)Y * nx + X;
+        double ux, uy;
+        if (PULL) {
+            if constexpr (sizeof(T) == 8) {
+                double f[9], r;
+                pull<double, BC>(src, X, Y, nx, ny, N, f);
+                ref::macro(f, r, ux, uy);
             } else {
-                if constexpr (sizeof(T) == 8) {
-                    ux = mac[N + n];
-                    uy = mac[2 * N + n];
-                } else {
-                    ux = (double)(float)mac[N + n];
-                    uy = (double)(float)mac[2 * N + n];
-                }
+                float g[9], dr, r, vx, vy;
+                pull<float, BC>(src, X, Y, nx, ny, N, g);
+                sh::macro(g, dr, r, vx, vy);
+                ux = vx;
+                uy = vy;
             }
-            const double D = md.w[16 * m + 4 * i + j];
-            uix += (ux * D);
-            uiy += (uy * D);
+        } else {
+            if constexpr (sizeof(T) == 8) {
+                ux = mac[N + n];
+                uy = mac[2 * N + n];
+            } else {
+                ux = (double)(float)mac[N + n];
+                uy = (double)(float)mac[2 * N + n];
+            }
         }
-    md.ui[2 * m] = uix;
-    md.ui[2 * m + 1] = uiy;
-    md.F[2 * m] = 2 * (md.ud[2 * m] - uix);
-    md.F[2 * m + 1] = 2 * (md.ud[2 * m + 1] - uiy);
+        const double D = md.w[16 * m + sidx];
+        cx[lm][sidx] = (ux * D);
+        cy[lm][sidx] = (uy * D);
+    }
+    __syncthreads();
+    if (m < md.nm && sidx == 0) {
+        double uix = 0, uiy = 0;
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            uix += cx[lm][s];
+            uiy += cy[lm][s];
+        }
+        md.ui[2 * m] = uix;
+        md.ui[2 * m + 1] = uiy;
+        md.F[2 * m] = 2 * (md.ud[2 * m] - uix);
+        md.F[2 * m + 1] = 2 * (md.ud[2 * m + 1] - uiy);
+    }
 }
 
 // SpreadForce (mesh.cpp:659-741) as a gather: each patch node sums its
 // (marker, weight) list in increasing marker order, the order of the
-// reference's marker loop. Also the per-body force record (minus the sum of
-// the marker forces, marker order).
-__global__ void k_ib_spread(const int* __restrict__ ptr, const int* __restrict__ mk, const double* __restrict__ w,
-                            const double* __restrict__ F, double* __restrict__ patch_f, int npn,
-                            const int* __restrict__ body_first, int nbody, double* __restrict__ hist) {
+// reference's marker loop. Block 0 also writes the per-body force record
+// (minus the sum of the marker forces, in marker order), staging the
+// marker forces through LDS in chunks.
+constexpr int IB_CHUNK = 2048;
+
+__global__ void __launch_bounds__(256) k_ib_spread(const int* __restrict__ ptr, const int* __restrict__ mk,
+                                                   const double* __restrict__ w, const double* __restrict__ F,
+                                                   double* __restrict__ patch_f, int npn,
+                                                   const int* __restrict__ body_first, int nbody, int nm,
+                                                   double* __restrict__ hist) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < npn) {
@@ -460,15 +480,27 @@ __global__ void k_ib_spread(const int* __restrict__ ptr, const int* __restrict__
         patch_f[2 * i] = fx;
         patch_f[2 * i + 1] = fy;
     }
-    if (hist && blockIdx.x == 0 && threadIdx.x < nbody) {
+    if (hist && blockIdx.x == 0) {
+        __shared__ double sf[2 * IB_CHUNK];
         const int b = threadIdx.x;
         double fx = 0.0, fy = 0.0;
-        for (int m = body_first[b]; m < body_first[b + 1]; m++) {
-            fx += F[2 * m];
-            fy += F[2 * m + 1];
+        const int b0 = b < nbody ? body_first[b] : 0, b1 = b < nbody ? body_first[b + 1] : 0;
+        for (int c0 = 0; c0 < nm; c0 += IB_CHUNK) {
+            const int c1 = min(nm, c0 + IB_CHUNK);
+            __syncthreads();
+            for (int j = c0 * 2 + (int)threadIdx.x; j < c1 * 2; j += blockDim.x) sf[j - 2 * c0] = F[j];
+            __syncthreads();
+            if (b < nbody) {
+                for (int m = max(b0, c0); m < min(b1, c1); m++) {
+                    fx += sf[2 * (m - c0)];
+                    fy += sf[2 * (m - c0) + 1];
+                }
+            }
         }
-        hist[2 * b] = -fx;
-        hist[2 * b + 1] = -fy;
+        if (b < nbody) {
+            hist[2 * b] = -fx;
+            hist[2 * b + 1] = -fy;
+        }
     }
 }
 
@@ -586,6 +618,10 @@ StepArgs make_args(ippi_handle* h) {
     a.cs = h->c.cs;
     a.uinx = h->c.u_in_x;
     a.uiny = h->c.u_in_y;
+    a.tau0f = (float)h->c.tau0;
+    a.csf = (float)h->c.cs;
+    a.uinxf = (float)h->c.u_in_x;
+    a.uinyf = (float)h->c.u_in_y;
     a.les = h->c.les;
     a.tau_field = h->c.tau_mode == IPPI_TAU_FIELD;
     a.tau_ptr = h->tau;
@@ -669,24 +705,24 @@ int launch_ib(ippi_handle* h, bool pull) {
     MarkerDev md{h->d_base, h->d_w, h->d_ud, h->d_ui, h->d_F, nm};
     const T* src = (const T*)h->buf[h->cur];
     const int nx = h->c.nx, ny = h->c.ny;
-    dim3 gm((nm + 127) / 128);
+    dim3 gm((nm + IB_MPB - 1) / IB_MPB);
     if (h->c.bc == IPPI_BC_PERIODIC_BB) {
         if (pull)
-            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_PERIODIC_BB, true>), gm, dim3(128), 0, h->s, src, h->mac, md, nx, ny);
+            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_PERIODIC_BB, true>), gm, dim3(256), 0, h->s, src, h->mac, md, nx, ny);
         else
-            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_PERIODIC_BB, false>), gm, dim3(128), 0, h->s, src, h->mac, md, nx, ny);
+            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_PERIODIC_BB, false>), gm, dim3(256), 0, h->s, src, h->mac, md, nx, ny);
     } else {
         if (pull)
-            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_INFLOW, true>), gm, dim3(128), 0, h->s, src, h->mac, md, nx, ny);
+            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_INFLOW, true>), gm, dim3(256), 0, h->s, src, h->mac, md, nx, ny);
         else
-            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_INFLOW, false>), gm, dim3(128), 0, h->s, src, h->mac, md, nx, ny);
+            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_INFLOW, false>), gm, dim3(256), 0, h->s, src, h->mac, md, nx, ny);
     }
     HIPC(h, hipGetLastError());
     const int nb = (int)h->bodies.size();
     double* rec = h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap);
     dim3 gs((std::max(h->npn, 1) + 255) / 256);
     hipLaunchKernelGGL(k_ib_spread, gs, dim3(256), 0, h->s, h->d_ptr, h->d_mk, h->d_cw, h->d_F, h->d_pf, h->npn,
-                       h->d_bfirst, nb, rec);
+                       h->d_bfirst, nb, nm, rec);
     HIPC(h, hipGetLastError());
     h->nrec++;
     return IPPI_OK;

@@ -265,7 +265,24 @@ IPPI_HD double les_tau(const double* f, double rho, double ux, double uy, double
 // g_k = f_k - w_k; rho = 1 + sum g; rho u = sum g e (sum w e = 0).
 namespace sh {
 
+// hardware reciprocal / square root (1 ulp) on the device, exact on the host
+IPPI_HD float rcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+IPPI_HD float fsqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+
 IPPI_HD void macro(const float* g, float& drho, float& rho, float& ux, float& uy) {
+#pragma clang fp contract(fast)
     float a = (g[1] + g[3]) + (g[2] + g[4]);
     float b = (g[5] + g[7]) + (g[6] + g[8]);
     drho = g[0] + (a + b);
@@ -276,7 +293,7 @@ IPPI_HD void macro(const float* g, float& drho, float& rho, float& ux, float& uy
         ux = 0.f;
         uy = 0.f;
     } else {
-        float inv = 1.0f / rho;
+        float inv = rcp(rho);
         ux = jx * inv;
         uy = jy * inv;
     }
@@ -284,6 +301,7 @@ IPPI_HD void macro(const float* g, float& drho, float& rho, float& ux, float& uy
 
 // shifted equilibrium feq - w = w (drho + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u))
 IPPI_HD void geq(float drho, float rho, float ux, float uy, float* q) {
+#pragma clang fp contract(fast)
     float uu = 1.5f * (ux * ux + uy * uy);
     float s = ux + uy, d = ux - uy;
     q[0] = WF[0] * (drho - rho * uu);
@@ -301,6 +319,7 @@ IPPI_HD void geq(float drho, float rho, float ux, float uy, float* q) {
 }
 
 IPPI_HD void neq_stress(const float* g, const float* q, float& pxx, float& pxy, float& pyy) {
+#pragma clang fp contract(fast)
     float d[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) d[k] = g[k] - q[k];
@@ -311,35 +330,68 @@ IPPI_HD void neq_stress(const float* g, const float* q, float& pxx, float& pxy, 
 }
 
 IPPI_HD float les_tau(float pxx, float pxy, float pyy, float rho, float tau0, float cs) {
-    float c = 3.0f / (2.0f * rho * tau0);
+#pragma clang fp contract(fast)
+    float ir = rcp(rho);
+    float c = 1.5f * ir * rcp(tau0);
     float S2 = c * c * (pxx * pxx + 2.0f * pxy * pxy + pyy * pyy);
-    float Q = sqrtf(2.0f * S2);
-    float g1 = tau0 * tau0 + 18.0f * cs * cs * Q / rho;
-    return 0.5f * (sqrtf(g1) + tau0);
+    float Q = fsqrt(2.0f * S2);
+    float g1 = tau0 * tau0 + 18.0f * cs * cs * Q * ir;
+    return 0.5f * (fsqrt(g1) + tau0);
 }
 
-IPPI_HD void bgk(float* g, const float* q, float tau) {
-    float om = 1.0f / tau;
+// Set the mass and momentum of a collision increment d to their exact
+// values: zero mass, momentum (tx, ty) (zero without forcing; with the
+// immersed-boundary velocity shift u = (j + F/2)/rho it is omega F/2 for
+// BGK and F/2 for RR). In FP32 the rounding of the equilibrium is nearly the
+// same at every node of a near-uniform flow, so its residual would add up
+// coherently over the domain and over time. The projection
+// d -= w (m + 3 e.(j - t)) removes it (sum w = 1, sum w e e = I/3).
+IPPI_HD void conserve(float* d, float tx, float ty) {
+#pragma clang fp contract(fast)
+    float c = (d[5] + d[6]) + (d[7] + d[8]);
+    float m = d[0] + ((d[1] + d[3]) + (d[2] + d[4])) + c;
+    float jx = (d[1] - d[3]) + ((d[5] - d[7]) + (d[8] - d[6]));
+    float jy = (d[2] - d[4]) + ((d[5] - d[7]) + (d[6] - d[8]));
+    jx = 3.0f * (jx - tx);
+    jy = 3.0f * (jy - ty);
 #pragma unroll
-    for (int k = 0; k < 9; k++) g[k] = g[k] + om * (q[k] - g[k]);
+    for (int k = 0; k < 9; k++) d[k] -= WF[k] * (m + (float)EX[k] * jx + (float)EY[k] * jy);
 }
 
-IPPI_HD void rr(float* g, const float* q, float pxx, float pxy, float pyy, float tau) {
-    float a = 1.0f - 1.0f / tau;
+// BGK; (fx, fy) is the immersed-boundary force already folded into u
+IPPI_HD void bgk(float* g, const float* q, float tau, float fx, float fy) {
+#pragma clang fp contract(fast)
+    float om = rcp(tau);
+    float d[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = om * (q[k] - g[k]);
+    conserve(d, 0.5f * om * fx, 0.5f * om * fy);
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] += d[k];
+}
+
+IPPI_HD void rr(float* g, const float* q, float pxx, float pxy, float pyy, float tau, float fx, float fy) {
+#pragma clang fp contract(fast)
+    float a = 1.0f - rcp(tau);
     const float third = 1.0f / 3.0f;
+    float d[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         float q0 = (float)(EX[k] * EX[k]) - third;
         float q3 = (float)(EY[k] * EY[k]) - third;
         float s = q0 * pxx + q3 * pyy;
         if (EX[k] * EY[k] != 0) s += 2.0f * (float)(EX[k] * EY[k]) * pxy;
-        g[k] = q[k] + a * (4.5f * WF[k]) * s;
+        d[k] = (q[k] - g[k]) + a * (4.5f * WF[k]) * s;
     }
+    conserve(d, 0.5f * fx, 0.5f * fy);
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] += d[k];
 }
 
 // MRT on shifted populations: M w = (1, -2, 1, 0, ...) so the shifted
 // moment equilibria are meq(rho, u) - M w.
 IPPI_HD void mrt(float* g, float drho, float rho, float ux, float uy, float tau) {
+#pragma clang fp contract(fast)
     float m[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -357,7 +409,7 @@ IPPI_HD void mrt(float* g, float drho, float rho, float ux, float uy, float tau)
     eq[6] = -rho * uy;
     eq[7] = rho * t2;
     eq[8] = rho * ux * uy;
-    float s78 = 1.0f / tau;
+    float s78 = rcp(tau);
     float dm[9];
 #pragma unroll
     for (int l = 0; l < 9; l++) {
@@ -379,7 +431,8 @@ IPPI_HD void mrt(float* g, float drho, float rho, float ux, float uy, float tau)
 }
 
 IPPI_HD void guo(float ux, float uy, float fx, float fy, float tau_begin, float* g) {
-    float omega = 1.0f - 0.5f / tau_begin;
+#pragma clang fp contract(fast)
+    float omega = 1.0f - 0.5f * rcp(tau_begin);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         float ex = (float)EX[k], ey = (float)EY[k];
