@@ -45,67 +45,84 @@ struct StepArgs {
     const double* patch_f;  // [node][2]
     int* flag;              // divergence flag
     int nbody;
+    int nt;                 // non-temporal stores
     Patch patch[IPPI_MAXB];
 };
 
 // ----------------------------------------------------------------- streaming
+// Loads use a wave-uniform base pointer (one population array, in SGPRs) and
+// a 32-bit per-lane element offset, so no 64-bit address arithmetic is done
+// per lane. ippi_create limits 9*nx*ny to < 2^32.
+template <typename T>
+__device__ __forceinline__ T ld(const T* __restrict__ base, unsigned idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(idx * (unsigned)sizeof(T)));
+}
+// Stores: while both population buffers fit in the 256 MB MALL (Infinity
+// Cache) the next step reads what this one writes, so the stores allocate
+// there; beyond that they stream past the caches (non-temporal).
+template <typename T>
+__device__ __forceinline__ void st(T* __restrict__ base, unsigned idx, T v, bool nt) {
+    T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(idx * (unsigned)sizeof(T)));
+    if (nt)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 // Pull of the post-collision populations into node (x, y), with the
-// boundary rules of the configured kind (not used for NEE columns).
-// 32-bit index math: ippi_create limits 9*nx*ny to < 2^32.
+// boundary rules of the configured kind (not used for NEE columns). Each
+// population's source (array, row, column offset) is chosen with
+// row-uniform selects, so the nine loads issue back to back with no branch
+// between them (y is uniform across a wave in every caller).
+//   IPPI_BC_PERIODIC_BB: periodic in x; halfway bounce-back below row 0 and
+//     above row ny-1 (the population leaving the node comes back reversed).
+//   IPPI_BC_INFLOW: rows 0 and ny-1 are specular free-slip rows
+//     (mesh.cpp:325-337 and its mirror image).
 template <typename T, int BC>
 __device__ __forceinline__ void pull(const T* __restrict__ s, int x, int y, int nx, int ny, unsigned N, T* f) {
-    const unsigned n = (unsigned)y * nx + x;
+    const bool bot = y == 0, top = y == ny - 1;
+    unsigned xc[3];  // lane columns for dx = -1, 0, +1
     if (BC == IPPI_BC_PERIODIC_BB) {
-        const unsigned xm = x == 0 ? nx - 1 : x - 1;
-        const unsigned xp = x == nx - 1 ? 0 : x + 1;
-        const unsigned r0 = (unsigned)y * nx, rm = r0 - nx, rp = r0 + nx;
-        const bool bot = y == 0, top = y == ny - 1;
-        // halfway bounce-back below row 0 / above row ny-1
-        const unsigned i2 = bot ? 4 * N + n : 2 * N + rm + x;
-        const unsigned i5 = bot ? 7 * N + n : 5 * N + rm + xm;
-        const unsigned i6 = bot ? 8 * N + n : 6 * N + rm + xp;
-        const unsigned i4 = top ? 2 * N + n : 4 * N + rp + x;
-        const unsigned i7 = top ? 5 * N + n : 7 * N + rp + xp;
-        const unsigned i8 = top ? 6 * N + n : 8 * N + rp + xm;
-        f[0] = s[n];
-        f[1] = s[1 * N + r0 + xm];
-        f[3] = s[3 * N + r0 + xp];
-        f[2] = s[i2];
-        f[4] = s[i4];
-        f[5] = s[i5];
-        f[6] = s[i6];
-        f[7] = s[i7];
-        f[8] = s[i8];
+        xc[0] = x == 0 ? nx - 1 : x - 1;
+        xc[1] = x;
+        xc[2] = x == nx - 1 ? 0 : x + 1;
     } else {
-        const unsigned r0 = (unsigned)y * nx;
-        f[0] = s[n];
-        f[1] = s[1 * N + r0 + x - 1];
-        f[3] = s[3 * N + r0 + x + 1];
-        if (y == 0) {  // specular free-slip row (mesh.cpp:325-337)
-            const unsigned rp = r0 + nx;
-            f[2] = s[4 * N + rp + x];
-            f[4] = s[4 * N + rp + x];
-            f[5] = s[8 * N + rp + x - 1];
-            f[6] = s[7 * N + rp + x + 1];
-            f[7] = s[7 * N + rp + x + 1];
-            f[8] = s[8 * N + rp + x - 1];
-        } else if (y == ny - 1) {  // mirror image for the top row
-            const unsigned rm = r0 - nx;
-            f[2] = s[2 * N + rm + x];
-            f[4] = s[2 * N + rm + x];
-            f[5] = s[5 * N + rm + x - 1];
-            f[6] = s[6 * N + rm + x + 1];
-            f[7] = s[6 * N + rm + x + 1];
-            f[8] = s[5 * N + rm + x - 1];
+        xc[0] = x - 1;
+        xc[1] = x;
+        xc[2] = x + 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int pk = k, dy = -EY[k], dx = -EX[k];
+        if (BC == IPPI_BC_PERIODIC_BB) {
+            if (EY[k] == 1 && bot) {  // source below row 0: bounce-back
+                pk = OPP[k];
+                dy = 0;
+                dx = 0;
+            }
+            if (EY[k] == -1 && top) {
+                pk = OPP[k];
+                dy = 0;
+                dx = 0;
+            }
         } else {
-            const unsigned rm = r0 - nx, rp = r0 + nx;
-            f[2] = s[2 * N + rm + x];
-            f[4] = s[4 * N + rp + x];
-            f[5] = s[5 * N + rm + x - 1];
-            f[6] = s[6 * N + rm + x + 1];
-            f[7] = s[7 * N + rp + x + 1];
-            f[8] = s[8 * N + rp + x - 1];
+            if (EY[k] == 1 && bot) {  // specular: k comes from its mirror k' one row up
+                pk = k == 2 ? 4 : (k == 5 ? 8 : 7);
+                dy = 1;
+            }
+            if (EY[k] == -1 && bot) {
+                dy = 1;
+            }
+            if (EY[k] == -1 && top) {
+                pk = k == 4 ? 2 : (k == 7 ? 6 : 5);
+                dy = -1;
+            }
+            if (EY[k] == 1 && top) {
+                dy = -1;
+            }
         }
+        const unsigned row = (unsigned)(y + dy) * nx;
+        f[k] = ld(s + (unsigned)pk * N, row + xc[dx + 1]);
     }
 }
 
@@ -114,10 +131,6 @@ __device__ __forceinline__ void pull(const T* __restrict__ s, int x, int y, int 
 // non-equilibrium extrapolation nodes (mesh.cpp:239-286): rho = 1, inlet
 // u = u_in, outlet u = u(nx-2), f = feq(1, u) + f_n - feq_n with n the
 // neighbouring column.
-struct Mom {
-    double rho, ux, uy;
-};
-
 __device__ __forceinline__ void node_state(const double* __restrict__ s, int x, int y, int nx, int ny, unsigned N,
                                            int bc, double uinx, double uiny, double* f, double& rho, double& ux,
                                            double& uy) {
@@ -144,30 +157,67 @@ __device__ __forceinline__ void node_state(const double* __restrict__ s, int x, 
     }
 }
 
-__device__ __forceinline__ void node_state(const float* __restrict__ s, int x, int y, int nx, int ny, unsigned N,
-                                           int bc, float uinx, float uiny, float* g, float& drho, float& rho,
-                                           float& ux, float& uy) {
+// FP32 node state: shifted populations g and their moments. `Mo` keeps the
+// partial sums shared by the moments, the stress and the MRT transform.
+struct Mo {
+    float drho, rho, ir, ux, uy, jx, jy;
+    float s13, s24, d13, d24, s57, s68, sx, sy;
+};
+
+__device__ __forceinline__ void moments32(const float* g, Mo& o) {
 #pragma clang fp contract(fast)
-    if (bc == IPPI_BC_PERIODIC_BB) {
+    o.s13 = g[1] + g[3];
+    o.s24 = g[2] + g[4];
+    o.d13 = g[1] - g[3];
+    o.d24 = g[2] - g[4];
+    o.s57 = g[5] + g[7];
+    o.s68 = g[6] + g[8];
+    const float d57 = g[5] - g[7];
+    o.sx = d57 + (g[8] - g[6]);  // g5 - g6 - g7 + g8
+    o.sy = d57 + (g[6] - g[8]);  // g5 + g6 - g7 - g8
+    o.drho = g[0] + ((o.s13 + o.s24) + (o.s57 + o.s68));
+    o.jx = o.d13 + o.sx;
+    o.jy = o.d24 + o.sy;
+    o.rho = 1.0f + o.drho;
+    if (o.rho < 1e-6f) {
+        o.ir = 0.f;
+        o.ux = 0.f;
+        o.uy = 0.f;
+    } else {
+        o.ir = sh::rcp(o.rho);
+        o.ux = o.jx * o.ir;
+        o.uy = o.jy * o.ir;
+    }
+}
+
+template <int BC>
+__device__ __forceinline__ void node_state32(const float* __restrict__ s, int x, int y, int nx, int ny, unsigned N,
+                                             float uinx, float uiny, float* g, Mo& o) {
+#pragma clang fp contract(fast)
+    if (BC == IPPI_BC_PERIODIC_BB) {
         pull<float, IPPI_BC_PERIODIC_BB>(s, x, y, nx, ny, N, g);
-        sh::macro(g, drho, rho, ux, uy);
+        moments32(g, o);
         return;
     }
     const bool nee = (x == 0) || (x == nx - 1);
     const int xs = x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x);
     pull<float, IPPI_BC_INFLOW>(s, xs, y, nx, ny, N, g);
-    sh::macro(g, drho, rho, ux, uy);
+    moments32(g, o);
     if (nee) {
-        float bx = x == 0 ? uinx : ux, by = x == 0 ? uiny : uy;
+        float bx = x == 0 ? uinx : o.ux, by = x == 0 ? uiny : o.uy;
         float qb[9], qn[9];
         sh::geq(0.f, 1.f, bx, by, qb);
-        sh::geq(drho, rho, ux, uy, qn);
+        sh::geq(o.drho, o.rho, o.ux, o.uy, qn);
 #pragma unroll
         for (int k = 0; k < 9; k++) g[k] = qb[k] + (g[k] - qn[k]);
-        drho = 0.f;
-        rho = 1.f;
-        ux = bx;
-        uy = by;
+        moments32(g, o);  // partial sums of the new populations
+        o.drho = 0.f;
+        o.rho = 1.f;
+        o.ir = 1.f;
+        o.ux = bx;
+        o.uy = by;
+        o.jx = bx;
+        o.jy = by;
     }
 }
 
@@ -190,25 +240,52 @@ __device__ __forceinline__ bool patch_force(const StepArgs& a, int x, int y, int
 
 constexpr int BX = 256;  // nodes per block along x (one row segment)
 
+// Rows per thread of the fused step kernels: each thread walks NPT rows of
+// one column and issues the loads of row j+1 before the arithmetic of row j,
+// so every wave keeps two rows of populations in flight.
+#ifndef IPPI_NPT
+#define IPPI_NPT 1
+#endif
+
 // ------------------------------------------------------------- FP64 step
-// PULL = false: collide the fnew state stored at the node itself with the
-// moments from the macro arrays (first collision after initialisation).
-template <int COLL, int BC, bool PULL>
-__global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src, double* __restrict__ dst,
-                                                 const double* __restrict__ mac, StepArgs a) {
-#pragma clang fp contract(off)
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y = blockIdx.y;
-    const int nx = a.nx, ny = a.ny;
-    if (x >= nx) return;
-    const unsigned N = (unsigned)nx * ny;
-    const unsigned n = (unsigned)y * nx + x;
-    double f[9], rho, ux, uy;
+// Raw pulled populations of node (x, y) (for IPPI_BC_INFLOW the NEE columns
+// pull the neighbouring column), or the stored fnew state when !PULL.
+template <typename T, int BC, bool PULL>
+__device__ __forceinline__ void load_raw(const T* __restrict__ src, int x, int y, int nx, int ny, unsigned N, T* f) {
     if (PULL) {
-        node_state(src, x, y, nx, ny, N, BC, a.uinx, a.uiny, f, rho, ux, uy);
+        const int xs = BC == IPPI_BC_INFLOW ? (x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x)) : x;
+        pull<T, BC>(src, xs, y, nx, ny, N, f);
     } else {
+        const unsigned n = (unsigned)y * nx + x;
 #pragma unroll
-        for (int k = 0; k < 9; k++) f[k] = src[k * N + n];
+        for (int k = 0; k < 9; k++) f[k] = ld(src + k * N, n);
+    }
+}
+
+// Collision of one node. The equilibrium is computed once and shared by the
+// LES and the BGK / RR collisions (the reference recomputes the identical
+// values); the MRT keeps the reference's moment-space evaluation.
+template <int COLL, int BC, bool PULL>
+__device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, const double* __restrict__ mac,
+                                         double* __restrict__ dst, const StepArgs& a) {
+#pragma clang fp contract(off)
+    const int nx = a.nx;
+    const unsigned n = (unsigned)y * nx + x;
+    double rho, ux, uy;
+    if (PULL) {
+        ref::macro(f, rho, ux, uy);
+        if (BC == IPPI_BC_INFLOW && (x == 0 || x == nx - 1)) {  // NEE, mesh.cpp:239-286
+            double bx = x == 0 ? a.uinx : ux, by = x == 0 ? a.uiny : uy;
+            double qb[9], qn[9];
+            ref::feq(1.0, bx, by, qb);
+            ref::feq(rho, ux, uy, qn);
+#pragma unroll
+            for (int k = 0; k < 9; k++) f[k] = qb[k] + f[k] - qn[k];
+            rho = 1.0;
+            ux = bx;
+            uy = by;
+        }
+    } else {
         rho = mac[n];
         ux = mac[N + n];
         uy = mac[2 * N + n];
@@ -224,16 +301,21 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
             uy = uy + 0.5 * 1 * fy / rho;
         }
     }
-    const double tb = a.tau_field ? static_cast<const double*>(a.tau_ptr)[n] : a.tau0;
+    const double tb = a.tau_field ? ld(static_cast<const double*>(a.tau_ptr), n) : a.tau0;
     double tau = tb;
-    if (a.les) tau = ref::les_tau(f, rho, ux, uy, tb, a.cs);
     double o[9];
-    if (COLL == IPPI_COLL_MRT)
+    if (COLL == IPPI_COLL_MRT) {
+        if (a.les) tau = ref::les_tau(f, rho, ux, uy, tb, a.cs);
         ref::mrt(f, rho, ux, uy, tau, o);
-    else if (COLL == IPPI_COLL_RR)
-        ref::rr(f, rho, ux, uy, tau, o);
-    else
-        ref::bgk(f, rho, ux, uy, tau, o);
+    } else {
+        double q[9];
+        ref::feq(rho, ux, uy, q);
+        if (a.les) tau = ref::les_tau_q(f, q, rho, tb, a.cs);
+        if (COLL == IPPI_COLL_RR)
+            ref::rr_q(f, q, tau, o);
+        else
+            ref::bgk_q(f, q, tau, o);
+    }
     if (forced) {
         double g[9];
         ref::guo(ux, uy, fx, fy, tb, g);
@@ -241,73 +323,178 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
         for (int k = 0; k < 9; k++) o[k] = o[k] + g[k];
     }
 #pragma unroll
-    for (int k = 0; k < 9; k++) __builtin_nontemporal_store(o[k], &dst[k * N + n]);
+    for (int k = 0; k < 9; k++) st(dst + k * N, n, o[k], a.nt);
 }
 
-// ------------------------------------------------------------- FP32 step
-template <int COLL, int BC, bool PULL>
-__global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
+// PULL = false: collide the fnew state stored at the node itself with the
+// moments from the macro arrays (first collision after initialisation).
+template <int COLL, int BC, bool PULL, int NPT>
+__global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src, double* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
-#pragma clang fp contract(fast)
     const int x = blockIdx.x * BX + threadIdx.x;
-    const int y = blockIdx.y;
+    const int y0 = blockIdx.y * NPT;
     const int nx = a.nx, ny = a.ny;
     if (x >= nx) return;
     const unsigned N = (unsigned)nx * ny;
-    const unsigned n = (unsigned)y * nx + x;
-    float g[9], drho, rho, ux, uy;
-    if (PULL) {
-        node_state(src, x, y, nx, ny, N, BC, a.uinxf, a.uinyf, g, drho, rho, ux, uy);
-    } else {
+    double cur[9];
+    load_raw<double, BC, PULL>(src, x, y0, nx, ny, N, cur);
 #pragma unroll
-        for (int k = 0; k < 9; k++) g[k] = src[k * N + n];
-        rho = (float)mac[n];
-        drho = (float)(mac[n] - 1.0);
-        ux = (float)mac[N + n];
-        uy = (float)mac[2 * N + n];
+    for (int j = 0; j < NPT; j++) {
+        const int y = y0 + j;
+        if (y >= ny) break;
+        const bool more = j + 1 < NPT && y + 1 < ny;
+        double nxt[9];
+        if (j + 1 < NPT) load_raw<double, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
+        node_f64<COLL, BC, PULL>(cur, x, y, N, mac, dst, a);
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) cur[k] = nxt[k];
+        }
     }
-    if (!(rho >= 0.f && rho <= 10.f)) *a.flag = 1;
-    double fxd = 0.0, fyd = 0.0;
-    bool forced = false;
+}
+
+// ------------------------------------------------------------- FP32 step
+// Shifted populations g = f - w. BGK and MRT are evaluated in moment space
+// (BGK = MRT with every non-conserved rate 1/tau): the conserved moments are
+// never touched, so FP32 rounding cannot drift mass or momentum. RR is
+// evaluated on populations with the increment projected onto exact mass and
+// momentum. The LES stress comes from the second moments directly.
+template <int COLL, int BC, bool PULL>
+__device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const double* __restrict__ mac,
+                                         float* __restrict__ dst, const StepArgs& a) {
+#pragma clang fp contract(fast)
+    const int nx = a.nx;
+    const unsigned n = (unsigned)y * nx + x;
+    Mo o;
+    moments32(g, o);
+    if (PULL) {
+        if (BC == IPPI_BC_INFLOW && (x == 0 || x == nx - 1)) {  // NEE, mesh.cpp:239-286
+            float bx = x == 0 ? a.uinxf : o.ux, by = x == 0 ? a.uinyf : o.uy;
+            float qb[9], qn[9];
+            sh::geq(0.f, 1.f, bx, by, qb);
+            sh::geq(o.drho, o.rho, o.ux, o.uy, qn);
+#pragma unroll
+            for (int k = 0; k < 9; k++) g[k] = qb[k] + (g[k] - qn[k]);
+            moments32(g, o);  // partial sums of the new populations
+            o.drho = 0.f;
+            o.rho = 1.f;
+            o.ir = 1.f;
+            o.ux = bx;
+            o.uy = by;
+        }
+    } else {
+        o.rho = (float)mac[n];
+        o.drho = (float)(mac[n] - 1.0);
+        o.ir = sh::rcp(o.rho);
+        o.ux = (float)mac[N + n];
+        o.uy = (float)mac[2 * N + n];
+    }
+    if (!(o.rho >= 0.f && o.rho <= 10.f)) *a.flag = 1;
     float fx = 0.f, fy = 0.f;
+    bool forced = false;
     if (a.nbody > 0) {
         const int bx0 = blockIdx.x * BX;
+        double fxd = 0.0, fyd = 0.0;
         forced = patch_force(a, x, y, bx0, bx0 + BX - 1, y, y, fxd, fyd);
         if (forced) {
             fx = (float)fxd;
             fy = (float)fyd;
-            float ir = 0.5f * sh::rcp(rho);
-            ux += fx * ir;
-            uy += fy * ir;
+            const float h = 0.5f * o.ir;
+            o.ux += fx * h;
+            o.uy += fy * h;
         }
     }
-    const float tb = a.tau_field ? static_cast<const float*>(a.tau_ptr)[n] : a.tau0f;
+    const float tb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tau0f;
     float tau = tb;
-    float q[9];
-    sh::geq(drho, rho, ux, uy, q);
-    if (COLL == IPPI_COLL_MRT) {
-        if (a.les) {
-            float pxx, pxy, pyy;
-            sh::neq_stress(g, q, pxx, pxy, pyy);
-            tau = sh::les_tau(pxx, pxy, pyy, rho, tb, a.csf);
-        }
-        sh::mrt(g, drho, rho, ux, uy, tau);
-    } else if (COLL == IPPI_COLL_RR) {
-        float pxx, pxy, pyy;
-        sh::neq_stress(g, q, pxx, pxy, pyy);
-        if (a.les) tau = sh::les_tau(pxx, pxy, pyy, rho, tb, a.csf);
-        sh::rr(g, q, pxx, pxy, pyy, tau, fx, fy);
+    const float rho = o.rho, ux = o.ux, uy = o.uy;
+    const float rux = rho * ux, ruy = rho * uy;
+    // non-equilibrium second moments: sum e e g - (rho u u + drho/3 I)
+    const float d3 = o.drho * (1.0f / 3.0f);
+    const float nxx = (o.s13 + (o.s57 + o.s68)) - (rux * ux + d3);
+    const float nyy = (o.s24 + (o.s57 + o.s68)) - (ruy * uy + d3);
+    const float nxy = (o.s57 - o.s68) - rux * uy;
+    if (a.les) tau = sh::les_tau(nxx, nxy, nyy, rho, tb, a.csf);
+    const float om = sh::rcp(tau);
+    if (COLL == IPPI_COLL_RR) {
+        float q[9];
+        sh::geq(o.drho, rho, ux, uy, q);
+        sh::rr(g, q, nxx, nxy, nyy, tau, fx, fy);
     } else {
-        if (a.les) {
-            float pxx, pxy, pyy;
-            sh::neq_stress(g, q, pxx, pxy, pyy);
-            tau = sh::les_tau(pxx, pxy, pyy, rho, tb, a.csf);
+        // moments of g (Lallemand-Luo basis, NodeAndLattice.cpp:18-27)
+        const float s4 = o.s13 + o.s24, s8 = o.s57 + o.s68;
+        const float m1 = -4.0f * g[0] - s4 + 2.0f * s8;
+        const float m2 = 4.0f * g[0] - 2.0f * s4 + s8;
+        const float m4 = -2.0f * o.d13 + o.sx;
+        const float m6 = -2.0f * o.d24 + o.sy;
+        const float m7 = o.s13 - o.s24;
+        const float m8 = o.s57 - o.s68;
+        const float t1 = ux * ux + uy * uy, t2 = ux * ux - uy * uy;
+        const float e1 = -2.0f * o.drho + 3.0f * rho * t1;
+        const float e2 = o.drho - 3.0f * rho * t1;
+        float r1, r2, r4;
+        if (COLL == IPPI_COLL_MRT) {  // NodeAndLattice.cpp:306
+            r1 = 1.2f;
+            r2 = 1.1f;
+            r4 = 1.2f;
+        } else {
+            r1 = om;
+            r2 = om;
+            r4 = om;
         }
-        sh::bgk(g, q, tau, fx, fy);
+        // c_l = s_l (m_l - meq_l) / |M_l|^2
+        const float c1 = r1 * (m1 - e1) * (1.0f / 36.0f);
+        const float c2 = r2 * (m2 - e2) * (1.0f / 36.0f);
+        const float c4 = r4 * (m4 + rux) * (1.0f / 12.0f);
+        const float c6 = r4 * (m6 + ruy) * (1.0f / 12.0f);
+        const float c7 = om * (m7 - rho * t2) * 0.25f;
+        const float c8 = om * (m8 - rux * uy) * 0.25f;
+        // BGK relaxes the momentum towards rho u (shifted by F/2 under IB
+        // forcing): exact target -omega F/2; MRT does not relax it.
+        float c3 = 0.f, c5 = 0.f;
+        if (COLL == IPPI_COLL_BGK) {
+            c3 = -0.5f * om * fx * (1.0f / 6.0f);
+            c5 = -0.5f * om * fy * (1.0f / 6.0f);
+        }
+        const float A = -c1 - 2.0f * c2, B = 2.0f * c1 + c2;
+        const float p46 = c4 + c6, q64 = c6 - c4;
+        g[0] -= 4.0f * (c2 - c1);
+        g[1] -= A - 2.0f * c4 + c7 + c3;
+        g[3] -= A + 2.0f * c4 + c7 - c3;
+        g[2] -= A - 2.0f * c6 - c7 + c5;
+        g[4] -= A + 2.0f * c6 - c7 - c5;
+        g[5] -= B + p46 + c8 + c3 + c5;
+        g[7] -= B - p46 + c8 - c3 - c5;
+        g[6] -= B + q64 - c8 - c3 + c5;
+        g[8] -= B - q64 - c8 + c3 - c5;
     }
     if (forced) sh::guo(ux, uy, fx, fy, tb, g);
 #pragma unroll
-    for (int k = 0; k < 9; k++) __builtin_nontemporal_store(g[k], &dst[k * N + n]);
+    for (int k = 0; k < 9; k++) st(dst + k * N, n, g[k], a.nt);
+}
+
+template <int COLL, int BC, bool PULL, int NPT>
+__global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
+                                                 const double* __restrict__ mac, StepArgs a) {
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y0 = blockIdx.y * NPT;
+    const int nx = a.nx, ny = a.ny;
+    if (x >= nx) return;
+    const unsigned N = (unsigned)nx * ny;
+    float cur[9];
+    load_raw<float, BC, PULL>(src, x, y0, nx, ny, N, cur);
+#pragma unroll
+    for (int j = 0; j < NPT; j++) {
+        const int y = y0 + j;
+        if (y >= ny) break;
+        const bool more = j + 1 < NPT && y + 1 < ny;
+        float nxt[9];
+        if (j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
+        node_f32<COLL, BC, PULL>(cur, x, y, N, mac, dst, a);
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) cur[k] = nxt[k];
+        }
+    }
 }
 
 // ----------------------------------------------------------- init kernels
@@ -359,8 +546,9 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
     if constexpr (sizeof(T) == 8) {
         node_state(src, x, y, nx, ny, N, BC, uinx, uiny, f, rho, ux, uy);
     } else {
-        float g[9], dr, r, vx, vy;
-        node_state(src, x, y, nx, ny, N, BC, (float)uinx, (float)uiny, g, dr, r, vx, vy);
+        float g[9];
+        Mo o;
+        node_state32<BC>(src, x, y, nx, ny, N, (float)uinx, (float)uiny, g, o);
         double s = 0.0, jx, jy;
         double gd[9];
 #pragma unroll
@@ -370,8 +558,8 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
         jy = ((gd[2] - gd[4]) + (gd[5] - gd[7])) + (gd[6] - gd[8]);
         const bool nee = BC == IPPI_BC_INFLOW && (x == 0 || x == nx - 1);
         rho = nee ? 1.0 : 1.0 + s;
-        ux = nee ? (double)vx : jx / rho;
-        uy = nee ? (double)vy : jy / rho;
+        ux = nee ? (double)o.ux : jx / rho;
+        uy = nee ? (double)o.uy : jy / rho;
 #pragma unroll
         for (int k = 0; k < 9; k++) f[k] = gd[k] + WD[k];
     }
@@ -380,126 +568,206 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
     mac[2 * N + n] = uy;
     if (f_out) {
 #pragma unroll
-        for (int k = 0; k < 9; k++) f_out[k * N + n] = f[k];
+        for (int k = 0; k < 9; k++) f_out[(size_t)k * N + n] = f[k];
     }
 }
 
-// ------------------------------------------------------------ IB kernels
-// InterpolateVel (4-point delta, mesh.cpp:555-605) + CalParticleForce
-// (DirectForce, mesh.cpp:514-532). One thread per marker; the 16 weights
-// phi(x - X) phi(y - Y) are precomputed on the host in the reference's
-// X-outer / Y-inner order. Velocities at the support nodes come from the
-// current state (pull) or from the macro arrays (first step).
-struct MarkerDev {
-    const int* base;    // [m][2]: XInt-1, YInt-1
-    const double* w;    // [m][16]
-    const double* ud;   // [m][2] desired velocity
-    double* ui;         // [m][2] interpolated velocity
-    double* F;          // [m][2] marker force
-    int nm;
+// ------------------------------------------------------------- IB kernel
+// One kernel per step for the immersed boundary, wired before the
+// collision:
+//   InterpolateVel (4-point delta, mesh.cpp:555-605)
+//   CalParticleForce (DirectForce, mesh.cpp:514-532)
+//   SpreadForce (4-point delta, mesh.cpp:659-741), as a gather
+// A body's "active" nodes are the union of its markers' 4x4 supports; they
+// are exactly the nodes the spread reaches. Each block serves one body and
+// a slice of its active nodes:
+//   A. the post-streaming velocity of every active node of the body goes to
+//      LDS (four nodes in flight per lane);
+//   B. one lane per marker sums u(X,Y) phi phi over its 16 support nodes in
+//      the reference's X-outer / Y-inner order (node indices and weights were
+//      prefetched before A), F = 2 (U_d - U_interp) to LDS;
+//   C. each active node of the slice sums its (marker, weight) list in
+//      increasing marker order, the order of the reference's marker loop.
+// The first block of a body writes the marker state and the body-force
+// record (minus the sum of the marker forces, fixed-order tree in LDS).
+constexpr int IB_T = 512;        // threads per IB block = active nodes per block
+constexpr int IB_MAXM = 1024;    // markers per body
+constexpr int IB_MAXACT = 3072;  // active nodes per body
+
+// Everything a block needs, fetched with one scalar load.
+struct alignas(16) IbBlk {
+    int body, a0, a1, m0;      // body, slice of active nodes [a0, a1), first marker
+    int M, act0, nact, lead;   // markers, first active node of the body, active count, first block of body
+    int x0, y0, w, pad;        // patch origin, width and offset in the patch numbering
 };
 
-// One thread per (marker, support node): 16 markers per 256-thread block.
-// Contributions u(X,Y) phi phi go to LDS and one thread per marker adds
-// them in the reference's X-outer / Y-inner order.
-constexpr int IB_MPB = 16;
+struct IbDev {
+    const double* w;     // [m][16] phi(x-X) phi(y-Y), X-outer / Y-inner
+    const int* pidx;     // [m][16] support node -> index in the body's active list
+    const double* ud;    // [m][2] desired velocity
+    double* ui;          // [m][2] interpolated velocity
+    double* F;           // [m][2] marker force
+    const IbBlk* blk;    // per block
+    const int* act;      // active node -> patch node (global patch numbering)
+    const int* ecnt;     // entries per active node
+    const int* emk;      // [active][E] marker index relative to the body's first marker
+    const double* ecw;   // [active][E] weight (0 in padding)
+    double* patch_f;     // [patch node][2]
+    double* hist;        // [body][2] of this step's record
+};
 
-template <typename T, int BC, bool PULL>
-__global__ void __launch_bounds__(256) k_ib_markers(const T* __restrict__ src, const double* __restrict__ mac,
-                                                    MarkerDev md, int nx, int ny) {
+template <typename T, bool PULL>
+__device__ __forceinline__ void support_u(const T* __restrict__ src, const double* __restrict__ mac, int X, int Y,
+                                          int nx, unsigned N, double& ux, double& uy) {
 #pragma clang fp contract(off)
-    __shared__ double cx[IB_MPB][16], cy[IB_MPB][16];
-    const int lm = threadIdx.x >> 4, sidx = threadIdx.x & 15;
-    const int m = blockIdx.x * IB_MPB + lm;
-    const unsigned N = (unsigned)nx * ny;
-    if (m < md.nm) {
-        const int X = md.base[2 * m] + (sidx >> 2), Y = md.base[2 * m + 1] + (sidx & 3);
-        const unsigned n = (unsigned)Y * nx + X;
-        double ux, uy;
-        if (PULL) {
-            if constexpr (sizeof(T) == 8) {
-                double f[9], r;
-                pull<double, BC>(src, X, Y, nx, ny, N, f);
-                ref::macro(f, r, ux, uy);
-            } else {
-                float g[9], dr, r, vx, vy;
-                pull<float, BC>(src, X, Y, nx, ny, N, g);
-                sh::macro(g, dr, r, vx, vy);
-                ux = vx;
-                uy = vy;
-            }
+    // supports lie in the interior, so both boundary kinds pull the plain stencil
+    const unsigned n = (unsigned)Y * nx + X;
+    if (PULL) {
+        T f[9];
+        const unsigned m = n - nx, p = n + nx;
+        f[0] = ld(src, n);
+        f[1] = ld(src + 1 * N, n - 1);
+        f[3] = ld(src + 3 * N, n + 1);
+        f[2] = ld(src + 2 * N, m);
+        f[4] = ld(src + 4 * N, p);
+        f[5] = ld(src + 5 * N, m - 1);
+        f[6] = ld(src + 6 * N, m + 1);
+        f[7] = ld(src + 7 * N, p + 1);
+        f[8] = ld(src + 8 * N, p - 1);
+        if constexpr (sizeof(T) == 8) {
+            double r;
+            ref::macro(f, r, ux, uy);
         } else {
-            if constexpr (sizeof(T) == 8) {
-                ux = mac[N + n];
-                uy = mac[2 * N + n];
-            } else {
-                ux = (double)(float)mac[N + n];
-                uy = (double)(float)mac[2 * N + n];
+            Mo o;
+            moments32(f, o);
+            ux = o.ux;
+            uy = o.uy;
+        }
+    } else {
+        if constexpr (sizeof(T) == 8) {
+            ux = mac[N + n];
+            uy = mac[2 * N + n];
+        } else {
+            ux = (double)(float)mac[N + n];
+            uy = (double)(float)mac[2 * N + n];
+        }
+    }
+}
+
+template <typename T, bool PULL, int E>
+__global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const double* __restrict__ mac, IbDev d,
+                                             int nx, int ny) {
+#pragma clang fp contract(off)
+    __shared__ double U[2 * IB_MAXACT];
+    __shared__ double FB[2 * IB_MAXM];
+    const IbBlk bk = d.blk[blockIdx.x];
+    const int t = threadIdx.x;
+    const unsigned N = (unsigned)nx * ny;
+    // prefetch everything that does not depend on the populations
+    const int mt = min(t, bk.M - 1);  // this lane's first marker (clamped)
+    int pi[16];
+    double pw[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        pi[s] = d.pidx[16 * (bk.m0 + mt) + s];
+        pw[s] = d.w[16 * (bk.m0 + mt) + s];
+    }
+    const double udx = d.ud[2 * (bk.m0 + mt)], udy = d.ud[2 * (bk.m0 + mt) + 1];
+    const int ia = min(bk.a0 + t, bk.a1 - 1);  // this lane's active node of the slice (clamped)
+    const int pn = d.act[ia];
+    const int cnt = d.ecnt[ia];
+    int em[E];
+    double ew[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        em[e] = d.emk[(size_t)E * ia + e];
+        ew[e] = d.ecw[(size_t)E * ia + e];
+    }
+    // A. velocity at every active node of the body, four nodes in flight per lane
+    for (int c0 = 0; c0 < bk.nact; c0 += 4 * IB_T) {
+        int nd[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) nd[j] = d.act[bk.act0 + min(c0 + j * IB_T + t, bk.nact - 1)];
+        double ux[4], uy[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = nd[j] - bk.pad;
+            support_u<T, PULL>(src, mac, bk.x0 + r % bk.w, bk.y0 + r / bk.w, nx, N, ux[j], uy[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int i = c0 + j * IB_T + t;
+            if (i < bk.nact) {
+                U[2 * i] = ux[j];
+                U[2 * i + 1] = uy[j];
             }
         }
-        const double D = md.w[16 * m + sidx];
-        cx[lm][sidx] = (ux * D);
-        cy[lm][sidx] = (uy * D);
     }
     __syncthreads();
-    if (m < md.nm && sidx == 0) {
+    // B. interpolation and direct forcing, one lane per marker
+    for (int ml = t; ml < bk.M; ml += IB_T) {
+        double ux0 = udx, uy0 = udy;
+        if (ml != t) {
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                pi[s] = d.pidx[16 * (bk.m0 + ml) + s];
+                pw[s] = d.w[16 * (bk.m0 + ml) + s];
+            }
+            ux0 = d.ud[2 * (bk.m0 + ml)];
+            uy0 = d.ud[2 * (bk.m0 + ml) + 1];
+        }
         double uix = 0, uiy = 0;
 #pragma unroll
         for (int s = 0; s < 16; s++) {
-            uix += cx[lm][s];
-            uiy += cy[lm][s];
+            uix += (U[2 * pi[s]] * pw[s]);
+            uiy += (U[2 * pi[s] + 1] * pw[s]);
         }
-        md.ui[2 * m] = uix;
-        md.ui[2 * m + 1] = uiy;
-        md.F[2 * m] = 2 * (md.ud[2 * m] - uix);
-        md.F[2 * m + 1] = 2 * (md.ud[2 * m + 1] - uiy);
-    }
-}
-
-// SpreadForce (mesh.cpp:659-741) as a gather: each patch node sums its
-// (marker, weight) list in increasing marker order, the order of the
-// reference's marker loop. Block 0 also writes the per-body force record
-// (minus the sum of the marker forces, in marker order), staging the
-// marker forces through LDS in chunks.
-constexpr int IB_CHUNK = 2048;
-
-__global__ void __launch_bounds__(256) k_ib_spread(const int* __restrict__ ptr, const int* __restrict__ mk,
-                                                   const double* __restrict__ w, const double* __restrict__ F,
-                                                   double* __restrict__ patch_f, int npn,
-                                                   const int* __restrict__ body_first, int nbody, int nm,
-                                                   double* __restrict__ hist) {
-#pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < npn) {
-        double fx = 0.0, fy = 0.0;
-        for (int e = ptr[i]; e < ptr[i + 1]; e++) {
-            const int m = mk[e];
-            fx += (F[2 * m] * w[e]);
-            fy += (F[2 * m + 1] * w[e]);
+        const double Fx = 2 * (ux0 - uix), Fy = 2 * (uy0 - uiy);
+        FB[2 * ml] = Fx;
+        FB[2 * ml + 1] = Fy;
+        if (bk.lead) {
+            const int m = bk.m0 + ml;
+            d.ui[2 * m] = uix;
+            d.ui[2 * m + 1] = uiy;
+            d.F[2 * m] = Fx;
+            d.F[2 * m + 1] = Fy;
         }
-        patch_f[2 * i] = fx;
-        patch_f[2 * i + 1] = fy;
     }
-    if (hist && blockIdx.x == 0) {
-        __shared__ double sf[2 * IB_CHUNK];
-        const int b = threadIdx.x;
+    __syncthreads();
+    // C. spread (gather) for this block's slice of active nodes
+    if (bk.a0 + t < bk.a1) {
         double fx = 0.0, fy = 0.0;
-        const int b0 = b < nbody ? body_first[b] : 0, b1 = b < nbody ? body_first[b + 1] : 0;
-        for (int c0 = 0; c0 < nm; c0 += IB_CHUNK) {
-            const int c1 = min(nm, c0 + IB_CHUNK);
-            __syncthreads();
-            for (int j = c0 * 2 + (int)threadIdx.x; j < c1 * 2; j += blockDim.x) sf[j - 2 * c0] = F[j];
-            __syncthreads();
-            if (b < nbody) {
-                for (int m = max(b0, c0); m < min(b1, c1); m++) {
-                    fx += sf[2 * (m - c0)];
-                    fy += sf[2 * (m - c0) + 1];
-                }
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            if (e < cnt) {
+                fx += (FB[2 * em[e]] * ew[e]);
+                fy += (FB[2 * em[e] + 1] * ew[e]);
             }
         }
-        if (b < nbody) {
-            hist[2 * b] = -fx;
-            hist[2 * b + 1] = -fy;
+        d.patch_f[2 * pn] = fx;
+        d.patch_f[2 * pn + 1] = fy;
+    }
+    if (bk.lead && d.hist) {
+        // fixed-order tree sum of the marker forces (reuses U)
+        __syncthreads();
+        double sx = 0.0, sy = 0.0;
+        for (int m = t; m < bk.M; m += IB_T) {
+            sx += FB[2 * m];
+            sy += FB[2 * m + 1];
+        }
+        U[2 * t] = sx;
+        U[2 * t + 1] = sy;
+        __syncthreads();
+        for (int off = IB_T / 2; off > 0; off >>= 1) {
+            if (t < off) {
+                U[2 * t] += U[2 * (t + off)];
+                U[2 * t + 1] += U[2 * (t + off) + 1];
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            d.hist[2 * bk.body] = -U[0];
+            d.hist[2 * bk.body + 1] = -U[1];
         }
     }
 }
@@ -547,9 +815,11 @@ struct ippi_handle {
     std::vector<int> mk_base;
     std::vector<double> mk_w;
     bool ib_dirty = false;
-    int *d_base = nullptr, *d_ptr = nullptr, *d_mk = nullptr, *d_bfirst = nullptr;
-    double *d_w = nullptr, *d_ud = nullptr, *d_ui = nullptr, *d_F = nullptr, *d_cw = nullptr, *d_pf = nullptr;
-    int npn = 0;
+    int *d_pidx = nullptr, *d_ecnt = nullptr, *d_emk = nullptr, *d_act = nullptr;
+    IbBlk* d_blk = nullptr;
+    int ell = 8;
+    double *d_w = nullptr, *d_ud = nullptr, *d_ui = nullptr, *d_F = nullptr, *d_ecw = nullptr, *d_pf = nullptr;
+    int npn = 0, nact = 0, nblk_ib = 0;
     double* hist = nullptr;
     int hist_cap = 0;
     int64_t nrec = 0;
@@ -596,9 +866,9 @@ StepKernel<T> pick_step(int coll, int bc) {
 #define IPPI_PICK(C, B)                                                                   \
     if (coll == C && bc == B) {                                                           \
         if constexpr (sizeof(T) == 8)                                                     \
-            return (StepKernel<T>)k_step_f64<C, B, PULL>;                                 \
+            return (StepKernel<T>)k_step_f64<C, B, PULL, PULL ? IPPI_NPT : 1>;            \
         else                                                                              \
-            return (StepKernel<T>)k_step_f32<C, B, PULL>;                                 \
+            return (StepKernel<T>)k_step_f32<C, B, PULL, PULL ? IPPI_NPT : 1>;            \
     }
     IPPI_PICK(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
     IPPI_PICK(IPPI_COLL_MRT, IPPI_BC_PERIODIC_BB)
@@ -628,72 +898,102 @@ StepArgs make_args(ippi_handle* h) {
     a.patch_f = h->d_pf;
     a.flag = h->flag;
     a.nbody = (int)h->bodies.size();
+    a.nt = 2 * 9 * h->N * h->esz > ((size_t)192 << 20);
     for (size_t b = 0; b < h->bodies.size(); b++) a.patch[b] = h->bodies[b].p;
     return a;
 }
 
-// Upload marker weights and the spread lists after bodies were added.
+// Upload marker weights, the active patch nodes with their gather lists and
+// the IB block table after bodies were added.
+void free_ib(ippi_handle* h) {
+    dfree(h->d_pidx); dfree(h->d_ecnt); dfree(h->d_emk); dfree(h->d_act); dfree(h->d_blk);
+    dfree(h->d_w); dfree(h->d_ud); dfree(h->d_ui); dfree(h->d_F); dfree(h->d_ecw); dfree(h->d_pf);
+    dfree(h->hist);
+}
+
 int upload_ib(ippi_handle* h) {
     if (!h->ib_dirty) return IPPI_OK;
     const int nm = (int)(h->mk_ud.size() / 2);
     const int nb = (int)h->bodies.size();
-    // patch layout and gather lists
     int off = 0;
     for (auto& b : h->bodies) {
         b.p.off = off;
         off += b.p.w * b.p.h;
     }
     h->npn = off;
+    // gather lists per patch node, markers in increasing order
     std::vector<std::vector<std::pair<int, double>>> rows(off);
+    size_t emax = 0;
     for (int bi = 0; bi < nb; bi++) {
         const auto& b = h->bodies[bi];
-        for (int m = b.first; m < b.first + b.count; m++) {
+        for (int m = b.first; m < b.first + b.count; m++)
             for (int i = 0; i < 4; i++)
                 for (int j = 0; j < 4; j++) {
                     const int X = h->mk_base[2 * m] + i, Y = h->mk_base[2 * m + 1] + j;
                     const int r = b.p.off + (Y - b.p.y0) * b.p.w + (X - b.p.x0);
-                    rows[r].push_back({m, h->mk_w[16 * m + 4 * i + j]});
+                    rows[r].push_back({m - b.first, h->mk_w[16 * m + 4 * i + j]});
+                    emax = std::max(emax, rows[r].size());
                 }
-        }
     }
-    std::vector<int> ptr(off + 1, 0), mk;
-    std::vector<double> cw;
-    for (int r = 0; r < off; r++) {
-        for (auto& e : rows[r]) {
-            mk.push_back(e.first);
-            cw.push_back(e.second);
+    if (emax > 16) return fail(h, IPPI_ERR_ARG, "more than 16 markers reach one node (markers too dense)");
+    const int E = emax <= 8 ? 8 : 16;
+    h->ell = E;
+    std::vector<int> act, ecnt, emk, pidx(16 * (size_t)nm), slot(off, -1);
+    std::vector<double> ecw;
+    std::vector<IbBlk> blk;
+    for (int bi = 0; bi < nb; bi++) {
+        const auto& b = h->bodies[bi];
+        const int a0 = (int)act.size();
+        for (int r = b.p.off; r < b.p.off + b.p.w * b.p.h; r++) {
+            if (rows[r].empty()) continue;
+            slot[r] = (int)act.size() - a0;
+            act.push_back(r);
+            ecnt.push_back((int)rows[r].size());
+            for (int e = 0; e < E; e++) {
+                const bool v = e < (int)rows[r].size();
+                emk.push_back(v ? rows[r][e].first : 0);
+                ecw.push_back(v ? rows[r][e].second : 0.0);
+            }
         }
-        ptr[r + 1] = (int)mk.size();
+        const int a1 = (int)act.size();
+        if (a1 - a0 > IB_MAXACT) return fail(h, IPPI_ERR_ARG, "body support too large (active nodes > 3072)");
+        for (int m = b.first; m < b.first + b.count; m++)
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    const int X = h->mk_base[2 * m] + i, Y = h->mk_base[2 * m + 1] + j;
+                    pidx[16 * m + 4 * i + j] = slot[b.p.off + (Y - b.p.y0) * b.p.w + (X - b.p.x0)];
+                }
+        for (int s = a0; s < a1; s += IB_T)
+            blk.push_back(IbBlk{bi, s, std::min(a1, s + IB_T), b.first, b.count, a0, a1 - a0, s == a0 ? 1 : 0,
+                                b.p.x0, b.p.y0, b.p.w, b.p.off});
     }
-    std::vector<int> bfirst(nb + 1);
-    for (int bi = 0; bi < nb; bi++) bfirst[bi] = h->bodies[bi].first;
-    bfirst[nb] = nm;
-    dfree(h->d_base); dfree(h->d_ptr); dfree(h->d_mk); dfree(h->d_bfirst);
-    dfree(h->d_w); dfree(h->d_ud); dfree(h->d_ui); dfree(h->d_F); dfree(h->d_cw); dfree(h->d_pf);
-    dfree(h->hist);
-    HIPC(h, hipMalloc(&h->d_base, sizeof(int) * 2 * nm));
+    h->nact = (int)act.size();
+    h->nblk_ib = (int)blk.size();
+    free_ib(h);
+    HIPC(h, hipMalloc(&h->d_pidx, sizeof(int) * 16 * nm));
     HIPC(h, hipMalloc(&h->d_w, sizeof(double) * 16 * nm));
     HIPC(h, hipMalloc(&h->d_ud, sizeof(double) * 2 * nm));
     HIPC(h, hipMalloc(&h->d_ui, sizeof(double) * 2 * nm));
     HIPC(h, hipMalloc(&h->d_F, sizeof(double) * 2 * nm));
-    HIPC(h, hipMalloc(&h->d_ptr, sizeof(int) * (off + 1)));
-    HIPC(h, hipMalloc(&h->d_mk, sizeof(int) * std::max<size_t>(1, mk.size())));
-    HIPC(h, hipMalloc(&h->d_cw, sizeof(double) * std::max<size_t>(1, cw.size())));
-    HIPC(h, hipMalloc(&h->d_pf, sizeof(double) * 2 * std::max(1, off)));
-    HIPC(h, hipMalloc(&h->d_bfirst, sizeof(int) * (nb + 1)));
+    HIPC(h, hipMalloc(&h->d_act, sizeof(int) * act.size()));
+    HIPC(h, hipMalloc(&h->d_ecnt, sizeof(int) * ecnt.size()));
+    HIPC(h, hipMalloc(&h->d_emk, sizeof(int) * emk.size()));
+    HIPC(h, hipMalloc(&h->d_ecw, sizeof(double) * ecw.size()));
+    HIPC(h, hipMalloc(&h->d_pf, sizeof(double) * 2 * off));
+    HIPC(h, hipMalloc(&h->d_blk, sizeof(IbBlk) * blk.size()));
     h->hist_cap = h->c.force_history > 0 ? h->c.force_history : 65536;
     HIPC(h, hipMalloc(&h->hist, sizeof(double) * 2 * (size_t)nb * h->hist_cap));
-    HIPC(h, hipMemcpy(h->d_base, h->mk_base.data(), sizeof(int) * 2 * nm, hipMemcpyHostToDevice));
+    HIPC(h, hipMemcpy(h->d_pidx, pidx.data(), sizeof(int) * 16 * nm, hipMemcpyHostToDevice));
     HIPC(h, hipMemcpy(h->d_w, h->mk_w.data(), sizeof(double) * 16 * nm, hipMemcpyHostToDevice));
     HIPC(h, hipMemcpy(h->d_ud, h->mk_ud.data(), sizeof(double) * 2 * nm, hipMemcpyHostToDevice));
     HIPC(h, hipMemset(h->d_ui, 0, sizeof(double) * 2 * nm));
     HIPC(h, hipMemset(h->d_F, 0, sizeof(double) * 2 * nm));
-    HIPC(h, hipMemcpy(h->d_ptr, ptr.data(), sizeof(int) * (off + 1), hipMemcpyHostToDevice));
-    if (!mk.empty()) {
-        HIPC(h, hipMemcpy(h->d_mk, mk.data(), sizeof(int) * mk.size(), hipMemcpyHostToDevice));
-        HIPC(h, hipMemcpy(h->d_cw, cw.data(), sizeof(double) * cw.size(), hipMemcpyHostToDevice));
-    }
-    HIPC(h, hipMemcpy(h->d_bfirst, bfirst.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+    HIPC(h, hipMemset(h->d_pf, 0, sizeof(double) * 2 * off));  // inactive patch nodes stay zero
+    HIPC(h, hipMemcpy(h->d_act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice));
+    HIPC(h, hipMemcpy(h->d_ecnt, ecnt.data(), sizeof(int) * ecnt.size(), hipMemcpyHostToDevice));
+    HIPC(h, hipMemcpy(h->d_emk, emk.data(), sizeof(int) * emk.size(), hipMemcpyHostToDevice));
+    HIPC(h, hipMemcpy(h->d_ecw, ecw.data(), sizeof(double) * ecw.size(), hipMemcpyHostToDevice));
+    HIPC(h, hipMemcpy(h->d_blk, blk.data(), sizeof(IbBlk) * blk.size(), hipMemcpyHostToDevice));
     h->nrec = 0;
     h->ib_dirty = false;
     return IPPI_OK;
@@ -701,28 +1001,23 @@ int upload_ib(ippi_handle* h) {
 
 template <typename T>
 int launch_ib(ippi_handle* h, bool pull) {
-    const int nm = (int)(h->mk_ud.size() / 2);
-    MarkerDev md{h->d_base, h->d_w, h->d_ud, h->d_ui, h->d_F, nm};
+    const int nb = (int)h->bodies.size();
+    IbDev d{h->d_w, h->d_pidx, h->d_ud, h->d_ui, h->d_F, h->d_blk, h->d_act, h->d_ecnt, h->d_emk, h->d_ecw,
+            h->d_pf, h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap)};
     const T* src = (const T*)h->buf[h->cur];
+    dim3 g(h->nblk_ib), b(IB_T);
     const int nx = h->c.nx, ny = h->c.ny;
-    dim3 gm((nm + IB_MPB - 1) / IB_MPB);
-    if (h->c.bc == IPPI_BC_PERIODIC_BB) {
+    if (h->ell == 8) {
         if (pull)
-            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_PERIODIC_BB, true>), gm, dim3(256), 0, h->s, src, h->mac, md, nx, ny);
+            hipLaunchKernelGGL((k_ib<T, true, 8>), g, b, 0, h->s, src, h->mac, d, nx, ny);
         else
-            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_PERIODIC_BB, false>), gm, dim3(256), 0, h->s, src, h->mac, md, nx, ny);
+            hipLaunchKernelGGL((k_ib<T, false, 8>), g, b, 0, h->s, src, h->mac, d, nx, ny);
     } else {
         if (pull)
-            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_INFLOW, true>), gm, dim3(256), 0, h->s, src, h->mac, md, nx, ny);
+            hipLaunchKernelGGL((k_ib<T, true, 16>), g, b, 0, h->s, src, h->mac, d, nx, ny);
         else
-            hipLaunchKernelGGL((k_ib_markers<T, IPPI_BC_INFLOW, false>), gm, dim3(256), 0, h->s, src, h->mac, md, nx, ny);
+            hipLaunchKernelGGL((k_ib<T, false, 16>), g, b, 0, h->s, src, h->mac, d, nx, ny);
     }
-    HIPC(h, hipGetLastError());
-    const int nb = (int)h->bodies.size();
-    double* rec = h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap);
-    dim3 gs((std::max(h->npn, 1) + 255) / 256);
-    hipLaunchKernelGGL(k_ib_spread, gs, dim3(256), 0, h->s, h->d_ptr, h->d_mk, h->d_cw, h->d_F, h->d_pf, h->npn,
-                       h->d_bfirst, nb, nm, rec);
     HIPC(h, hipGetLastError());
     h->nrec++;
     return IPPI_OK;
@@ -744,7 +1039,7 @@ void clear_events(std::vector<hipEvent_t>& v) {
 template <typename T>
 int run_steps(ippi_handle* h, int nsteps) {
     const int nx = h->c.nx, ny = h->c.ny;
-    dim3 grid((nx + BX - 1) / BX, ny);
+    const dim3 grid_pull((nx + BX - 1) / BX, (ny + IPPI_NPT - 1) / IPPI_NPT), grid_first((nx + BX - 1) / BX, ny);
     const bool ib = !h->bodies.empty();
     if (ib) {
         int r = upload_ib(h);
@@ -762,7 +1057,8 @@ int run_steps(ippi_handle* h, int nsteps) {
         StepKernel<T> k = pull ? pick_step<T, true>(h->c.collision, h->c.bc) : pick_step<T, false>(h->c.collision, h->c.bc);
         if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
         if (h->timing && pull) h->ev_step.push_back(new_event(h));
-        hipLaunchKernelGGL(k, grid, dim3(BX), 0, h->s, (const T*)h->buf[h->cur], (T*)h->buf[h->cur ^ 1], h->mac, a);
+        hipLaunchKernelGGL(k, pull ? grid_pull : grid_first, dim3(BX), 0, h->s, (const T*)h->buf[h->cur],
+                           (T*)h->buf[h->cur ^ 1], h->mac, a);
         HIPC(h, hipGetLastError());
         if (h->timing && pull) h->ev_step.push_back(new_event(h));
         h->cur ^= 1;
@@ -851,9 +1147,7 @@ int ippi_destroy(ippi_handle* h) {
     clear_events(h->ev_step);
     clear_events(h->ev_ib);
     dfree(h->buf[0]); dfree(h->buf[1]); dfree(h->mac); dfree(h->tau); dfree(h->flag); dfree(h->part);
-    dfree(h->d_base); dfree(h->d_ptr); dfree(h->d_mk); dfree(h->d_bfirst);
-    dfree(h->d_w); dfree(h->d_ud); dfree(h->d_ui); dfree(h->d_F); dfree(h->d_cw); dfree(h->d_pf);
-    dfree(h->hist);
+    free_ib(h);
     if (h->s) (void)hipStreamDestroy(h->s);
     delete h;
     return IPPI_OK;
@@ -894,6 +1188,7 @@ int ippi_set_fields(ippi_handle* h, const double* rho, const double* ux, const d
 
 int ippi_add_body(ippi_handle* h, double cx, double cy, double diameter, int M, double udx, double udy, int* body) {
     if (!h || M <= 0 || !(diameter > 0)) return IPPI_ERR_ARG;
+    if (M > IB_MAXM) return fail(h, IPPI_ERR_ARG, "at most 1024 markers per body");
     if ((int)h->bodies.size() >= IPPI_MAXB) return fail(h, IPPI_ERR_ARG, "too many bodies");
     const double R = 0.5 * diameter;
     ippi_handle::Body b{};

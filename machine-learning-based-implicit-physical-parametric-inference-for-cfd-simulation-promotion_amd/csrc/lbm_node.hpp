@@ -132,14 +132,17 @@ IPPI_HD void macro(const double* f, double& rho, double& ux, double& uy) {
     uy = ty;
 }
 
-// ComputeBGKCollision, NodeAndLattice.cpp:150-161
-IPPI_HD void bgk(const double* f, double rho, double ux, double uy, double tau, double* o) {
+// ComputeBGKCollision, NodeAndLattice.cpp:150-161, given q = feq(rho, u)
+IPPI_HD void bgk_q(const double* f, const double* q, double tau, double* o) {
 #pragma clang fp contract(off)
-    double q[9];
-    feq(rho, ux, uy, q);
     double a = (1.0 - 1.0 / tau);
 #pragma unroll
     for (int k = 0; k < 9; k++) o[k] = a * f[k] + q[k] / tau;
+}
+IPPI_HD void bgk(const double* f, double rho, double ux, double uy, double tau, double* o) {
+    double q[9];
+    feq(rho, ux, uy, q);
+    bgk_q(f, q, tau, o);
 }
 
 // ComputeMRTCollision, NodeAndLattice.cpp:290-360. Zero coefficients are
@@ -192,11 +195,9 @@ IPPI_HD void mrt(const double* f, double rho, double ux, double uy, double tau, 
     }
 }
 
-// ComputeRRCollision, NodeAndLattice.cpp:228-286
-IPPI_HD void rr(const double* f, double rho, double ux, double uy, double tau, double* o) {
+// ComputeRRCollision, NodeAndLattice.cpp:228-286, given q = feq(rho, u)
+IPPI_HD void rr_q(const double* f, const double* q, double tau, double* o) {
 #pragma clang fp contract(off)
-    double q[9];
-    feq(rho, ux, uy, q);
     // second moments of f and feq, reference loop order with e = 0 terms skipped
     double xx = ((((f[1] + f[3]) + f[5]) + f[6]) + f[7]) + f[8];
     double txx = ((((q[1] + q[3]) + q[5]) + q[6]) + q[7]) + q[8];
@@ -223,6 +224,12 @@ IPPI_HD void rr(const double* f, double rho, double ux, double uy, double tau, d
     }
 }
 
+IPPI_HD void rr(const double* f, double rho, double ux, double uy, double tau, double* o) {
+    double q[9];
+    feq(rho, ux, uy, q);
+    rr_q(f, q, tau, o);
+}
+
 // Compute_Disforce (Guo term), NodeAndLattice.cpp:162-195
 IPPI_HD void guo(double ux, double uy, double fx, double fy, double tau_begin, double* o) {
 #pragma clang fp contract(off)
@@ -237,11 +244,9 @@ IPPI_HD void guo(double ux, double uy, double fx, double fy, double tau_begin, d
     }
 }
 
-// CaculateLESTau live body, mesh.cpp:877-910
-IPPI_HD double les_tau(const double* f, double rho, double ux, double uy, double tau0, double cs) {
+// CaculateLESTau live body, mesh.cpp:877-910, given q = feq(rho, u)
+IPPI_HD double les_tau_q(const double* f, const double* q, double rho, double tau0, double cs) {
 #pragma clang fp contract(off)
-    double q[9];
-    feq(rho, ux, uy, q);
     double d[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) d[k] = f[k] - q[k];
@@ -257,6 +262,11 @@ IPPI_HD double les_tau(const double* f, double rho, double ux, double uy, double
     double Q = sqrt(2.0 * S2);
     double g1 = tau0 * tau0 + 18.0 * cs * cs * Q / (rho);
     return (sqrt(g1) + tau0) / 2.0;
+}
+IPPI_HD double les_tau(const double* f, double rho, double ux, double uy, double tau0, double cs) {
+    double q[9];
+    feq(rho, ux, uy, q);
+    return les_tau_q(f, q, rho, tau0, cs);
 }
 
 }  // namespace ref
@@ -318,17 +328,6 @@ IPPI_HD void geq(float drho, float rho, float ux, float uy, float* q) {
     q[6] = WF[6] * (drho + ad - bd);
 }
 
-IPPI_HD void neq_stress(const float* g, const float* q, float& pxx, float& pxy, float& pyy) {
-#pragma clang fp contract(fast)
-    float d[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) d[k] = g[k] - q[k];
-    float c = (d[5] + d[6]) + (d[7] + d[8]);
-    pxx = (d[1] + d[3]) + c;
-    pyy = (d[2] + d[4]) + c;
-    pxy = (d[5] - d[6]) + (d[7] - d[8]);
-}
-
 IPPI_HD float les_tau(float pxx, float pxy, float pyy, float rho, float tau0, float cs) {
 #pragma clang fp contract(fast)
     float ir = rcp(rho);
@@ -358,18 +357,6 @@ IPPI_HD void conserve(float* d, float tx, float ty) {
     for (int k = 0; k < 9; k++) d[k] -= WF[k] * (m + (float)EX[k] * jx + (float)EY[k] * jy);
 }
 
-// BGK; (fx, fy) is the immersed-boundary force already folded into u
-IPPI_HD void bgk(float* g, const float* q, float tau, float fx, float fy) {
-#pragma clang fp contract(fast)
-    float om = rcp(tau);
-    float d[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) d[k] = om * (q[k] - g[k]);
-    conserve(d, 0.5f * om * fx, 0.5f * om * fy);
-#pragma unroll
-    for (int k = 0; k < 9; k++) g[k] += d[k];
-}
-
 IPPI_HD void rr(float* g, const float* q, float pxx, float pxy, float pyy, float tau, float fx, float fy) {
 #pragma clang fp contract(fast)
     float a = 1.0f - rcp(tau);
@@ -386,48 +373,6 @@ IPPI_HD void rr(float* g, const float* q, float pxx, float pxy, float pyy, float
     conserve(d, 0.5f * fx, 0.5f * fy);
 #pragma unroll
     for (int k = 0; k < 9; k++) g[k] += d[k];
-}
-
-// MRT on shifted populations: M w = (1, -2, 1, 0, ...) so the shifted
-// moment equilibria are meq(rho, u) - M w.
-IPPI_HD void mrt(float* g, float drho, float rho, float ux, float uy, float tau) {
-#pragma clang fp contract(fast)
-    float m[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        float s = 0.f;
-#pragma unroll
-        for (int l = 0; l < 9; l++)
-            if (TM[k][l] != 0.0) s += (float)TM[k][l] * g[l];
-        m[k] = s;
-    }
-    float t1 = ux * ux + uy * uy, t2 = ux * ux - uy * uy;
-    float eq[9];
-    eq[1] = -2.0f * drho + 3.0f * rho * t1;
-    eq[2] = drho - 3.0f * rho * t1;
-    eq[4] = -rho * ux;
-    eq[6] = -rho * uy;
-    eq[7] = rho * t2;
-    eq[8] = rho * ux * uy;
-    float s78 = rcp(tau);
-    float dm[9];
-#pragma unroll
-    for (int l = 0; l < 9; l++) {
-        if (SM_FIXED[l] == 0.0) {
-            dm[l] = 0.f;
-        } else {
-            float sl = SM_FIXED[l] < 0 ? s78 : (float)SM_FIXED[l];
-            dm[l] = sl * (m[l] - eq[l]) * (1.0f / 36.0f);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        float s = 0.f;
-#pragma unroll
-        for (int l = 0; l < 9; l++)
-            if (TMINV_K[k][l] != 0 && SM_FIXED[l] != 0.0) s += (float)TMINV_K[k][l] * dm[l];
-        g[k] -= s;
-    }
 }
 
 IPPI_HD void guo(float ux, float uy, float fx, float fy, float tau_begin, float* g) {
