@@ -102,6 +102,11 @@ int ippi_steps_done(const ippi_handle* h, int64_t* steps);
 
 /* Macroscopic fields after the last completed step (synchronising). */
 int ippi_get_fields(ippi_handle* h, double* rho, double* ux, double* uy);
+/* Relaxation time the next collision uses at each node (base value, LES or
+ * the network with its >= 0.501 guard), evaluated on the current state
+ * without the immersed-boundary velocity correction; nx*ny doubles.
+ * Replaces reads of LatMesh[i][j].tau (WriteDatFluid, mesh.cpp:1231). */
+int ippi_get_tau(ippi_handle* h, double* tau);
 /* Post-streaming populations f (FP32 path: w + stored deviation), 9*nx*ny. */
 int ippi_get_populations(ippi_handle* h, double* f);
 /* Per-step force on each body (minus the sum of its marker forces, FP64),

@@ -64,6 +64,7 @@ _SIGS = {
     "ippi_steps_done": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "ippi_get_fields": ([ctypes.c_void_p, _D, _D, _D], ctypes.c_int),
     "ippi_get_populations": ([ctypes.c_void_p, _D], ctypes.c_int),
+    "ippi_get_tau": ([ctypes.c_void_p, _D], ctypes.c_int),
     "ippi_get_body_forces": ([ctypes.c_void_p, _D, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "ippi_get_markers": ([ctypes.c_void_p, _D, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "ippi_mass": ([ctypes.c_void_p, _D], ctypes.c_int),
@@ -183,6 +184,11 @@ class Solver:
         self._check(library().ippi_get_fields(self.h, _p(r), _p(ux), _p(uy)))
         s = (self.ny, self.nx)
         return r.reshape(s), ux.reshape(s), uy.reshape(s)
+
+    def tau(self):
+        t = np.empty(self.N)
+        self._check(library().ippi_get_tau(self.h, _p(t)))
+        return t.reshape(self.ny, self.nx)
 
     def populations(self):
         f = np.empty(9 * self.N)

@@ -24,6 +24,7 @@
 
 #include "../../include/ippi_lbm.h"
 #include "lbm_node.hpp"
+#include "mlp.hpp"
 
 using namespace lbm;
 
@@ -46,6 +47,7 @@ struct StepArgs {
     int* flag;              // divergence flag
     int nbody;
     int nt;                 // non-temporal stores
+    MlpDev mlp;             // relaxation-time network (tau_mode NN)
     Patch patch[IPPI_MAXB];
 };
 
@@ -265,9 +267,32 @@ __device__ __forceinline__ void load_raw(const T* __restrict__ src, int x, int y
 // Collision of one node. The equilibrium is computed once and shared by the
 // LES and the BGK / RR collisions (the reference recomputes the identical
 // values); the MRT keeps the reference's moment-space evaluation.
-template <int COLL, int BC, bool PULL>
+// NN tau of the FP64 path: features as the oracle's or_mlp_features
+// (rho, ux, uy, |S| with S = -3/(2 rho) Pi_neq, Pi_xx - Pi_yy, Pi_xy).
+__device__ __forceinline__ double nn_tau_f64(const double* f, const double* q, double rho, double ux, double uy,
+                                             const double* __restrict__ blob) {
+#pragma clang fp contract(off)
+    double d[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = f[k] - q[k];
+    const double pxx = ((((d[1] + d[3]) + d[5]) + d[6]) + d[7]) + d[8];
+    const double pxy = ((d[5] - d[6]) + d[7]) - d[8];
+    const double pyy = ((((d[2] + d[4]) + d[5]) + d[6]) + d[7]) + d[8];
+    const double c = -3.0 / (2 * rho);
+    const double sxx = pxx * c, sxy = pxy * c, syy = pyy * c;
+    double in[6];
+    in[0] = rho;
+    in[1] = ux;
+    in[2] = uy;
+    in[3] = sqrt(2.0 * (sxx * sxx + 2.0 * sxy * sxy + syy * syy));
+    in[4] = pxx - pyy;
+    in[5] = pxy;
+    return mlp_tau_f64(in, blob);
+}
+
+template <int COLL, int BC, bool PULL, bool NN>
 __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, const double* __restrict__ mac,
-                                         double* __restrict__ dst, const StepArgs& a) {
+                                         double* __restrict__ dst, const StepArgs& a, bool active) {
 #pragma clang fp contract(off)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
@@ -304,13 +329,19 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
     const double tb = a.tau_field ? ld(static_cast<const double*>(a.tau_ptr), n) : a.tau0;
     double tau = tb;
     double o[9];
+    double q[9];
+    ref::feq(rho, ux, uy, q);
+    bool nn_ok = false;
+    if (NN) {  // guard of mesh.cpp:2821-2827: NN value if >= 0.501, else LES / base
+        const double t = nn_tau_f64(f, q, rho, ux, uy, a.mlp.blob);
+        nn_ok = t >= 0.501;
+        if (nn_ok) tau = t;
+    }
     if (COLL == IPPI_COLL_MRT) {
-        if (a.les) tau = ref::les_tau(f, rho, ux, uy, tb, a.cs);
+        if (a.les && !nn_ok) tau = ref::les_tau_q(f, q, rho, tb, a.cs);
         ref::mrt(f, rho, ux, uy, tau, o);
     } else {
-        double q[9];
-        ref::feq(rho, ux, uy, q);
-        if (a.les) tau = ref::les_tau_q(f, q, rho, tb, a.cs);
+        if (a.les && !nn_ok) tau = ref::les_tau_q(f, q, rho, tb, a.cs);
         if (COLL == IPPI_COLL_RR)
             ref::rr_q(f, q, tau, o);
         else
@@ -322,19 +353,21 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
 #pragma unroll
         for (int k = 0; k < 9; k++) o[k] = o[k] + g[k];
     }
+    if (active) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) st(dst + k * N, n, o[k], a.nt);
+        for (int k = 0; k < 9; k++) st(dst + k * N, n, o[k], a.nt);
+    }
 }
 
 // PULL = false: collide the fnew state stored at the node itself with the
 // moments from the macro arrays (first collision after initialisation).
-template <int COLL, int BC, bool PULL, int NPT>
+template <int COLL, int BC, bool PULL, int NPT, bool NN>
 __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src, double* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y0 = blockIdx.y * NPT;
     const int nx = a.nx, ny = a.ny;
-    if (x >= nx) return;
+    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
+    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
+    const int y0 = blockIdx.y * NPT;
     const unsigned N = (unsigned)nx * ny;
     double cur[9];
     load_raw<double, BC, PULL>(src, x, y0, nx, ny, N, cur);
@@ -345,7 +378,7 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
         const bool more = j + 1 < NPT && y + 1 < ny;
         double nxt[9];
         if (j + 1 < NPT) load_raw<double, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f64<COLL, BC, PULL>(cur, x, y, N, mac, dst, a);
+        node_f64<COLL, BC, PULL, NN>(cur, x, y, N, mac, dst, a, active);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -359,9 +392,9 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
 // never touched, so FP32 rounding cannot drift mass or momentum. RR is
 // evaluated on populations with the increment projected onto exact mass and
 // momentum. The LES stress comes from the second moments directly.
-template <int COLL, int BC, bool PULL>
+template <int COLL, int BC, bool PULL, bool NN>
 __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const double* __restrict__ mac,
-                                         float* __restrict__ dst, const StepArgs& a) {
+                                         float* __restrict__ dst, const StepArgs& a, bool active, __bf16* lds) {
 #pragma clang fp contract(fast)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
@@ -413,7 +446,20 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     const float nxx = (o.s13 + (o.s57 + o.s68)) - (rux * ux + d3);
     const float nyy = (o.s24 + (o.s57 + o.s68)) - (ruy * uy + d3);
     const float nxy = (o.s57 - o.s68) - rux * uy;
-    if (a.les) tau = sh::les_tau(nxx, nxy, nyy, rho, tb, a.csf);
+    bool nn_ok = false;
+    if (NN) {  // guard of mesh.cpp:2821-2827: NN value if >= 0.501, else LES / base
+        float feat[6];
+        feat[0] = rho;
+        feat[1] = ux;
+        feat[2] = uy;
+        feat[3] = 1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy));
+        feat[4] = nxx - nyy;
+        feat[5] = nxy;
+        const float t = mlp_tau_wave(feat, a.mlp, lds);
+        nn_ok = t >= 0.501f;
+        if (nn_ok) tau = t;
+    }
+    if (a.les && !nn_ok) tau = sh::les_tau(nxx, nxy, nyy, rho, tb, a.csf);
     const float om = sh::rcp(tau);
     if (COLL == IPPI_COLL_RR) {
         float q[9];
@@ -468,17 +514,23 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         g[8] -= B - q64 - c8 + c3 - c5;
     }
     if (forced) sh::guo(ux, uy, fx, fy, tb, g);
+    if (active) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) st(dst + k * N, n, g[k], a.nt);
+        for (int k = 0; k < 9; k++) st(dst + k * N, n, g[k], a.nt);
+    }
 }
 
-template <int COLL, int BC, bool PULL, int NPT>
+template <int COLL, int BC, bool PULL, int NPT, bool NN>
 __global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y0 = blockIdx.y * NPT;
     const int nx = a.nx, ny = a.ny;
-    if (x >= nx) return;
+    // all lanes stay resident (the MLP's MFMAs are wave-wide); lanes past
+    // the last column recompute column nx-1 and do not store
+    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
+    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
+    const int y0 = blockIdx.y * NPT;
+    __shared__ __bf16 lds_mlp[NN ? BX * 16 : 1];
+    __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
     const unsigned N = (unsigned)nx * ny;
     float cur[9];
     load_raw<float, BC, PULL>(src, x, y0, nx, ny, N, cur);
@@ -489,12 +541,70 @@ __global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, 
         const bool more = j + 1 < NPT && y + 1 < ny;
         float nxt[9];
         if (j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f32<COLL, BC, PULL>(cur, x, y, N, mac, dst, a);
+        node_f32<COLL, BC, PULL, NN>(cur, x, y, N, mac, dst, a, active, lds);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
         }
     }
+}
+
+
+// Relaxation time the next collision uses at each node, from the current
+// post-streaming state (IB velocity correction not applied): base value,
+// LES, or the network with its guard.
+template <int BC, bool NN>
+__global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, double* __restrict__ out, StepArgs a) {
+#pragma clang fp contract(fast)
+    const int nx = a.nx, ny = a.ny;
+    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
+    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
+    const int y = blockIdx.y;
+    const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
+    __shared__ __bf16 lds_mlp[NN ? BX * 16 : 1];
+    float g[9];
+    Mo o;
+    node_state32<BC>(src, x, y, nx, ny, N, a.uinxf, a.uinyf, g, o);
+    const float tb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tau0f;
+    const float d3 = o.drho * (1.0f / 3.0f);
+    const float nxx = (o.s13 + (o.s57 + o.s68)) - (o.rho * o.ux * o.ux + d3);
+    const float nyy = (o.s24 + (o.s57 + o.s68)) - (o.rho * o.uy * o.uy + d3);
+    const float nxy = (o.s57 - o.s68) - o.rho * o.ux * o.uy;
+    float tau = tb;
+    bool nn_ok = false;
+    if (NN) {
+        float feat[6] = {o.rho, o.ux, o.uy,
+                         1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy)), nxx - nyy, nxy};
+        const float t = mlp_tau_wave(feat, a.mlp, lds_mlp + (threadIdx.x & ~63) * 16);
+        nn_ok = t >= 0.501f;
+        if (nn_ok) tau = t;
+    }
+    if (a.les && !nn_ok) tau = sh::les_tau(nxx, nxy, nyy, o.rho, tb, a.csf);
+    if (active) out[n] = tau;
+}
+
+template <int BC, bool NN>
+__global__ void __launch_bounds__(BX) k_tau64(const double* __restrict__ src, double* __restrict__ out, StepArgs a) {
+#pragma clang fp contract(off)
+    const int nx = a.nx, ny = a.ny;
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= nx) return;
+    const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
+    double f[9], rho, ux, uy;
+    node_state(src, x, y, nx, ny, N, BC, a.uinx, a.uiny, f, rho, ux, uy);
+    const double tb = a.tau_field ? ld(static_cast<const double*>(a.tau_ptr), n) : a.tau0;
+    double q[9];
+    ref::feq(rho, ux, uy, q);
+    double tau = tb;
+    bool nn_ok = false;
+    if (NN) {
+        const double t = nn_tau_f64(f, q, rho, ux, uy, a.mlp.blob);
+        nn_ok = t >= 0.501;
+        if (nn_ok) tau = t;
+    }
+    if (a.les && !nn_ok) tau = ref::les_tau_q(f, q, rho, tb, a.cs);
+    out[n] = tau;
 }
 
 // ----------------------------------------------------------- init kernels
@@ -823,6 +933,11 @@ struct ippi_handle {
     double* hist = nullptr;
     int hist_cap = 0;
     int64_t nrec = 0;
+    // relaxation-time network
+    bool has_mlp = false;
+    MlpFrag* d_frag = nullptr;
+    double* d_blob = nullptr;
+    float mlp_mu[6] = {0}, mlp_isig[6] = {0}, mlp_b3 = 0.f;
     // optional per-launch timing (HIP events on this handle's stream)
     int timing = 0;
     std::vector<hipEvent_t> ev_step, ev_ib;  // start/end pairs
@@ -861,14 +976,14 @@ int check_flag(ippi_handle* h) {
 template <typename T>
 using StepKernel = void (*)(const T*, T*, const double*, StepArgs);
 
-template <typename T, bool PULL>
+template <typename T, bool PULL, bool NN>
 StepKernel<T> pick_step(int coll, int bc) {
 #define IPPI_PICK(C, B)                                                                   \
     if (coll == C && bc == B) {                                                           \
         if constexpr (sizeof(T) == 8)                                                     \
-            return (StepKernel<T>)k_step_f64<C, B, PULL, PULL ? IPPI_NPT : 1>;            \
+            return (StepKernel<T>)k_step_f64<C, B, PULL, PULL ? IPPI_NPT : 1, NN>;        \
         else                                                                              \
-            return (StepKernel<T>)k_step_f32<C, B, PULL, PULL ? IPPI_NPT : 1>;            \
+            return (StepKernel<T>)k_step_f32<C, B, PULL, PULL ? IPPI_NPT : 1, NN>;        \
     }
     IPPI_PICK(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
     IPPI_PICK(IPPI_COLL_MRT, IPPI_BC_PERIODIC_BB)
@@ -899,6 +1014,13 @@ StepArgs make_args(ippi_handle* h) {
     a.flag = h->flag;
     a.nbody = (int)h->bodies.size();
     a.nt = 2 * 9 * h->N * h->esz > ((size_t)192 << 20);
+    a.mlp.frag = h->d_frag;
+    a.mlp.blob = h->d_blob;
+    for (int i = 0; i < 6; i++) {
+        a.mlp.mu[i] = h->mlp_mu[i];
+        a.mlp.isig[i] = h->mlp_isig[i];
+    }
+    a.mlp.b3 = h->mlp_b3;
     for (size_t b = 0; b < h->bodies.size(); b++) a.patch[b] = h->bodies[b].p;
     return a;
 }
@@ -1054,7 +1176,11 @@ int run_steps(ippi_handle* h, int nsteps) {
             if (r) return r;
             if (h->timing) h->ev_ib.push_back(new_event(h));
         }
-        StepKernel<T> k = pull ? pick_step<T, true>(h->c.collision, h->c.bc) : pick_step<T, false>(h->c.collision, h->c.bc);
+        const bool nn = h->c.tau_mode == IPPI_TAU_NN;
+        StepKernel<T> k = pull ? (nn ? pick_step<T, true, true>(h->c.collision, h->c.bc)
+                                     : pick_step<T, true, false>(h->c.collision, h->c.bc))
+                               : (nn ? pick_step<T, false, true>(h->c.collision, h->c.bc)
+                                     : pick_step<T, false, false>(h->c.collision, h->c.bc));
         if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
         if (h->timing && pull) h->ev_step.push_back(new_event(h));
         hipLaunchKernelGGL(k, pull ? grid_pull : grid_first, dim3(BX), 0, h->s, (const T*)h->buf[h->cur],
@@ -1148,6 +1274,8 @@ int ippi_destroy(ippi_handle* h) {
     clear_events(h->ev_ib);
     dfree(h->buf[0]); dfree(h->buf[1]); dfree(h->mac); dfree(h->tau); dfree(h->flag); dfree(h->part);
     free_ib(h);
+    dfree(h->d_frag);
+    dfree(h->d_blob);
     if (h->s) (void)hipStreamDestroy(h->s);
     delete h;
     return IPPI_OK;
@@ -1264,14 +1392,55 @@ int ippi_set_tau_field(ippi_handle* h, const double* tau) {
 
 int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
     if (!h || !blob) return IPPI_ERR_ARG;
-    (void)n;
-    return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not available in this build");
+    if (n != MLP_BLOB) return fail(h, IPPI_ERR_ARG, "MLP blob must hold 1325 values");
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(blob[i])) return fail(h, IPPI_ERR_ARG, "MLP blob holds a non-finite value");
+    // per-lane MFMA fragments of the FP32 path (layouts described in mlp.hpp)
+    std::vector<MlpFrag> fr(64);
+    auto hi = [](double v) { return (__bf16)(float)v; };
+    auto lo = [](double v) { return (__bf16)((float)v - (float)(__bf16)(float)v); };
+    for (int l = 0; l < 64; l++) {
+        const int g = l >> 4, c = l & 15;
+        MlpFrag& F = fr[l];
+        for (int mb = 0; mb < 2; mb++) {
+            const int o = mb * 16 + c;
+            for (int j = 0; j < 4; j++) {
+                const int k = 4 * g + j;
+                const double w = k < MLP_NIN ? blob[OFF_W1 + o * MLP_NIN + k] : 0.0;
+                F.a1h[mb][j] = hi(w);
+                F.a1l[mb][j] = lo(w);
+            }
+            for (int j = 0; j < 8; j++) {
+                const int k = (j / 4) * 16 + 4 * g + (j % 4);  // order of the layer-1 outputs in the lanes
+                F.a2h[mb][j] = hi(blob[OFF_W2 + o * MLP_NH + k]);
+                F.a2l[mb][j] = lo(blob[OFF_W2 + o * MLP_NH + k]);
+            }
+            for (int r = 0; r < 4; r++) {
+                const int oo = mb * 16 + 4 * g + r;
+                F.b1[4 * mb + r] = (float)blob[OFF_B1 + oo];
+                F.b2[4 * mb + r] = (float)blob[OFF_B2 + oo];
+                F.w3[4 * mb + r] = (float)blob[OFF_W3 + oo];
+            }
+        }
+    }
+    if (!h->d_frag) HIPC(h, hipMalloc(&h->d_frag, sizeof(MlpFrag) * 64));
+    if (!h->d_blob) HIPC(h, hipMalloc(&h->d_blob, sizeof(double) * MLP_BLOB));
+    HIPC(h, hipStreamSynchronize(h->s));
+    HIPC(h, hipMemcpy(h->d_frag, fr.data(), sizeof(MlpFrag) * 64, hipMemcpyHostToDevice));
+    HIPC(h, hipMemcpy(h->d_blob, blob, sizeof(double) * MLP_BLOB, hipMemcpyHostToDevice));
+    for (int i = 0; i < 6; i++) {
+        h->mlp_mu[i] = (float)blob[OFF_MU + i];
+        h->mlp_isig[i] = (float)blob[OFF_IS + i];
+    }
+    h->mlp_b3 = (float)blob[OFF_B3];
+    h->has_mlp = true;
+    return IPPI_OK;
 }
 
 int ippi_step(ippi_handle* h, int nsteps) {
     if (!h || nsteps < 0) return IPPI_ERR_ARG;
     if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
-    if (h->c.tau_mode == IPPI_TAU_NN) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
+    if (h->c.tau_mode == IPPI_TAU_NN && !h->has_mlp) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
     return h->esz == 8 ? run_steps<double>(h, nsteps) : run_steps<float>(h, nsteps);
 }
 
@@ -1327,6 +1496,37 @@ int ippi_get_fields(ippi_handle* h, double* rho, double* ux, double* uy) {
     if (ux) HIPC(h, hipMemcpyAsync(ux, h->mac + h->N, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s));
     if (uy) HIPC(h, hipMemcpyAsync(uy, h->mac + 2 * h->N, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s));
     HIPC(h, hipStreamSynchronize(h->s));
+    return check_flag(h);
+}
+
+int ippi_get_tau(ippi_handle* h, double* tau) {
+    if (!h || !tau) return IPPI_ERR_ARG;
+    if (h->phase == 0) return fail(h, IPPI_ERR_STATE, "no step taken yet");
+    if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
+    if (h->c.tau_mode == IPPI_TAU_NN && !h->has_mlp) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
+    StepArgs a = make_args(h);
+    double* d_out = nullptr;
+    HIPC(h, hipMalloc(&d_out, sizeof(double) * h->N));
+    dim3 grid((h->c.nx + BX - 1) / BX, h->c.ny);
+    const bool nn = h->c.tau_mode == IPPI_TAU_NN, pb = h->c.bc == IPPI_BC_PERIODIC_BB;
+    if (h->esz == 4) {
+        const float* src = (const float*)h->buf[h->cur];
+        if (pb && nn) hipLaunchKernelGGL((k_tau32<IPPI_BC_PERIODIC_BB, true>), grid, dim3(BX), 0, h->s, src, d_out, a);
+        if (pb && !nn) hipLaunchKernelGGL((k_tau32<IPPI_BC_PERIODIC_BB, false>), grid, dim3(BX), 0, h->s, src, d_out, a);
+        if (!pb && nn) hipLaunchKernelGGL((k_tau32<IPPI_BC_INFLOW, true>), grid, dim3(BX), 0, h->s, src, d_out, a);
+        if (!pb && !nn) hipLaunchKernelGGL((k_tau32<IPPI_BC_INFLOW, false>), grid, dim3(BX), 0, h->s, src, d_out, a);
+    } else {
+        const double* src = (const double*)h->buf[h->cur];
+        if (pb && nn) hipLaunchKernelGGL((k_tau64<IPPI_BC_PERIODIC_BB, true>), grid, dim3(BX), 0, h->s, src, d_out, a);
+        if (pb && !nn) hipLaunchKernelGGL((k_tau64<IPPI_BC_PERIODIC_BB, false>), grid, dim3(BX), 0, h->s, src, d_out, a);
+        if (!pb && nn) hipLaunchKernelGGL((k_tau64<IPPI_BC_INFLOW, true>), grid, dim3(BX), 0, h->s, src, d_out, a);
+        if (!pb && !nn) hipLaunchKernelGGL((k_tau64<IPPI_BC_INFLOW, false>), grid, dim3(BX), 0, h->s, src, d_out, a);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(tau, d_out, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(h, IPPI_ERR_HIP, std::string("ippi_get_tau: ") + hipGetErrorString(e));
     return check_flag(h);
 }
 

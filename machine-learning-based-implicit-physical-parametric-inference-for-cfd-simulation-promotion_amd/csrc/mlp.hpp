@@ -1,0 +1,170 @@
+// Relaxation-time MLP 6 -> 32 -> 32 -> 1 (SiLU, tau = softplus(o) + 0.5),
+// evaluated inside the collision kernels.
+//
+// Reference counterpart: the per-node libtorch inference of
+// Mesh::NNmodel_Type00..07 (IPPI-LBM/mesh.cpp:2767-3260) with its guard
+// tau = NN if NN >= 0.501 else the LES value (mesh.cpp:2821-2827, :393).
+//
+// Parameter blob (host, double): W1[32][6] b1[32] W2[32][32] b2[32] W3[32] b3
+// mu[6] isig[6]; features are normalised as (f - mu) * isig.
+//
+// FP32 path: one wave (64 nodes) evaluates the two hidden layers with bf16
+// MFMA tiles (v_mfma_f32_16x16x16_bf16 for layer 1, K = 6 padded to 16;
+// v_mfma_f32_16x16x32_bf16 for layer 2) with FP32 accumulation. Each bf16
+// operand is split into hi + lo parts and the product is taken as
+// hi*hi + hi*lo + lo*hi, which carries ~16 mantissa bits. Tiles are laid
+// out as D[o][node]: A = weights (16 outputs x K), B = activations
+// (K x 16 nodes). Lane l of an MFMA supplies A[l%16][4(l/16)..+4] (or
+// 8(l/16)..+8 for K = 32), B[4(l/16)..+4][l%16] and receives
+// D[4(l/16)+r][l%16], r = 0..3. The layer-1 result tiles feed the layer-2 B
+// operand without a transpose: the host permutes W2's columns to the
+// order in which the lanes hold the layer-1 outputs.
+//
+// FP64 path: the same network on VALU in FP64 with the oracle's loop order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lbm {
+
+constexpr int MLP_NIN = 6, MLP_NH = 32;
+constexpr int MLP_NPARAM = MLP_NIN * MLP_NH + MLP_NH + MLP_NH * MLP_NH + MLP_NH + MLP_NH + 1;  // 1313
+constexpr int MLP_BLOB = MLP_NPARAM + 2 * MLP_NIN;                                              // 1325
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + MLP_NH * MLP_NIN, OFF_W2 = OFF_B1 + MLP_NH,
+              OFF_B2 = OFF_W2 + MLP_NH * MLP_NH, OFF_W3 = OFF_B2 + MLP_NH, OFF_B3 = OFF_W3 + MLP_NH,
+              OFF_MU = OFF_B3 + 1, OFF_IS = OFF_MU + MLP_NIN;
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Per-lane fragments of the FP32/MFMA path (built on the host from the blob).
+struct MlpFrag {
+    bf16x4 a1h[2], a1l[2];  // layer-1 A tiles (two 16-output blocks), hi / lo
+    bf16x8 a2h[2], a2l[2];  // layer-2 A tiles, columns in lane order
+    float b1[8], b2[8], w3[8];  // bias / output weight of this lane's 8 outputs
+};
+struct MlpDev {
+    const MlpFrag* frag;  // [64 lanes]
+    float mu[6], isig[6], b3;
+    const double* blob;   // FP64 path
+};
+
+__device__ __forceinline__ float silu_f(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float softplus_f(float z) { return z > 15.0f ? z : __logf(1.0f + __expf(z)); }
+
+__device__ __forceinline__ void split(float v, __bf16& h, __bf16& l) {
+    h = (__bf16)v;
+    l = (__bf16)(v - (float)h);
+}
+
+// tau of this lane's node from its six raw features; every lane of the wave
+// must call it (the MFMAs are wave-wide). lds: 64 x 8 bf16 x 2 (hi, lo) per wave.
+__device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m, __bf16* lds) {
+    const int l = threadIdx.x & 63;
+    const int g = l >> 4;  // lane group
+    const MlpFrag& fr = m.frag[l];
+    // normalised features, split hi / lo, to LDS as [node][k] (k 0..7)
+    __bf16 xh[8], xl[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float v = k < 6 ? (feat[k] - m.mu[k]) * m.isig[k] : 0.0f;
+        split(v, xh[k], xl[k]);
+    }
+    bf16x8 vh, vl;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        vh[k] = xh[k];
+        vl[k] = xl[k];
+    }
+    *reinterpret_cast<bf16x8*>(lds + 16 * l) = vh;
+    *reinterpret_cast<bf16x8*>(lds + 16 * l + 8) = vl;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // layer 1: D1[mb][nb] = W1[mb] x X[nb]
+    f32x4 d1[2][4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        const int node = nb * 16 + (l & 15);
+        bf16x4 bh, bl;
+        if (g < 2) {
+            bh = *reinterpret_cast<const bf16x4*>(lds + 16 * node + 4 * g);
+            bl = *reinterpret_cast<const bf16x4*>(lds + 16 * node + 8 + 4 * g);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                bh[j] = (__bf16)0.0f;
+                bl[j] = (__bf16)0.0f;
+            }
+        }
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++) {
+            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1h[mb], bh, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1h[mb], bl, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1l[mb], bh, c, 0, 0, 0);
+            d1[mb][nb] = c;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // bias + SiLU -> layer-2 B operand (8 values per lane and node block)
+    float part[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        bf16x8 bh, bl;
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float h = silu_f(d1[mb][nb][r] + fr.b1[4 * mb + r]);
+                __bf16 hh, hl;
+                split(h, hh, hl);
+                bh[4 * mb + r] = hh;
+                bl[4 * mb + r] = hl;
+            }
+        float acc = 0.0f;
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++) {
+            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], bh, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], bl, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2l[mb], bh, c, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc += fr.w3[4 * mb + r] * silu_f(c[r] + fr.b2[4 * mb + r]);
+        }
+        part[nb] = acc;
+    }
+    // layer 3: sum the four lane groups holding the same node
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        part[nb] += __shfl_xor(part[nb], 16);
+        part[nb] += __shfl_xor(part[nb], 32);
+    }
+    const float o = (g == 0 ? part[0] : g == 1 ? part[1] : g == 2 ? part[2] : part[3]) + m.b3;
+    return softplus_f(o) + 0.5f;
+}
+
+// FP64 network on VALU, the oracle's loop order (weights read wave-uniformly).
+__device__ __forceinline__ double mlp_tau_f64(const double* in, const double* __restrict__ p) {
+#pragma clang fp contract(off)
+    double x[MLP_NIN], h1[MLP_NH];
+#pragma unroll
+    for (int i = 0; i < MLP_NIN; i++) x[i] = (in[i] - p[OFF_MU + i]) * p[OFF_IS + i];
+#pragma unroll 4
+    for (int j = 0; j < MLP_NH; j++) {
+        double s = p[OFF_B1 + j];
+#pragma unroll
+        for (int i = 0; i < MLP_NIN; i++) s += p[OFF_W1 + j * MLP_NIN + i] * x[i];
+        h1[j] = s / (1.0 + exp(-s));
+    }
+    double o = p[OFF_B3];
+    for (int j = 0; j < MLP_NH; j++) {
+        double s = p[OFF_B2 + j];
+#pragma unroll
+        for (int i = 0; i < MLP_NH; i++) s += p[OFF_W2 + j * MLP_NH + i] * h1[i];
+        o += p[OFF_W3 + j] * (s / (1.0 + exp(-s)));
+    }
+    const double sp = o > 30.0 ? o : log1p(exp(o));
+    return sp + 0.5;
+}
+
+}  // namespace lbm
