@@ -65,6 +65,8 @@ typedef struct ippi_config {
     double u_in_y;
     int device;          /* HIP device ordinal */
     int force_history;   /* body-force records kept (ring); 0 -> 65536 */
+    double tau_nn_max;   /* IPPI_TAU_NN: network values above this are rejected
+                            like those below 0.501 (0 = no upper bound) */
 } ippi_config;
 
 typedef struct ippi_handle ippi_handle;
@@ -79,6 +81,11 @@ int ippi_version(void);
 int ippi_init_fields(ippi_handle* h, uint64_t seed, double rho0, double ux0, double uy0, double noise);
 /* rho/u given per node ([y][x]); f = feq(rho, u). */
 int ippi_set_fields(ippi_handle* h, const double* rho, const double* ux, const double* uy);
+
+/* Post-streaming populations [k][y][x] (9*nx*ny doubles); moments from
+ * CalMacro. Used to start a run from a given distribution (e.g. a restricted
+ * finer solution). Replaces direct writes of LatMesh[i][j].fnew. */
+int ippi_set_populations(ippi_handle* h, const double* f);
 
 /* A circular body of nmarkers Lagrangian markers at x = cx + R sin(2 pi i/M),
  * y = cy + R cos(2 pi i/M), R = diameter/2, desired velocity (udx, udy).
@@ -116,6 +123,38 @@ int ippi_get_body_forces(ippi_handle* h, double* out, int max_records, int* nrec
 int ippi_get_markers(ippi_handle* h, double* out, int max_markers, int* n);
 /* Global mass sum(rho) in FP64, fixed-order reduction. */
 int ippi_mass(ippi_handle* h, double* mass);
+
+/* Training pairs for the relaxation-time network, generated by this solver
+ * at twice the resolution of the grid they are for (the GPU analogue of
+ * FindAndWriteTau, mesh.cpp:2625-2715). The current state is restricted to
+ * (nx/2) x (ny/2) (2x2 averages, non-equilibrium scaled by
+ * 2 tau_coarse / tau0), the solver advances two steps (one coarse step),
+ * and for every coarse node the relaxation time is found whose collision at
+ * the upstream nodes, once streamed, best matches the restricted mass and
+ * momentum (least squares, closed form; coarse_collision BGK or RR).
+ * Outputs per coarse node: feat[6] (rho, ux, uy, |S|, Pi_xx-Pi_yy, Pi_xy of
+ * the restricted state), label omega = 1/tau (the fit is linear in omega)
+ * and weight |a|^2 of the fit (0 = unusable: near the domain edge or a
+ * body, or omega outside (0, 4)). */
+int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, float* feat, float* label,
+                      float* weight);
+
+/* GPU trainer of the relaxation-time network (no counterpart in the
+ * reference, whose networks are trained offline). Targets are relaxation
+ * rates omega; loss sum w (1/tau(x) - y)^2 / (2 sum w) in FP32;
+ * deterministic gradient; Adam. */
+typedef struct ippi_trainer ippi_trainer;
+int ippi_trainer_create(int capacity, ippi_trainer** out);
+int ippi_trainer_destroy(ippi_trainer* t);
+const char* ippi_trainer_last_error(const ippi_trainer* t);
+/* x: [n][6] raw features, y: [n] targets, w: [n] weights (NULL = 1). */
+int ippi_trainer_set_data(ippi_trainer* t, const float* x, const float* y, const float* w, int n);
+int ippi_trainer_set_params(ippi_trainer* t, const double* blob, int n);
+int ippi_trainer_get_params(ippi_trainer* t, double* blob, int n);
+/* Full-data loss and gradient (1313 values, blob order without mu/isig). */
+int ippi_trainer_loss_grad(ippi_trainer* t, double* loss, double* grad);
+/* iters Adam steps on minibatches of `batch` samples drawn with splitmix64. */
+int ippi_trainer_adam(ippi_trainer* t, int iters, int batch, double lr, uint64_t seed, double* loss);
 
 /* Kernel timing (no counterpart in the reference: measurement only). While
  * enabled, every fused stream-collide launch and every immersed-boundary

@@ -39,10 +39,12 @@ class Config(ctypes.Structure):
         ("tau0", ctypes.c_double), ("cs", ctypes.c_double),
         ("u_in_x", ctypes.c_double), ("u_in_y", ctypes.c_double),
         ("device", ctypes.c_int), ("force_history", ctypes.c_int),
+        ("tau_nn_max", ctypes.c_double),
     ]
 
 
 _D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
 _lib = None
 
 # every entry point of include/ippi_lbm.h with its ctypes signature
@@ -71,6 +73,17 @@ _SIGS = {
     "ippi_set_timing": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "ippi_get_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64), _D, ctypes.POINTER(ctypes.c_int64)],
                         ctypes.c_int),
+    "ippi_set_populations": ([ctypes.c_void_p, _D], ctypes.c_int),
+    "ippi_coarse_pairs": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _F, _F, _F], ctypes.c_int),
+    "ippi_trainer_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "ippi_trainer_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "ippi_trainer_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
+    "ippi_trainer_set_data": ([ctypes.c_void_p, _F, _F, _F, ctypes.c_int], ctypes.c_int),
+    "ippi_trainer_set_params": ([ctypes.c_void_p, _D, ctypes.c_int], ctypes.c_int),
+    "ippi_trainer_get_params": ([ctypes.c_void_p, _D, ctypes.c_int], ctypes.c_int),
+    "ippi_trainer_loss_grad": ([ctypes.c_void_p, _D, _D], ctypes.c_int),
+    "ippi_trainer_adam": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D],
+                          ctypes.c_int),
     "ippi_save": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "ippi_load": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
 }
@@ -95,6 +108,10 @@ def _p(a):
     return a.ctypes.data_as(_D)
 
 
+def _pf(a):
+    return a.ctypes.data_as(_F) if a is not None else None
+
+
 def _f64(a, n):
     a = np.ascontiguousarray(a, dtype=np.float64).ravel()
     if a.size != n:
@@ -106,13 +123,14 @@ class Solver:
     """One whole-domain D2Q9 solver on one GPU (a handle of the C ABI)."""
 
     def __init__(self, nx, ny, collision="bgk", precision="fp64", bc="periodic_bb", les=False,
-                 tau_mode="const", tau0=0.6, cs=0.0, u_in=(0.0, 0.0), device=0, force_history=0):
+                 tau_mode="const", tau0=0.6, cs=0.0, u_in=(0.0, 0.0), device=0, force_history=0,
+                 tau_nn_max=0.0):
         L = library()
         self.nx, self.ny, self.N = int(nx), int(ny), int(nx) * int(ny)
         self.precision = precision
         cfg = Config(self.nx, self.ny, COLLISION[collision], PRECISION[precision], BC[bc], int(bool(les)),
                      TAU_MODE[tau_mode], float(tau0), float(cs), float(u_in[0]), float(u_in[1]), int(device),
-                     int(force_history))
+                     int(force_history), float(tau_nn_max))
         h = ctypes.c_void_p()
         rc = L.ippi_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != IPPI_OK:
@@ -158,6 +176,21 @@ class Solver:
                                             float(ud[0]), float(ud[1]), ctypes.byref(b)))
         self.nbodies = b.value + 1
         return b.value
+
+    def set_populations(self, f):
+        a = _f64(f, 9 * self.N)
+        self._check(library().ippi_set_populations(self.h, _p(a)))
+
+    def coarse_pairs(self, tau_coarse, coarse_collision="rr"):
+        """Training pairs at half resolution; advances this solver two steps.
+        Returns (features [n,6], labels [n], weights [n]) over the coarse grid."""
+        nc = (self.nx // 2) * (self.ny // 2)
+        feat = np.empty(6 * nc, dtype=np.float32)
+        lab = np.empty(nc, dtype=np.float32)
+        w = np.empty(nc, dtype=np.float32)
+        self._check(library().ippi_coarse_pairs(self.h, float(tau_coarse), COLLISION[coarse_collision],
+                                                _pf(feat), _pf(lab), _pf(w)))
+        return feat.reshape(nc, 6), lab, w
 
     def set_tau_field(self, tau):
         t = _f64(tau, self.N)
@@ -235,6 +268,61 @@ class Solver:
     def load(self, blob):
         buf = ctypes.create_string_buffer(blob, len(blob))
         self._check(library().ippi_load(self.h, buf, len(blob)))
+
+
+class Trainer:
+    """GPU trainer of the relaxation-time network (include/ippi_lbm.h)."""
+
+    def __init__(self, capacity):
+        h = ctypes.c_void_p()
+        rc = library().ippi_trainer_create(int(capacity), ctypes.byref(h))
+        if rc != IPPI_OK:
+            raise IppiError(rc, "ippi_trainer_create failed")
+        self.h = h
+
+    def _check(self, rc):
+        if rc != IPPI_OK:
+            msg = library().ippi_trainer_last_error(self.h)
+            raise IppiError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().ippi_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_data(self, x, y, w=None):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.float32).ravel()
+        self._keep = (x, y, w)
+        self._check(library().ippi_trainer_set_data(self.h, _pf(x), _pf(y), _pf(w), len(y)))
+
+    def set_params(self, blob):
+        b = _f64(blob, MLP_BLOB)
+        self._check(library().ippi_trainer_set_params(self.h, _p(b), MLP_BLOB))
+
+    def params(self):
+        b = np.empty(MLP_BLOB)
+        self._check(library().ippi_trainer_get_params(self.h, _p(b), MLP_BLOB))
+        return b
+
+    def loss_grad(self):
+        loss = ctypes.c_double()
+        g = np.empty(MLP_BLOB - 12)
+        self._check(library().ippi_trainer_loss_grad(self.h, ctypes.byref(loss), _p(g)))
+        return loss.value, g
+
+    def adam(self, iters, batch, lr, seed=0):
+        loss = ctypes.c_double()
+        self._check(library().ippi_trainer_adam(self.h, int(iters), int(batch), float(lr), int(seed),
+                                                ctypes.byref(loss)))
+        return loss.value
 
 
 def drag_coefficient(force_x, u_ref, radius):

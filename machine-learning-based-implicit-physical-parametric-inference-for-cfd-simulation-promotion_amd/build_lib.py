@@ -13,7 +13,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = [os.path.join(HERE, "csrc", "ippi_lbm.hip")]
+SRC = [os.path.join(HERE, "csrc", "ippi_lbm.hip"), os.path.join(HERE, "csrc", "ippi_train.hip")]
 OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libippi_lbm.so")
 

@@ -47,6 +47,8 @@ struct StepArgs {
     int* flag;              // divergence flag
     int nbody;
     int nt;                 // non-temporal stores
+    double nn_max;          // upper bound of accepted network values
+    float nn_maxf;
     MlpDev mlp;             // relaxation-time network (tau_mode NN)
     Patch patch[IPPI_MAXB];
 };
@@ -334,7 +336,7 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
     bool nn_ok = false;
     if (NN) {  // guard of mesh.cpp:2821-2827: NN value if >= 0.501, else LES / base
         const double t = nn_tau_f64(f, q, rho, ux, uy, a.mlp.blob);
-        nn_ok = t >= 0.501;
+        nn_ok = t >= 0.501 && t <= a.nn_max;
         if (nn_ok) tau = t;
     }
     if (COLL == IPPI_COLL_MRT) {
@@ -456,7 +458,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         feat[4] = nxx - nyy;
         feat[5] = nxy;
         const float t = mlp_tau_wave(feat, a.mlp, lds);
-        nn_ok = t >= 0.501f;
+        nn_ok = t >= 0.501f && t <= a.nn_maxf;
         if (nn_ok) tau = t;
     }
     if (a.les && !nn_ok) tau = sh::les_tau(nxx, nxy, nyy, rho, tb, a.csf);
@@ -576,7 +578,7 @@ __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, dou
         float feat[6] = {o.rho, o.ux, o.uy,
                          1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy)), nxx - nyy, nxy};
         const float t = mlp_tau_wave(feat, a.mlp, lds_mlp + (threadIdx.x & ~63) * 16);
-        nn_ok = t >= 0.501f;
+        nn_ok = t >= 0.501f && t <= a.nn_maxf;
         if (nn_ok) tau = t;
     }
     if (a.les && !nn_ok) tau = sh::les_tau(nxx, nxy, nyy, o.rho, tb, a.csf);
@@ -600,11 +602,153 @@ __global__ void __launch_bounds__(BX) k_tau64(const double* __restrict__ src, do
     bool nn_ok = false;
     if (NN) {
         const double t = nn_tau_f64(f, q, rho, ux, uy, a.mlp.blob);
-        nn_ok = t >= 0.501;
+        nn_ok = t >= 0.501 && t <= a.nn_max;
         if (nn_ok) tau = t;
     }
     if (a.les && !nn_ok) tau = ref::les_tau_q(f, q, rho, tb, a.cs);
     out[n] = tau;
+}
+
+
+// --------------------------------------------------- training-pair kernels
+// Restriction of this (fine) solver's post-streaming state to the grid of
+// half the resolution: coarse node (I, J) takes the mass and momentum of
+// fine nodes (2I..2I+1, 2J..2J+1) and the average of their non-equilibrium
+// parts f - feq(rho_f, u_f), scaled by s = 2 tau_c / tau_f (the stress in
+// lattice units of a grid twice as coarse at the same lattice velocity):
+// f_c = feq(rho_c, u_c) + s <f - feq>. The unresolved stress
+// <feq> - feq(<.>) is left out, as the coarse solver's own state has none.
+template <typename T, int BC>
+__global__ void __launch_bounds__(BX) k_restrict(const T* __restrict__ src, double* __restrict__ fc,
+                                                 double* __restrict__ mc, int nx, int ny, double uinx,
+                                                 double uiny, double s, int want_f) {
+#pragma clang fp contract(off)
+    const int nxc = nx / 2, nyc = ny / 2;
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y;
+    if (I >= nxc) return;
+    const unsigned N = (unsigned)nx * ny, Nc = (unsigned)nxc * nyc, nc = (unsigned)J * nxc + I;
+    double fa[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, na[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int dj = 0; dj < 2; dj++)
+        for (int di = 0; di < 2; di++) {
+            const int x = 2 * I + di, y = 2 * J + dj;
+            double f[9], r, vx, vy;
+            if constexpr (sizeof(T) == 8) {
+                node_state(src, x, y, nx, ny, N, BC, uinx, uiny, f, r, vx, vy);
+            } else {
+                float g[9];
+                Mo o;
+                node_state32<BC>(src, x, y, nx, ny, N, (float)uinx, (float)uiny, g, o);
+#pragma unroll
+                for (int k = 0; k < 9; k++) f[k] = (double)g[k] + WD[k];
+                ref::macro(f, r, vx, vy);
+            }
+            double q[9];
+            ref::feq(r, vx, vy, q);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                fa[k] += 0.25 * f[k];
+                na[k] += 0.25 * (f[k] - q[k]);
+            }
+        }
+    double rho, ux, uy;
+    ref::macro(fa, rho, ux, uy);
+    mc[nc] = rho;
+    mc[Nc + nc] = ux;
+    mc[2 * Nc + nc] = uy;
+    if (want_f) {
+        double q[9];
+        ref::feq(rho, ux, uy, q);
+#pragma unroll
+        for (int k = 0; k < 9; k++) fc[(size_t)k * Nc + nc] = q[k] + s * na[k];
+    }
+}
+
+// Labels: for coarse node p, the relaxation rate omega whose collision at
+// the nine upstream nodes q_k = p - e_k, once streamed into p, best matches
+// the target mass and momentum (least squares, closed form). This is the
+// search of FindAndWriteTau (mesh.cpp:2625-2715: recompute the upstream
+// collisions with tau + dtau, stream, compare with the high-fidelity
+// moments) solved exactly instead of over 20001 candidates. BGK: post =
+// f - omega (f - feq); RR: post = feq + f1 - omega f1 (f1 the regularised
+// non-equilibrium of NodeAndLattice.cpp:258-283).
+__global__ void __launch_bounds__(BX) k_labels(const double* __restrict__ fc, const double* __restrict__ mc,
+                                               const double* __restrict__ tgt, int nxc, int nyc, int rr,
+                                               const int4* __restrict__ excl, int nexcl, float* __restrict__ feat,
+                                               float* __restrict__ label, float* __restrict__ weight) {
+#pragma clang fp contract(off)
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y;
+    if (I >= nxc) return;
+    const unsigned Nc = (unsigned)nxc * nyc, p = (unsigned)J * nxc + I;
+    // features of p (the network's inputs, from p's own state)
+    double f[9], q[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) f[k] = fc[(size_t)k * Nc + p];
+    const double rho = mc[p], ux = mc[Nc + p], uy = mc[2 * Nc + p];
+    ref::feq(rho, ux, uy, q);
+    {
+        double d[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) d[k] = f[k] - q[k];
+        const double pxx = ((((d[1] + d[3]) + d[5]) + d[6]) + d[7]) + d[8];
+        const double pxy = ((d[5] - d[6]) + d[7]) - d[8];
+        const double pyy = ((((d[2] + d[4]) + d[5]) + d[6]) + d[7]) + d[8];
+        const double c = -3.0 / (2 * rho);
+        const double sxx = pxx * c, sxy = pxy * c, syy = pyy * c;
+        feat[6 * p + 0] = (float)rho;
+        feat[6 * p + 1] = (float)ux;
+        feat[6 * p + 2] = (float)uy;
+        feat[6 * p + 3] = (float)sqrt(2.0 * (sxx * sxx + 2.0 * sxy * sxy + syy * syy));
+        feat[6 * p + 4] = (float)(pxx - pyy);
+        feat[6 * p + 5] = (float)pxy;
+    }
+    bool ok = I >= 2 && I < nxc - 2 && J >= 2 && J < nyc - 2;
+    for (int b = 0; b < nexcl; b++) {
+        const int4 e = excl[b];
+        if (I >= e.x && I <= e.z && J >= e.y && J <= e.w) ok = false;
+    }
+    double a0 = 0, a1 = 0, a2 = 0, r0 = 0, r1 = 0, r2 = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const unsigned qn = (unsigned)(J - EY[k]) * nxc + (unsigned)(I - EX[k]);
+        const unsigned qq = ok ? qn : p;
+        double g[9], e[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) g[m] = fc[(size_t)m * Nc + qq];
+        const double rq = mc[qq], xq = mc[Nc + qq], yq = mc[2 * Nc + qq];
+        ref::feq(rq, xq, yq, e);
+        double slope, base;
+        if (rr) {
+            double d[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) d[m] = g[m] - e[m];
+            const double nxx = ((((d[1] + d[3]) + d[5]) + d[6]) + d[7]) + d[8];
+            const double nxy = ((d[5] - d[6]) + d[7]) - d[8];
+            const double nyy = ((((d[2] + d[4]) + d[5]) + d[6]) + d[7]) + d[8];
+            const double q0 = EX[k] * EX[k] - 1. / 3, q3 = EY[k] * EY[k] - 1. / 3, q1 = EX[k] * EY[k];
+            slope = 4.5 * WD[k] * (q0 * nxx + 2.0 * q1 * nxy + q3 * nyy);
+            base = e[k] + slope;
+        } else {
+            slope = g[k] - e[k];
+            base = g[k];
+        }
+        a0 += slope;
+        a1 += EX[k] * slope;
+        a2 += EY[k] * slope;
+        r0 += base;
+        r1 += EX[k] * base;
+        r2 += EY[k] * base;
+    }
+    const double tr = tgt[p], tx = tgt[Nc + p] * tr, ty = tgt[2 * Nc + p] * tr;
+    r0 -= tr;
+    r1 -= tx;
+    r2 -= ty;
+    const double aa = a0 * a0 + a1 * a1 + a2 * a2;
+    const double om = aa > 0 ? (a0 * r0 + a1 * r1 + a2 * r2) / aa : 0.0;
+    // the fit is linear in omega = 1/tau: labels stay in omega (no window on
+    // tau, which would cut the noise distribution one-sidedly near tau = 1/2)
+    ok = ok && aa > 1e-30 && om > 0.0 && om < 4.0;
+    label[p] = ok ? (float)om : 0.0f;
+    weight[p] = ok ? (float)aa : 0.0f;
 }
 
 // ----------------------------------------------------------- init kernels
@@ -1014,6 +1158,8 @@ StepArgs make_args(ippi_handle* h) {
     a.flag = h->flag;
     a.nbody = (int)h->bodies.size();
     a.nt = 2 * 9 * h->N * h->esz > ((size_t)192 << 20);
+    a.nn_max = h->c.tau_nn_max > 0 ? h->c.tau_nn_max : 1e30;
+    a.nn_maxf = (float)a.nn_max;
     a.mlp.frag = h->d_frag;
     a.mlp.blob = h->d_blob;
     for (int i = 0; i < 6; i++) {
@@ -1244,7 +1390,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     *out = nullptr;
     if (cfg->nx < 4 || cfg->ny < 4 || cfg->collision < 0 || cfg->collision > 2 || cfg->precision < 0 ||
         cfg->precision > 1 || cfg->bc < 0 || cfg->bc > 1 || cfg->tau_mode < 0 || cfg->tau_mode > 2 ||
-        !(cfg->tau0 > 0.5))
+        !(cfg->tau0 > 0.5) || cfg->tau_nn_max < 0)
         return IPPI_ERR_ARG;
     if (9 * (uint64_t)cfg->nx * (uint64_t)cfg->ny >= ((uint64_t)1 << 32)) return IPPI_ERR_ARG;
     ippi_handle* h = new ippi_handle();
@@ -1311,6 +1457,121 @@ int ippi_set_fields(ippi_handle* h, const double* rho, const double* ux, const d
     int r = init_common(h, 0, 0, 0, 0, 0, 0);
     if (r) return r;
     HIPC(h, hipStreamSynchronize(h->s));
+    return IPPI_OK;
+}
+
+
+// Training pairs at half this solver's resolution (see k_restrict / k_labels).
+int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, float* feat, float* label,
+                      float* weight) {
+    if (!h || !feat || !label || !weight) return IPPI_ERR_ARG;
+    if (h->c.nx % 2 || h->c.ny % 2 || h->c.nx < 16 || h->c.ny < 16) return fail(h, IPPI_ERR_ARG, "grid must be even and >= 16");
+    if (coarse_collision != IPPI_COLL_BGK && coarse_collision != IPPI_COLL_RR)
+        return fail(h, IPPI_ERR_ARG, "labels are defined for BGK and RR coarse collisions");
+    if (!(tau_coarse > 0.5)) return IPPI_ERR_ARG;
+    if (h->phase == 0) {
+        int r = ippi_step(h, 1);
+        if (r) return r;
+    }
+    const int nxc = h->c.nx / 2, nyc = h->c.ny / 2;
+    const size_t Nc = (size_t)nxc * nyc;
+    double *fc = nullptr, *mc = nullptr, *tg = nullptr;
+    float *df = nullptr, *dl = nullptr, *dw = nullptr;
+    int4* dex = nullptr;
+    std::vector<int4> ex;
+    for (const auto& b : h->bodies)  // coarse cells covered by a body's support, plus a margin of 2
+        ex.push_back(make_int4(b.p.x0 / 2 - 2, b.p.y0 / 2 - 2, (b.p.x0 + b.p.w) / 2 + 2, (b.p.y0 + b.p.h) / 2 + 2));
+    int rc = IPPI_OK;
+    auto cleanup = [&]() {
+        for (void* q : {(void*)fc, (void*)mc, (void*)tg, (void*)df, (void*)dl, (void*)dw, (void*)dex})
+            if (q) (void)hipFree(q);
+    };
+#define PCK(expr)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            cleanup();                                                                         \
+            return fail(h, IPPI_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+        }                                                                                      \
+    } while (0)
+    PCK(hipMalloc(&fc, sizeof(double) * 9 * Nc));
+    PCK(hipMalloc(&mc, sizeof(double) * 3 * Nc));
+    PCK(hipMalloc(&tg, sizeof(double) * 3 * Nc));
+    PCK(hipMalloc(&df, sizeof(float) * 6 * Nc));
+    PCK(hipMalloc(&dl, sizeof(float) * Nc));
+    PCK(hipMalloc(&dw, sizeof(float) * Nc));
+    PCK(hipMalloc(&dex, sizeof(int4) * std::max<size_t>(1, ex.size())));
+    if (!ex.empty()) PCK(hipMemcpy(dex, ex.data(), sizeof(int4) * ex.size(), hipMemcpyHostToDevice));
+    const double sc = 2.0 * tau_coarse / h->c.tau0;
+    dim3 grid((nxc + BX - 1) / BX, nyc);
+    auto restrict_ = [&](double* f_out, double* m_out) {
+        const bool pb = h->c.bc == IPPI_BC_PERIODIC_BB;
+        if (h->esz == 8) {
+            const double* src = (const double*)h->buf[h->cur];
+            if (pb)
+                hipLaunchKernelGGL((k_restrict<double, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
+                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0);
+            else
+                hipLaunchKernelGGL((k_restrict<double, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
+                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0);
+        } else {
+            const float* src = (const float*)h->buf[h->cur];
+            if (pb)
+                hipLaunchKernelGGL((k_restrict<float, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
+                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0);
+            else
+                hipLaunchKernelGGL((k_restrict<float, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
+                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0);
+        }
+        return hipGetLastError();
+    };
+    PCK(restrict_(fc, mc));
+    rc = ippi_step(h, 2);  // one coarse step = two fine steps
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    PCK(restrict_(nullptr, tg));
+    hipLaunchKernelGGL(k_labels, grid, dim3(BX), 0, h->s, (const double*)fc, (const double*)mc, (const double*)tg, nxc,
+                       nyc, coarse_collision == IPPI_COLL_RR ? 1 : 0, (const int4*)dex, (int)ex.size(), df, dl, dw);
+    PCK(hipGetLastError());
+    PCK(hipMemcpyAsync(feat, df, sizeof(float) * 6 * Nc, hipMemcpyDeviceToHost, h->s));
+    PCK(hipMemcpyAsync(label, dl, sizeof(float) * Nc, hipMemcpyDeviceToHost, h->s));
+    PCK(hipMemcpyAsync(weight, dw, sizeof(float) * Nc, hipMemcpyDeviceToHost, h->s));
+    PCK(hipStreamSynchronize(h->s));
+#undef PCK
+    cleanup();
+    return check_flag(h);
+}
+
+// Populations of the post-streaming state, [k][y][x] (FP32 path: stored as
+// f - w); moments from CalMacro (NodeAndLattice.cpp:196-226).
+int ippi_set_populations(ippi_handle* h, const double* f) {
+    if (!h || !f) return IPPI_ERR_ARG;
+    std::vector<double> m(3 * h->N);
+    for (size_t n = 0; n < h->N; n++) {
+        double q[9], r, ux, uy;
+        for (int k = 0; k < 9; k++) q[k] = f[k * h->N + n];
+        ref::macro(q, r, ux, uy);
+        m[n] = r;
+        m[h->N + n] = ux;
+        m[2 * h->N + n] = uy;
+    }
+    HIPC(h, hipStreamSynchronize(h->s));
+    if (h->esz == 8) {
+        HIPC(h, hipMemcpy(h->buf[0], f, sizeof(double) * 9 * h->N, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> g(9 * h->N);
+        for (size_t k = 0; k < 9; k++)
+            for (size_t n = 0; n < h->N; n++) g[k * h->N + n] = (float)(f[k * h->N + n] - WD[k]);
+        HIPC(h, hipMemcpy(h->buf[0], g.data(), sizeof(float) * 9 * h->N, hipMemcpyHostToDevice));
+    }
+    HIPC(h, hipMemcpy(h->mac, m.data(), sizeof(double) * 3 * h->N, hipMemcpyHostToDevice));
+    HIPC(h, hipMemset(h->flag, 0, sizeof(int)));
+    h->cur = 0;
+    h->phase = 0;
+    h->steps = 0;
+    h->nrec = 0;
     return IPPI_OK;
 }
 

@@ -318,9 +318,9 @@ double or_mlp_loss_grad(const double* p, const double* xs, const double* ys, int
     for (int s = 0; s < n; s++) {
         double z1[NH], h1[NH], z2[NH], h2[NH], x[NIN], o;
         double t = mlp_fwd(p, xs + (size_t)s * NIN, z1, h1, z2, h2, x, &o);
-        double r = t - ys[s];
+        double r = 1.0 / t - ys[s];
         loss += 0.5 * r * r;
-        double dt = r / n;
+        double dt = -r / (t * t) / n;
         double dout = dt * sigm(o);
         g[OFF_B3] += dout;
         double dz2[NH];
@@ -541,7 +541,7 @@ static void collide_phase(or_sim* s) {
             double feat[6];
             or_mlp_features(f, rho, ux, uy, feat);
             double t = or_mlp_forward(s->mlp, feat);
-            if (t >= 0.501) {
+            if (t >= 0.501 && (c->tau_nn_max <= 0 || t <= c->tau_nn_max)) {
                 tau = t;
                 nn_ok = 1;
             }

@@ -38,6 +38,7 @@ typedef struct {
     double cs;       /* Smagorinsky constant */
     double u_in_x;   /* inlet velocity for OR_BC_INFLOW */
     double u_in_y;
+    double tau_nn_max;  /* network values above this fall back too (0 = none) */
 } or_config;
 
 /* ---- per-node kernels (NodeAndLattice.cpp, mesh.cpp) ---- */
@@ -58,7 +59,7 @@ double or_noise(uint64_t seed, int64_t node, int comp);
 #define OR_MLP_NPARAM (OR_MLP_NIN * OR_MLP_NH + OR_MLP_NH + OR_MLP_NH * OR_MLP_NH + OR_MLP_NH + OR_MLP_NH + 1)
 void   or_mlp_features(const double f[9], double rho, double ux, double uy, double feat[6]);
 double or_mlp_forward(const double* params, const double in[6]);
-/* loss = mean_i 0.5*(tau(x_i) - y_i)^2 ; returns loss, writes dL/dparams */
+/* loss = mean_i 0.5*(1/tau(x_i) - y_i)^2 (y: relaxation rates); returns loss, writes dL/dparams */
 double or_mlp_loss_grad(const double* params, const double* x, const double* y, int n, double* grad);
 
 /* ---- whole-domain solver ---- */

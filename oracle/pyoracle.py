@@ -26,6 +26,7 @@ class OrConfig(ctypes.Structure):
         ("bc", ctypes.c_int), ("les", ctypes.c_int), ("tau_mode", ctypes.c_int),
         ("tau0", ctypes.c_double), ("cs", ctypes.c_double),
         ("u_in_x", ctypes.c_double), ("u_in_y", ctypes.c_double),
+        ("tau_nn_max", ctypes.c_double),
     ]
 
 
@@ -163,11 +164,11 @@ class Sim:
     """FP64 CPU solver; same configuration vocabulary as the HIP library."""
 
     def __init__(self, nx, ny, collision="bgk", bc="periodic_bb", les=False, tau_mode="const",
-                 tau0=0.6, cs=0.0, u_in=(0.0, 0.0), threads=1):
+                 tau0=0.6, cs=0.0, u_in=(0.0, 0.0), threads=1, tau_nn_max=0.0):
         self.nx, self.ny = nx, ny
         self.N = nx * ny
         cfg = OrConfig(nx, ny, COLL[collision], BC[bc], int(bool(les)), TAU[tau_mode],
-                       tau0, cs, u_in[0], u_in[1])
+                       tau0, cs, u_in[0], u_in[1], tau_nn_max)
         lib().or_set_threads(int(threads))
         self.h = lib().or_create(ctypes.byref(cfg))
 

@@ -33,6 +33,6 @@ def test_binding_covers_header():
 
 def test_config_struct_layout():
     ippi = _pkg.load()
-    # 7 ints (28 B, padded to 32), 4 doubles, 2 ints
-    assert ctypes.sizeof(ippi.Config) == 72
+    # 7 ints (28 B, padded to 32), 4 doubles, 2 ints, 1 double
+    assert ctypes.sizeof(ippi.Config) == 80
     assert ippi.library().ippi_version() == 1
