@@ -31,6 +31,7 @@
  *   ippi_get_body_forces  OutputCd                   mesh.cpp:1461-1526
  *   ippi_mass          ErrorCatch / global checks    mesh.cpp:2101-2111
  *   ippi_destroy       Mesh::~Mesh                   mesh.cpp:52-55
+ *   ippi_adjoint_drag  (none: gradient of the averaged drag, see its comment)
  */
 #ifndef IPPI_LBM_H
 #define IPPI_LBM_H
@@ -164,6 +165,40 @@ int ippi_trainer_adam(ippi_trainer* t, int iters, int batch, double lr, uint64_t
 int ippi_set_timing(ippi_handle* h, int enable);
 int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, double* ib_ms,
                     int64_t* ib_launches);
+
+/* Discrete adjoint of the time-averaged body force (no counterpart in the
+ * reference, which has no gradient code; it differentiates the step that
+ * ippi_step computes: collision + Guo source NodeAndLattice.cpp:150-195,
+ * streaming mesh.cpp:192-226, IB mesh.cpp:514-741,1244-1281).
+ * Runs nsteps forward steps from the handle's current state (the handle
+ * ends advanced by nsteps), keeping a checkpoint of the state every `ckpt`
+ * steps, then the reverse sweep, which recomputes each segment of ckpt
+ * steps from its checkpoint. Objective, over the force records of this
+ * run's steps avg_from..nsteps (1-based):
+ *     J = sum_b  w[2b] mean(Fx_b) + w[2b+1] mean(Fy_b)
+ * with (Fx_b, Fy_b) the body-force record of ippi_get_body_forces (minus
+ * the sum of the marker forces); body_w = NULL selects the drag of body 0.
+ * Outputs (each may be NULL): *J; g_tau[nx*ny] = dJ/d(base tau) per node
+ * (the per-node field, or the local sensitivity to tau0 in IPPI_TAU_CONST
+ * mode, whose sum is dJ/dtau0); g_center[2*nbody] = dJ/d(cx, cy) of each
+ * body (all markers translate with the centre). The adjoint populations
+ * and arithmetic use the handle's precision, or FP64 over an FP32 forward
+ * state when flags has IPPI_ADJ_FP64 (the adjoint accumulates the
+ * sensitivity of every later step, and FP32 storage drops the small late
+ * increments: measured on 1024x512 / 4000 steps, FP32 adjoint vs FP64
+ * adjoint 3e-5 (tau field) and up to 4e-4 (centre), FP32 forward with FP64
+ * adjoint 6e-6 and 1e-5). The tau gradient is summed in FP64 per
+ * checkpoint segment; the centre gradient in FP64.
+ * Supported: IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB, les = 0, tau_mode CONST
+ * or FIELD, at least one body; otherwise IPPI_ERR_ARG / IPPI_ERR_STATE.
+ * Device memory: (ceil(nsteps/ckpt) + ckpt - 1) state arrays + 2 adjoint
+ * population arrays. */
+#define IPPI_ADJ_FP64 1
+int ippi_adjoint_drag(ippi_handle* h, int nsteps, int avg_from, int ckpt, int flags, const double* body_w,
+                      double* J, double* g_tau, double* g_center);
+/* Summed duration (ms) and count of the adjoint's all-node kernel launches
+ * recorded while timing was enabled; clears them. */
+int ippi_get_adjoint_timing(ippi_handle* h, double* ms, int64_t* launches);
 
 /* Checkpoint: the full device state as a flat byte image. *size gives the
  * required size when buf is NULL. */

@@ -22,6 +22,7 @@ PRECISION = {"fp64": 0, "fp32": 1}
 BC = {"periodic_bb": 0, "inflow": 1}
 TAU_MODE = {"const": 0, "field": 1, "nn": 2}
 MLP_BLOB = 6 * 32 + 32 + 32 * 32 + 32 + 32 + 1 + 12
+IPPI_ADJ_FP64 = 1
 
 
 class IppiError(RuntimeError):
@@ -84,6 +85,9 @@ _SIGS = {
     "ippi_trainer_loss_grad": ([ctypes.c_void_p, _D, _D], ctypes.c_int),
     "ippi_trainer_adam": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D],
                           ctypes.c_int),
+    "ippi_adjoint_drag": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D],
+                          ctypes.c_int),
+    "ippi_get_adjoint_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "ippi_save": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "ippi_load": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
 }
@@ -257,6 +261,31 @@ class Solver:
         self._check(library().ippi_get_timing(self.h, ctypes.byref(sm), ctypes.byref(sc), ctypes.byref(im),
                                               ctypes.byref(ic)))
         return sm.value, sc.value, im.value, ic.value
+
+    def adjoint_drag(self, nsteps, avg_from, ckpt=100, body_w=None, fp64_adjoint=False):
+        """Run nsteps steps and the discrete adjoint of the averaged body force.
+
+        Returns (J, dJ/dtau [ny, nx], dJ/d(cx, cy) [nbody, 2]); J is the mean
+        over steps avg_from..nsteps of sum_b w_b . record_b (drag of body 0
+        when body_w is None). The solver ends advanced by nsteps.
+        fp64_adjoint: FP64 adjoint over an FP32 forward state.
+        """
+        nb = ctypes.c_int()
+        self._check(library().ippi_num_bodies(self.h, ctypes.byref(nb)))
+        w = None if body_w is None else _f64(body_w, 2 * nb.value)
+        J = ctypes.c_double()
+        gt = np.zeros(self.N)
+        gc = np.zeros(2 * max(nb.value, 1))
+        self._check(library().ippi_adjoint_drag(self.h, int(nsteps), int(avg_from), int(ckpt),
+                                                IPPI_ADJ_FP64 if fp64_adjoint else 0,
+                                                None if w is None else _p(w), ctypes.byref(J), _p(gt), _p(gc)))
+        return J.value, gt.reshape(self.ny, self.nx), gc[: 2 * nb.value].reshape(nb.value, 2)
+
+    def adjoint_timing(self):
+        """(ms, launches) of the adjoint's all-node kernel since the last call."""
+        ms, c = ctypes.c_double(), ctypes.c_int64()
+        self._check(library().ippi_get_adjoint_timing(self.h, ctypes.byref(ms), ctypes.byref(c)))
+        return ms.value, c.value
 
     def save(self):
         n = ctypes.c_size_t()

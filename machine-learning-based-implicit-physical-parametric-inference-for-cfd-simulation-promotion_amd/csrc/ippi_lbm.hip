@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/ippi_lbm.h"
+#include "adjoint.hpp"
 #include "lbm_node.hpp"
 #include "mlp.hpp"
 
@@ -1042,6 +1043,275 @@ __global__ void k_sum_partial(const double* __restrict__ v, size_t n, double* __
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
+// ----------------------------------------------------------------- adjoint
+// Discrete adjoint of the BGK + IB step with periodic x and bounce-back
+// walls (node-local algebra in adjoint.hpp). The adjoint state is the
+// adjoint of the stored post-collision populations; one reverse step runs
+//   k_adj_act: force adjoint at every active (marker-support) node,
+//   k_adj_mk:  marker transpose per body (force -> interpolation), centre
+//              gradient, gather of the velocity adjoint back to the nodes,
+//   k_adj_step: all nodes: reverse pull, collision / macro transpose, tau
+//              gradient accumulation.
+// A spatially constant adjoint (the same value for every population at
+// every node) is mapped to itself by the transposed step (the adjoint of
+// mass conservation) and contributes to no gradient. Every reverse step
+// subtracts the adjoint of population 0 at node 0 from all populations as
+// they are loaded, so no common offset builds up in the stored adjoint;
+// the gradients are unchanged.
+
+// Transpose of pull<T, IPPI_BC_PERIODIC_BB>: the adjoint of population k at
+// node m is population k of the downstream node m + e_k, or, where m + e_k
+// lies beyond a wall, the reversed population at m itself.
+template <typename T>
+__device__ __forceinline__ void pull_adj(const T* __restrict__ s, int x, int y, int nx, int ny, unsigned N, T* l) {
+    const bool bot = y == 0, top = y == ny - 1;
+    const unsigned xc[3] = {(unsigned)(x == 0 ? nx - 1 : x - 1), (unsigned)x, (unsigned)(x == nx - 1 ? 0 : x + 1)};
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int pk = k, dy = EY[k], dx = EX[k];
+        if ((EY[k] == -1 && bot) || (EY[k] == 1 && top)) {
+            pk = OPP[k];
+            dy = 0;
+            dx = 0;
+        }
+        l[k] = ld(s + (unsigned)pk * N, (unsigned)(y + dy) * nx + xc[dx + 1]);
+    }
+}
+
+struct AdjArgs {
+    int nx, ny;
+    double tau0;
+    int tau_field;
+    const void* tau_ptr;
+    const double* patch_f;  // IB force of the step [patch node][2]
+    const double* patch_l;  // velocity adjoint from the interpolation [patch node][2]
+    int nbody;
+    int store;              // write the adjoint populations
+    Patch patch[IPPI_MAXB];
+};
+
+struct AdjBody {
+    int m0, M, a0, nact;
+    double wx, wy;  // objective weights of the body's force record
+};
+
+struct AdjIb {
+    int nact, E;
+    const int* act;       // active node -> patch node
+    const int* act_node;  // active node -> y * nx + x
+    double* lf;           // [active][2] force adjoint
+    double* u;            // [active][2] velocity seen by the interpolation
+    const int* pidx;      // [m][16] support -> active index within the body
+    const double* w;      // [m][16] phi phi
+    const double* wx;     // [m][16] phi'(x - X) phi(y - Y)
+    const double* wy;     // [m][16] phi(x - X) phi'(y - Y)
+    const double* F;      // [m][2] marker forces of the step
+    const int* ecnt;
+    const int* emk;
+    const double* ecw;
+    const AdjBody* body;
+    double* patch_l;      // [patch node][2]
+    double* gc;           // [body][2] centre gradient (accumulated)
+};
+
+__device__ __forceinline__ int adj_patch(const AdjArgs& a, int x, int y, int bx0, int bx1) {
+    for (int b = 0; b < a.nbody; ++b) {
+        const Patch p = a.patch[b];
+        if (bx1 < p.x0 || bx0 >= p.x0 + p.w || y < p.y0 || y >= p.y0 + p.h) continue;
+        if (x >= p.x0 && x < p.x0 + p.w) return p.off + (y - p.y0) * p.w + (x - p.x0);
+    }
+    return -1;
+}
+
+template <typename T>
+__device__ __forceinline__ void adj_eq(T drho, T rho, T ux, T uy, T* q) {
+    if constexpr (sizeof(T) == 8)
+        ref::feq(rho, ux, uy, q);
+    else
+        sh::geq(drho, rho, ux, uy, q);
+}
+
+// Forward state of the step at node (x, y): pulled populations (or the
+// stored first state with the macro arrays), density and velocity.
+template <typename T, bool PULL>
+__device__ __forceinline__ void adj_fwd(const T* __restrict__ src, const double* __restrict__ mac, int x, int y,
+                                        int nx, int ny, unsigned N, T* f, T& drho, T& rho, T& ux, T& uy) {
+    load_raw<T, IPPI_BC_PERIODIC_BB, PULL>(src, x, y, nx, ny, N, f);
+    const unsigned n = (unsigned)y * nx + x;
+    if (PULL) {
+        T s = (f[0] + (f[1] + f[3])) + ((f[2] + f[4]) + ((f[5] + f[7]) + (f[6] + f[8])));
+        T jx = ((f[1] - f[3]) + (f[5] - f[7])) + (f[8] - f[6]);
+        T jy = ((f[2] - f[4]) + (f[5] - f[7])) + (f[6] - f[8]);
+        if constexpr (sizeof(T) == 8) {
+            rho = s;
+            drho = s - 1.0;
+        } else {
+            drho = s;
+            rho = 1.0f + s;
+        }
+        if (rho < T(1e-6)) {
+            ux = T(0);
+            uy = T(0);
+        } else {
+            ux = jx / rho;
+            uy = jy / rho;
+        }
+    } else {
+        rho = (T)mac[n];
+        drho = (T)(mac[n] - 1.0);
+        ux = (T)mac[N + n];
+        uy = (T)mac[2 * N + n];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T adj_tau(const AdjArgs& a, unsigned n) {
+    return a.tau_field ? ld(static_cast<const T*>(a.tau_ptr), n) : (T)a.tau0;
+}
+
+// S: type of the stored forward populations, A: adjoint arithmetic and
+// storage.
+template <typename S, typename A, bool PULL>
+__global__ void __launch_bounds__(256) k_adj_act(const A* __restrict__ lin, const S* __restrict__ src,
+                                                 const double* __restrict__ mac, AdjIb d, AdjArgs a) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.nact) return;
+    const int nx = a.nx, ny = a.ny;
+    const unsigned N = (unsigned)nx * ny;
+    const int node = d.act_node[i], r = d.act[i];
+    const int x = node % nx, y = node / nx;
+    A l[9];
+    S f[9], drho, rho, ux, uy;
+    pull_adj(lin, x, y, nx, ny, N, l);
+    const A c0 = lin[0];
+#pragma unroll
+    for (int k = 0; k < 9; k++) l[k] -= c0;
+    adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, f, drho, rho, ux, uy);
+    const A om = A(1) / (A)adj_tau<S>(a, (unsigned)node);
+    const adj::NodeFwd<A> s{(A)rho, (A)ux, (A)uy, (A)a.patch_f[2 * r], (A)a.patch_f[2 * r + 1], om,
+                            A(1) - A(0.5) * om};
+    const adj::WMom<A> m = adj::wmoments(l);
+    const adj::UsAdj<A> ua = adj::us_adjoint(m, s, true);
+    A lfx, lfy;
+    adj::force_adjoint(ua, s, lfx, lfy);
+    d.lf[2 * i] = (double)lfx;
+    d.lf[2 * i + 1] = (double)lfy;
+    d.u[2 * i] = (double)ux;
+    d.u[2 * i + 1] = (double)uy;
+}
+
+// One block per body. cj = weight of this step's force record in the
+// objective.
+__global__ void __launch_bounds__(IB_T) k_adj_mk(AdjIb d, double cj) {
+    __shared__ double LU[2 * IB_MAXM];
+    __shared__ double red[2 * IB_T];
+    const AdjBody b = d.body[blockIdx.x];
+    const int t = threadIdx.x;
+    double gx = 0.0, gy = 0.0;
+    for (int ml = t; ml < b.M; ml += IB_T) {
+        const int m = b.m0 + ml;
+        double ax = -cj * b.wx, ay = -cj * b.wy;  // record = - sum of marker forces
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const int i = b.a0 + d.pidx[16 * m + s];
+            ax += d.w[16 * m + s] * d.lf[2 * i];
+            ay += d.w[16 * m + s] * d.lf[2 * i + 1];
+        }
+        const double lux = -2.0 * ax, luy = -2.0 * ay;  // F = 2 (U_d - U)
+        LU[2 * ml] = lux;
+        LU[2 * ml + 1] = luy;
+        const double Fx = d.F[2 * m], Fy = d.F[2 * m + 1];
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            const int i = b.a0 + d.pidx[16 * m + s];
+            const double c = lux * d.u[2 * i] + luy * d.u[2 * i + 1] + d.lf[2 * i] * Fx + d.lf[2 * i + 1] * Fy;
+            gx += d.wx[16 * m + s] * c;
+            gy += d.wy[16 * m + s] * c;
+        }
+    }
+    red[2 * t] = gx;
+    red[2 * t + 1] = gy;
+    __syncthreads();
+    for (int off = IB_T / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            red[2 * t] += red[2 * (t + off)];
+            red[2 * t + 1] += red[2 * (t + off) + 1];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        d.gc[2 * blockIdx.x] += red[0];
+        d.gc[2 * blockIdx.x + 1] += red[1];
+    }
+    // velocity adjoint at the support nodes: the spread's gather lists
+    for (int il = t; il < b.nact; il += IB_T) {
+        const int i = b.a0 + il;
+        const int cnt = d.ecnt[i];
+        double sx = 0.0, sy = 0.0;
+        for (int e = 0; e < cnt; e++) {
+            const int mk = d.emk[(size_t)d.E * i + e];
+            const double w = d.ecw[(size_t)d.E * i + e];
+            sx += w * LU[2 * mk];
+            sy += w * LU[2 * mk + 1];
+        }
+        const int r = d.act[i];
+        d.patch_l[2 * r] = sx;
+        d.patch_l[2 * r + 1] = sy;
+    }
+}
+
+template <typename S, typename A, bool PULL>
+__global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* __restrict__ lout,
+                                                 const S* __restrict__ src, const double* __restrict__ mac,
+                                                 A* __restrict__ gacc, AdjArgs a) {
+    const int nx = a.nx, ny = a.ny;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y;
+    if (x >= nx) return;
+    const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
+    A l[9];
+    S f[9], drho, rho, ux, uy;
+    pull_adj(lin, x, y, nx, ny, N, l);
+    const A c0 = lin[0];  // constant-mode removal
+#pragma unroll
+    for (int k = 0; k < 9; k++) l[k] -= c0;
+    adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, f, drho, rho, ux, uy);
+    A fx = A(0), fy = A(0), lix = A(0), liy = A(0);
+    const int bx0 = blockIdx.x * BX;
+    const int r = a.nbody > 0 ? adj_patch(a, x, y, bx0, bx0 + BX - 1) : -1;
+    const bool forced = r >= 0;
+    if (forced) {
+        fx = (A)a.patch_f[2 * r];
+        fy = (A)a.patch_f[2 * r + 1];
+        lix = (A)a.patch_l[2 * r];
+        liy = (A)a.patch_l[2 * r + 1];
+    }
+    const A om = A(1) / (A)adj_tau<S>(a, n);
+    const adj::NodeFwd<A> s{(A)rho, (A)ux, (A)uy, fx, fy, om, A(1) - A(0.5) * om};
+    const adj::WMom<A> m = adj::wmoments(l);
+    const adj::UsAdj<A> ua = adj::us_adjoint(m, s, forced);
+    // non-equilibrium part in the representation of the stored populations
+    S q[9];
+    adj_eq(drho, rho, (S)ua.usx, (S)ua.usy, q);
+    A dd[9], nu[9], dt;
+#pragma unroll
+    for (int k = 0; k < 9; k++) dd[k] = (A)(f[k] - q[k]);
+    adj::node_adjoint(l, dd, m, ua, s, forced, lix, liy, nu, dt);
+    if (a.store) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) lout[k * N + n] = nu[k];
+    }
+    gacc[n] += dt;
+}
+
+// total += part; part = 0 (per-segment tau-gradient partial sums).
+template <typename T>
+__global__ void k_adj_flush(T* __restrict__ part, double* __restrict__ total, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    total[i] += (double)part[i];
+    part[i] = T(0);
+}
+
 }  // namespace
 
 // ====================================================================== host
@@ -1074,6 +1344,8 @@ struct ippi_handle {
     int ell = 8;
     double *d_w = nullptr, *d_ud = nullptr, *d_ui = nullptr, *d_F = nullptr, *d_ecw = nullptr, *d_pf = nullptr;
     int npn = 0, nact = 0, nblk_ib = 0;
+    std::vector<int> h_act;      // active node -> patch node (host copy)
+    std::vector<int> body_act0;  // first active node of each body (+ total)
     double* hist = nullptr;
     int hist_cap = 0;
     int64_t nrec = 0;
@@ -1084,7 +1356,7 @@ struct ippi_handle {
     float mlp_mu[6] = {0}, mlp_isig[6] = {0}, mlp_b3 = 0.f;
     // optional per-launch timing (HIP events on this handle's stream)
     int timing = 0;
-    std::vector<hipEvent_t> ev_step, ev_ib;  // start/end pairs
+    std::vector<hipEvent_t> ev_step, ev_ib, ev_adj;  // start/end pairs
     std::string err;
 };
 
@@ -1209,9 +1481,11 @@ int upload_ib(ippi_handle* h) {
     std::vector<int> act, ecnt, emk, pidx(16 * (size_t)nm), slot(off, -1);
     std::vector<double> ecw;
     std::vector<IbBlk> blk;
+    std::vector<int> act0s;
     for (int bi = 0; bi < nb; bi++) {
         const auto& b = h->bodies[bi];
         const int a0 = (int)act.size();
+        act0s.push_back(a0);
         for (int r = b.p.off; r < b.p.off + b.p.w * b.p.h; r++) {
             if (rows[r].empty()) continue;
             slot[r] = (int)act.size() - a0;
@@ -1225,6 +1499,7 @@ int upload_ib(ippi_handle* h) {
         }
         const int a1 = (int)act.size();
         if (a1 - a0 > IB_MAXACT) return fail(h, IPPI_ERR_ARG, "body support too large (active nodes > 3072)");
+        if (bi == nb - 1) act0s.push_back(a1);
         for (int m = b.first; m < b.first + b.count; m++)
             for (int i = 0; i < 4; i++)
                 for (int j = 0; j < 4; j++) {
@@ -1237,6 +1512,8 @@ int upload_ib(ippi_handle* h) {
     }
     h->nact = (int)act.size();
     h->nblk_ib = (int)blk.size();
+    h->h_act = act;
+    h->body_act0 = act0s;
     free_ib(h);
     HIPC(h, hipMalloc(&h->d_pidx, sizeof(int) * 16 * nm));
     HIPC(h, hipMalloc(&h->d_w, sizeof(double) * 16 * nm));
@@ -1302,6 +1579,30 @@ void clear_events(std::vector<hipEvent_t>& v) {
     for (auto e : v)
         if (e) (void)hipEventDestroy(e);
     v.clear();
+}
+
+// Total time of the (start, end) event pairs in v; clears v.
+int sum_events(ippi_handle* h, std::vector<hipEvent_t>& v, double* ms, int64_t* cnt) {
+    double t = 0.0;
+    int64_t c = 0;
+    for (size_t i = 0; i + 1 < v.size(); i += 2) {
+        if (!v[i] || !v[i + 1]) {
+            clear_events(v);
+            return fail(h, IPPI_ERR_HIP, "event creation failed");
+        }
+        float e = 0.f;
+        hipError_t err = hipEventElapsedTime(&e, v[i], v[i + 1]);
+        if (err != hipSuccess) {
+            clear_events(v);
+            return fail(h, IPPI_ERR_HIP, std::string("hipEventElapsedTime: ") + hipGetErrorString(err));
+        }
+        t += e;
+        c++;
+    }
+    if (ms) *ms = t;
+    if (cnt) *cnt = c;
+    clear_events(v);
+    return IPPI_OK;
 }
 
 template <typename T>
@@ -1375,6 +1676,198 @@ int macro(ippi_handle* h, double* f_out) {
     return h->esz == 8 ? run_macro<double>(h, f_out) : run_macro<float>(h, f_out);
 }
 
+// ------------------------------------------------------------ adjoint run
+// Scratch of one adjoint run, freed when it ends.
+struct AdjBufs {
+    std::vector<void*> ptrs;
+    ~AdjBufs() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename P>
+    hipError_t alloc(P** p, size_t bytes) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes > 0 ? bytes : 16);
+        if (e == hipSuccess) ptrs.push_back(q);
+        *p = (P*)q;
+        return e;
+    }
+};
+
+template <typename T>
+void launch_ib_to(ippi_handle* h, const T* src, bool pull, const IbDev& d) {
+    dim3 g(h->nblk_ib), b(IB_T);
+    const int nx = h->c.nx, ny = h->c.ny;
+    if (h->ell == 8) {
+        if (pull)
+            hipLaunchKernelGGL((k_ib<T, true, 8>), g, b, 0, h->s, src, h->mac, d, nx, ny);
+        else
+            hipLaunchKernelGGL((k_ib<T, false, 8>), g, b, 0, h->s, src, h->mac, d, nx, ny);
+    } else {
+        if (pull)
+            hipLaunchKernelGGL((k_ib<T, true, 16>), g, b, 0, h->s, src, h->mac, d, nx, ny);
+        else
+            hipLaunchKernelGGL((k_ib<T, false, 16>), g, b, 0, h->s, src, h->mac, d, nx, ny);
+    }
+}
+
+template <typename T, typename A>
+int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double* bw, double* J, double* g_tau,
+                double* g_c) {
+    const int nx = h->c.nx, ny = h->c.ny, nb = (int)h->bodies.size();
+    const size_t N = h->N, SZ = 9 * N;
+    int r = upload_ib(h);
+    if (r) return r;
+    const int nwin = nsteps - avg_from + 1;
+    if (nwin > h->hist_cap) return fail(h, IPPI_ERR_ARG, "averaging window longer than the force history");
+    const int nm = (int)(h->mk_ud.size() / 2);
+    // marker weight derivatives and the per-body tables
+    std::vector<double> wx(16 * (size_t)nm), wy(16 * (size_t)nm);
+    for (int m = 0; m < nm; m++)
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                const double dx = h->mk_xy[2 * m] - (h->mk_base[2 * m] + i);
+                const double dy = h->mk_xy[2 * m + 1] - (h->mk_base[2 * m + 1] + j);
+                wx[16 * m + 4 * i + j] = adj::ddelta4(dx) * delta4(dy);
+                wy[16 * m + 4 * i + j] = delta4(dx) * adj::ddelta4(dy);
+            }
+    std::vector<int> act_node(h->nact);
+    std::vector<AdjBody> tab(nb);
+    for (int b = 0; b < nb; b++) {
+        const Patch& p = h->bodies[b].p;
+        for (int i = h->body_act0[b]; i < h->body_act0[b + 1]; i++) {
+            const int q = h->h_act[i] - p.off;
+            act_node[i] = (p.y0 + q / p.w) * nx + p.x0 + q % p.w;
+        }
+        tab[b] = AdjBody{h->bodies[b].first, h->bodies[b].count, h->body_act0[b],
+                         h->body_act0[b + 1] - h->body_act0[b], bw ? bw[2 * b] : (b == 0 ? 1.0 : 0.0),
+                         bw ? bw[2 * b + 1] : 0.0};
+    }
+    const int K = std::min(ckpt, nsteps);
+    const int nseg = (nsteps + K - 1) / K;
+    AdjBufs mem;
+    T *ck = nullptr, *seg = nullptr;
+    A *lam = nullptr, *gacc = nullptr;
+    double *pf = nullptr, *Fs = nullptr, *ui = nullptr, *gtot = nullptr, *lf = nullptr, *uact = nullptr;
+    double *pl = nullptr, *gc = nullptr, *dwx = nullptr, *dwy = nullptr;
+    int* dan = nullptr;
+    AdjBody* dtab = nullptr;
+    HIPC(h, mem.alloc(&ck, sizeof(T) * SZ * nseg));
+    HIPC(h, mem.alloc(&seg, sizeof(T) * SZ * (K - 1)));
+    HIPC(h, mem.alloc(&lam, sizeof(A) * SZ * 2));
+    HIPC(h, mem.alloc(&gacc, sizeof(A) * N));
+    HIPC(h, mem.alloc(&gtot, sizeof(double) * N));
+    HIPC(h, mem.alloc(&pf, sizeof(double) * 2 * (size_t)h->npn * K));
+    HIPC(h, mem.alloc(&Fs, sizeof(double) * 2 * (size_t)nm * K));
+    HIPC(h, mem.alloc(&ui, sizeof(double) * 2 * (size_t)nm));
+    HIPC(h, mem.alloc(&lf, sizeof(double) * 2 * (size_t)h->nact));
+    HIPC(h, mem.alloc(&uact, sizeof(double) * 2 * (size_t)h->nact));
+    HIPC(h, mem.alloc(&pl, sizeof(double) * 2 * (size_t)h->npn));
+    HIPC(h, mem.alloc(&gc, sizeof(double) * 2 * nb));
+    HIPC(h, mem.alloc(&dwx, sizeof(double) * 16 * (size_t)nm));
+    HIPC(h, mem.alloc(&dwy, sizeof(double) * 16 * (size_t)nm));
+    HIPC(h, mem.alloc(&dan, sizeof(int) * (size_t)h->nact));
+    HIPC(h, mem.alloc(&dtab, sizeof(AdjBody) * nb));
+    HIPC(h, hipMemcpyAsync(dwx, wx.data(), sizeof(double) * 16 * nm, hipMemcpyHostToDevice, h->s));
+    HIPC(h, hipMemcpyAsync(dwy, wy.data(), sizeof(double) * 16 * nm, hipMemcpyHostToDevice, h->s));
+    HIPC(h, hipMemcpyAsync(dan, act_node.data(), sizeof(int) * h->nact, hipMemcpyHostToDevice, h->s));
+    HIPC(h, hipMemcpyAsync(dtab, tab.data(), sizeof(AdjBody) * nb, hipMemcpyHostToDevice, h->s));
+    HIPC(h, hipMemsetAsync(pf, 0, sizeof(double) * 2 * (size_t)h->npn * K, h->s));  // inactive patch nodes
+    HIPC(h, hipMemsetAsync(pl, 0, sizeof(double) * 2 * (size_t)h->npn, h->s));
+    HIPC(h, hipMemsetAsync(gc, 0, sizeof(double) * 2 * nb, h->s));
+    HIPC(h, hipMemsetAsync(lam, 0, sizeof(A) * SZ, h->s));  // adjoint after the last step
+    HIPC(h, hipMemsetAsync(gacc, 0, sizeof(A) * N, h->s));
+    HIPC(h, hipMemsetAsync(gtot, 0, sizeof(double) * N, h->s));
+
+    // forward pass, checkpointing the stored state every K steps
+    const int phase0 = h->phase;
+    const int64_t rec0 = h->nrec;
+    for (int sg = 0; sg < nseg; sg++) {
+        HIPC(h, hipMemcpyAsync(ck + SZ * sg, h->buf[h->cur], sizeof(T) * SZ, hipMemcpyDeviceToDevice, h->s));
+        r = run_steps<T>(h, std::min(K, nsteps - sg * K));
+        if (r) return r;
+    }
+    r = check_flag(h);
+    if (r) return r;
+    // objective from the force records of the window
+    {
+        std::vector<double> rec(2 * (size_t)nb);
+        double acc = 0.0;
+        for (int t = avg_from - 1; t < nsteps; t++) {
+            const int64_t slot = (rec0 + t) % h->hist_cap;
+            HIPC(h, hipMemcpy(rec.data(), h->hist + 2 * (size_t)nb * slot, sizeof(double) * 2 * nb,
+                              hipMemcpyDeviceToHost));
+            for (int b = 0; b < nb; b++) acc += tab[b].wx * rec[2 * b] + tab[b].wy * rec[2 * b + 1];
+        }
+        if (J) *J = acc / nwin;
+    }
+
+    // reverse sweep
+    StepArgs sa = make_args(h);
+    AdjArgs aa{};
+    aa.nx = nx;
+    aa.ny = ny;
+    aa.tau0 = h->c.tau0;
+    aa.tau_field = h->c.tau_mode == IPPI_TAU_FIELD;
+    aa.tau_ptr = h->tau;
+    aa.patch_l = pl;
+    aa.nbody = nb;
+    for (int b = 0; b < nb; b++) aa.patch[b] = h->bodies[b].p;
+    AdjIb ad{h->nact, h->ell, h->d_act, dan, lf, uact, h->d_pidx, h->d_w, dwx, dwy, nullptr,
+             h->d_ecnt, h->d_emk, h->d_ecw, dtab, pl, gc};
+    const dim3 grid_pull((nx + BX - 1) / BX, (ny + IPPI_NPT - 1) / IPPI_NPT), grid_first((nx + BX - 1) / BX, ny);
+    const dim3 grid_adj((nx + BX - 1) / BX, ny), grid_act((h->nact + 255) / 256);
+    const StepKernel<T> kpull = pick_step<T, true, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB);
+    const StepKernel<T> kfirst = pick_step<T, false, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB);
+    int cl = 0;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int t0 = sg * K, Ks = std::min(K, nsteps - t0);
+        auto state = [&](int i) { return i == 0 ? ck + SZ * sg : seg + SZ * (i - 1); };
+        // recompute the segment: states, patch forces and marker forces
+        for (int i = 0; i < Ks; i++) {
+            const bool pull = !(t0 + i == 0 && phase0 == 0);
+            IbDev d{h->d_w, h->d_pidx, h->d_ud, ui, Fs + 2 * (size_t)nm * i, h->d_blk, h->d_act, h->d_ecnt,
+                    h->d_emk, h->d_ecw, pf + 2 * (size_t)h->npn * i, nullptr};
+            launch_ib_to<T>(h, state(i), pull, d);
+            if (i + 1 < Ks) {
+                sa.patch_f = pf + 2 * (size_t)h->npn * i;
+                hipLaunchKernelGGL(pull ? kpull : kfirst, pull ? grid_pull : grid_first, dim3(BX), 0, h->s,
+                                   (const T*)state(i), state(i + 1), h->mac, sa);
+            }
+        }
+        HIPC(h, hipGetLastError());
+        for (int i = Ks - 1; i >= 0; i--) {
+            const int t = t0 + i;
+            const bool pull = !(t == 0 && phase0 == 0);
+            const double cj = t + 1 >= avg_from ? 1.0 / nwin : 0.0;
+            const A* lin = lam + SZ * cl;
+            A* lout = lam + SZ * (cl ^ 1);
+            aa.patch_f = pf + 2 * (size_t)h->npn * i;
+            aa.store = t > 0;
+            ad.F = Fs + 2 * (size_t)nm * i;
+            if (pull)
+                hipLaunchKernelGGL((k_adj_act<T, A, true>), grid_act, dim3(256), 0, h->s, lin, state(i), h->mac, ad, aa);
+            else
+                hipLaunchKernelGGL((k_adj_act<T, A, false>), grid_act, dim3(256), 0, h->s, lin, state(i), h->mac, ad, aa);
+            hipLaunchKernelGGL(k_adj_mk, dim3(nb), dim3(IB_T), 0, h->s, ad, cj);
+            if (h->timing) h->ev_adj.push_back(new_event(h));
+            if (pull)
+                hipLaunchKernelGGL((k_adj_step<T, A, true>), grid_adj, dim3(BX), 0, h->s, lin, lout, state(i), h->mac,
+                                   gacc, aa);
+            else
+                hipLaunchKernelGGL((k_adj_step<T, A, false>), grid_adj, dim3(BX), 0, h->s, lin, lout, state(i), h->mac,
+                                   gacc, aa);
+            if (h->timing) h->ev_adj.push_back(new_event(h));
+            cl ^= 1;
+        }
+        hipLaunchKernelGGL(k_adj_flush<A>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->s, gacc, gtot, N);
+        HIPC(h, hipGetLastError());
+    }
+    if (g_tau) HIPC(h, hipMemcpyAsync(g_tau, gtot, sizeof(double) * N, hipMemcpyDeviceToHost, h->s));
+    if (g_c) HIPC(h, hipMemcpyAsync(g_c, gc, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, h->s));
+    HIPC(h, hipStreamSynchronize(h->s));
+    return IPPI_OK;
+}
+
 }  // namespace
 
 // ================================================================== C ABI
@@ -1418,6 +1911,7 @@ int ippi_destroy(ippi_handle* h) {
     if (h->s) (void)hipStreamSynchronize(h->s);
     clear_events(h->ev_step);
     clear_events(h->ev_ib);
+    clear_events(h->ev_adj);
     dfree(h->buf[0]); dfree(h->buf[1]); dfree(h->mac); dfree(h->tau); dfree(h->flag); dfree(h->part);
     free_ib(h);
     dfree(h->d_frag);
@@ -1710,6 +2204,7 @@ int ippi_set_timing(ippi_handle* h, int enable) {
     HIPC(h, hipStreamSynchronize(h->s));
     clear_events(h->ev_step);
     clear_events(h->ev_ib);
+    clear_events(h->ev_adj);
     h->timing = enable ? 1 : 0;
     return IPPI_OK;
 }
@@ -1717,24 +2212,30 @@ int ippi_set_timing(ippi_handle* h, int enable) {
 int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, double* ib_ms, int64_t* ib_launches) {
     if (!h) return IPPI_ERR_ARG;
     HIPC(h, hipStreamSynchronize(h->s));
-    auto sum = [&](std::vector<hipEvent_t>& v, double* ms, int64_t* cnt) -> int {
-        double t = 0.0;
-        int64_t c = 0;
-        for (size_t i = 0; i + 1 < v.size(); i += 2) {
-            if (!v[i] || !v[i + 1]) return fail(h, IPPI_ERR_HIP, "event creation failed");
-            float e = 0.f;
-            HIPC(h, hipEventElapsedTime(&e, v[i], v[i + 1]));
-            t += e;
-            c++;
-        }
-        if (ms) *ms = t;
-        if (cnt) *cnt = c;
-        clear_events(v);
-        return IPPI_OK;
-    };
-    int r = sum(h->ev_step, step_ms, step_launches);
+    int r = sum_events(h, h->ev_step, step_ms, step_launches);
     if (r) return r;
-    return sum(h->ev_ib, ib_ms, ib_launches);
+    return sum_events(h, h->ev_ib, ib_ms, ib_launches);
+}
+
+int ippi_get_adjoint_timing(ippi_handle* h, double* ms, int64_t* launches) {
+    if (!h) return IPPI_ERR_ARG;
+    HIPC(h, hipStreamSynchronize(h->s));
+    return sum_events(h, h->ev_adj, ms, launches);
+}
+
+int ippi_adjoint_drag(ippi_handle* h, int nsteps, int avg_from, int ckpt, int flags, const double* body_w,
+                      double* J, double* g_tau, double* g_center) {
+    if (!h || nsteps < 1 || avg_from < 1 || avg_from > nsteps || ckpt < 1 || (flags & ~IPPI_ADJ_FP64))
+        return IPPI_ERR_ARG;
+    if (h->c.collision != IPPI_COLL_BGK || h->c.bc != IPPI_BC_PERIODIC_BB || h->c.les ||
+        h->c.tau_mode == IPPI_TAU_NN)
+        return fail(h, IPPI_ERR_ARG, "the adjoint covers BGK, periodic x / bounce-back walls, no LES, "
+                                     "constant or per-node tau");
+    if (h->bodies.empty()) return fail(h, IPPI_ERR_STATE, "the drag objective needs an immersed body");
+    if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
+    if (h->esz == 8) return adjoint_run<double, double>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
+    if (flags & IPPI_ADJ_FP64) return adjoint_run<float, double>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
+    return adjoint_run<float, float>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
 }
 
 int ippi_synchronize(ippi_handle* h) {

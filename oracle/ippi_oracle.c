@@ -710,3 +710,205 @@ void or_get_force_field(const or_sim* s, double* fx, double* fy) {
     memcpy(fx, s->Fx, sizeof(double) * s->N);
     memcpy(fy, s->Fy, sizeof(double) * s->N);
 }
+
+/* ------------------------------------------------------------------ */
+/* Discrete adjoint of the BGK + IB step (periodic x, bounce-back walls),  */
+/* objective J = mean over steps [avg_from, nsteps] of the body's drag     */
+/* record (minus the sum of the marker forces, x component).               */
+/* Gradients: dJ/dtau (per node base relaxation time) and dJ/d(cx, cy).    */
+/* Forward model = or_step with tau_mode FIELD, les off, one body.         */
+/* Every stage is transposed by hand; states are kept for all steps.       */
+/* ------------------------------------------------------------------ */
+static double dphi(double x) { /* derivative of or_delta */
+    double ax = fabs(x), sg = x < 0 ? -1.0 : 1.0;
+    if (ax < 1) {
+        double R = 1.0 + 4.0 * ax - 4 * (x * x);
+        return sg * (1.0 / 8.0) * (-2.0 + (4.0 - 8.0 * ax) / (2.0 * sqrt(R)));
+    } else if (ax < 2) {
+        double R = -7.0 + 12.0 * ax - 4 * (x * x);
+        return sg * (1.0 / 8.0) * (-2.0 - (12.0 - 8.0 * ax) / (2.0 * sqrt(R)));
+    }
+    return 0.0;
+}
+
+int or_adjoint_drag(int nx, int ny, uint64_t seed, double ux0, double uy0, double noise, const double* tau,
+                    double cx, double cy, double diam, int M, int nsteps, int avg_from, double* J, double* g_tau,
+                    double* g_c) {
+    or_config c = {nx, ny, OR_COLL_BGK, OR_BC_PERIODIC_BB, 0, OR_TAU_FIELD, 0.6, 0.0, 0.0, 0.0, 0.0};
+    or_sim* s = or_create(&c);
+    or_init_fields(s, seed, 1.0, ux0, uy0, noise);
+    or_set_tau_field(s, tau);
+    or_add_body(s, cx, cy, diam, M, 0.0, 0.0);
+    const int N = s->N;
+    const size_t SZ = 9 * (size_t)N;
+    /* states fnew^t and the fields u^t, rho^t for t = 0..nsteps-1 */
+    double* F = (double*)malloc(sizeof(double) * SZ * nsteps);
+    double* R = (double*)malloc(sizeof(double) * 3 * (size_t)N * nsteps);
+    const int nwin = nsteps - avg_from + 1;
+    double Jv = 0.0;
+    for (int t = 0; t < nsteps; t++) {
+        memcpy(F + SZ * t, s->fnew, sizeof(double) * SZ);
+        memcpy(R + 3 * (size_t)N * t, s->rho, sizeof(double) * N);
+        memcpy(R + 3 * (size_t)N * t + N, s->ux, sizeof(double) * N);
+        memcpy(R + 3 * (size_t)N * t + 2 * N, s->uy, sizeof(double) * N);
+        or_step(s, 1);
+        if (t + 1 >= avg_from) Jv += s->forces[(size_t)t * 2] / nwin;
+    }
+    if (J) *J = Jv;
+    double* lam = (double*)calloc(SZ, sizeof(double));   /* adjoint of fnew^{t+1} */
+    double* lamn = (double*)calloc(SZ, sizeof(double));  /* adjoint of fnew^t */
+    double* lc = (double*)calloc(SZ, sizeof(double));    /* adjoint of fcol^t */
+    double* lFx = (double*)calloc(N, sizeof(double));    /* adjoint of Eulerian force */
+    double* lFy = (double*)calloc(N, sizeof(double));
+    double* luIx = (double*)calloc(N, sizeof(double));   /* velocity adjoint from interpolation */
+    double* luIy = (double*)calloc(N, sizeof(double));
+    double* Ex = (double*)calloc(N, sizeof(double));     /* Eulerian force of the step */
+    double* Ey = (double*)calloc(N, sizeof(double));
+    double* mUx = (double*)malloc(sizeof(double) * M);
+    double* mUy = (double*)malloc(sizeof(double) * M);
+    double* mFx = (double*)malloc(sizeof(double) * M);
+    double* mFy = (double*)malloc(sizeof(double) * M);
+    if (g_tau) memset(g_tau, 0, sizeof(double) * N);
+    double gcx = 0.0, gcy = 0.0;
+    or_marker* mk = s->mk;
+    for (int t = nsteps - 1; t >= 0; t--) {
+        const double* f = F + SZ * t;
+        const double* rho = R + 3 * (size_t)N * t;
+        const double* ux = rho + N;
+        const double* uy = rho + 2 * N;
+        const double cJ = (t + 1 >= avg_from) ? 1.0 / nwin : 0.0; /* weight of this step's drag */
+        /* recompute the IB forward of this step */
+        memset(Ex, 0, sizeof(double) * N);
+        memset(Ey, 0, sizeof(double) * N);
+        for (int m = 0; m < M; m++) {
+            double Ux = 0, Uy = 0;
+            int XI = (int)mk[m].x, YI = (int)mk[m].y;
+            for (int X = XI - 1; X <= XI + 2; X++)
+                for (int Y = YI - 1; Y <= YI + 2; Y++) {
+                    double D = or_delta(mk[m].x - X) * or_delta(mk[m].y - Y);
+                    Ux += ux[Y * nx + X] * D;
+                    Uy += uy[Y * nx + X] * D;
+                }
+            mUx[m] = Ux;
+            mUy[m] = Uy;
+            mFx[m] = 2 * (0.0 - Ux);
+            mFy[m] = 2 * (0.0 - Uy);
+            for (int X = XI - 1; X <= XI + 2; X++)
+                for (int Y = YI - 1; Y <= YI + 2; Y++) {
+                    double D = or_delta(mk[m].x - X) * or_delta(mk[m].y - Y);
+                    Ex[Y * nx + X] += mFx[m] * D;
+                    Ey[Y * nx + X] += mFy[m] * D;
+                }
+        }
+        /* streaming transpose: lc_j(m) = lam_j(m + e_j), or lam_opp(j)(m) when m + e_j leaves the walls */
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                int n = y * nx + x;
+                for (int j = 0; j < 9; j++) {
+                    int yd = y + EY[j], xd = (x + EX[j] + nx) % nx;
+                    lc[(size_t)j * N + n] = (yd < 0 || yd >= ny) ? lam[(size_t)OPP[j] * N + n]
+                                                                 : lam[(size_t)j * N + yd * nx + xd];
+                }
+            }
+        /* collision transpose, first for the force adjoint (needed by the markers) */
+        for (int n = 0; n < N; n++) {
+            double r = rho[n], tb = tau[n], om = 1.0 / tb, ct = 1.0 - 0.5 * om;
+            double us_x = ux[n] + 0.5 * Ex[n] / r, us_y = uy[n] + 0.5 * Ey[n] / r;
+            double lux = 0, luy = 0, lfx = 0, lfy = 0;
+            for (int k = 0; k < 9; k++) {
+                double l = lc[(size_t)k * N + n];
+                double eu = EX[k] * us_x + EY[k] * us_y;
+                lux += l * (om * WEIGHT[k] * r * (3.0 * EX[k] + 9.0 * eu * EX[k] - 3.0 * us_x) +
+                            WEIGHT[k] * ct * (-3.0 * Ex[n] + 9.0 * EX[k] * (EX[k] * Ex[n] + EY[k] * Ey[n])));
+                luy += l * (om * WEIGHT[k] * r * (3.0 * EY[k] + 9.0 * eu * EY[k] - 3.0 * us_y) +
+                            WEIGHT[k] * ct * (-3.0 * Ey[n] + 9.0 * EY[k] * (EX[k] * Ex[n] + EY[k] * Ey[n])));
+                lfx += l * WEIGHT[k] * ct * (3.0 * (EX[k] - us_x) + 9.0 * eu * EX[k]);
+                lfy += l * WEIGHT[k] * ct * (3.0 * (EY[k] - us_y) + 9.0 * eu * EY[k]);
+            }
+            lFx[n] = lfx + lux / (2.0 * r);
+            lFy[n] = lfy + luy / (2.0 * r);
+        }
+        /* markers: adjoint of F_m (spread + drag), of U_m, position gradient */
+        memset(luIx, 0, sizeof(double) * N);
+        memset(luIy, 0, sizeof(double) * N);
+        for (int m = 0; m < M; m++) {
+            int XI = (int)mk[m].x, YI = (int)mk[m].y;
+            double lFmx = -cJ, lFmy = 0.0; /* d(drag)/dF_m = -1 on x */
+            for (int X = XI - 1; X <= XI + 2; X++)
+                for (int Y = YI - 1; Y <= YI + 2; Y++) {
+                    double D = or_delta(mk[m].x - X) * or_delta(mk[m].y - Y);
+                    lFmx += D * lFx[Y * nx + X];
+                    lFmy += D * lFy[Y * nx + X];
+                }
+            double lUx = -2.0 * lFmx, lUy = -2.0 * lFmy;
+            for (int X = XI - 1; X <= XI + 2; X++)
+                for (int Y = YI - 1; Y <= YI + 2; Y++) {
+                    int n = Y * nx + X;
+                    double px = or_delta(mk[m].x - X), py = or_delta(mk[m].y - Y);
+                    double D = px * py, Dx = dphi(mk[m].x - X) * py, Dy = px * dphi(mk[m].y - Y);
+                    luIx[n] += D * lUx;
+                    luIy[n] += D * lUy;
+                    /* d/dx_m of U_m (interpolation) and of F(n) (spreading) */
+                    gcx += lUx * Dx * ux[n] + lUy * Dx * uy[n] + Dx * (lFx[n] * mFx[m] + lFy[n] * mFy[m]);
+                    gcy += lUx * Dy * ux[n] + lUy * Dy * uy[n] + Dy * (lFx[n] * mFx[m] + lFy[n] * mFy[m]);
+                }
+        }
+        /* collision and macro transpose -> adjoint of fnew^t; tau gradient */
+        for (int n = 0; n < N; n++) {
+            double r = rho[n], tb = tau[n], om = 1.0 / tb, ct = 1.0 - 0.5 * om;
+            double us_x = ux[n] + 0.5 * Ex[n] / r, us_y = uy[n] + 0.5 * Ey[n] / r;
+            double feq[9], fc[9];
+            or_feq(r, us_x, us_y, feq);
+            for (int k = 0; k < 9; k++) fc[k] = f[(size_t)k * N + n];
+            double lux = 0, luy = 0, lr = 0, lt = 0;
+            for (int k = 0; k < 9; k++) {
+                double l = lc[(size_t)k * N + n];
+                double eu = EX[k] * us_x + EY[k] * us_y;
+                double S = WEIGHT[k] * ct * (3.0 * ((EX[k] - us_x) * Ex[n] + (EY[k] - us_y) * Ey[n]) +
+                                             9.0 * eu * (EX[k] * Ex[n] + EY[k] * Ey[n]));
+                lux += l * (om * WEIGHT[k] * r * (3.0 * EX[k] + 9.0 * eu * EX[k] - 3.0 * us_x) +
+                            WEIGHT[k] * ct * (-3.0 * Ex[n] + 9.0 * EX[k] * (EX[k] * Ex[n] + EY[k] * Ey[n])));
+                luy += l * (om * WEIGHT[k] * r * (3.0 * EY[k] + 9.0 * eu * EY[k] - 3.0 * us_y) +
+                            WEIGHT[k] * ct * (-3.0 * Ey[n] + 9.0 * EY[k] * (EX[k] * Ex[n] + EY[k] * Ey[n])));
+                lr += l * om * feq[k] / r;
+                lt += l * ((fc[k] - feq[k]) * om * om + S * 0.5 * om * om / ct);
+            }
+            /* u* = u + F/(2 rho) */
+            lr += -(lux * Ex[n] + luy * Ey[n]) / (2.0 * r * r);
+            lux += luIx[n];
+            luy += luIy[n];
+            for (int k = 0; k < 9; k++)
+                lamn[(size_t)k * N + n] = (1.0 - om) * lc[(size_t)k * N + n] + lr +
+                                          (lux * (EX[k] - ux[n]) + luy * (EY[k] - uy[n])) / r;
+            if (g_tau) g_tau[n] += lt;
+        }
+        double* tmp = lam;
+        lam = lamn;
+        lamn = tmp;
+    }
+    if (g_c) {
+        g_c[0] = gcx;
+        g_c[1] = gcy;
+    }
+    free(F); free(R); free(lam); free(lamn); free(lc); free(lFx); free(lFy); free(luIx); free(luIy);
+    free(Ex); free(Ey); free(mUx); free(mUy); free(mFx); free(mFy);
+    or_destroy(s);
+    return 0;
+}
+
+double or_drag_objective(int nx, int ny, uint64_t seed, double ux0, double uy0, double noise, const double* tau,
+                         double cx, double cy, double diam, int M, int nsteps, int avg_from) {
+    double J = 0.0;
+    or_config c = {nx, ny, OR_COLL_BGK, OR_BC_PERIODIC_BB, 0, OR_TAU_FIELD, 0.6, 0.0, 0.0, 0.0, 0.0};
+    or_sim* s = or_create(&c);
+    or_init_fields(s, seed, 1.0, ux0, uy0, noise);
+    or_set_tau_field(s, tau);
+    or_add_body(s, cx, cy, diam, M, 0.0, 0.0);
+    const int nwin = nsteps - avg_from + 1;
+    for (int t = 0; t < nsteps; t++) {
+        or_step(s, 1);
+        if (t + 1 >= avg_from) J += s->forces[(size_t)t * 2] / nwin;
+    }
+    or_destroy(s);
+    return J;
+}

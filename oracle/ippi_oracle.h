@@ -84,6 +84,16 @@ void    or_get_markers(const or_sim* s, double* out); /* [m][x,y,ux_interp,uy_in
 double  or_mass(const or_sim* s);
 void    or_get_force_field(const or_sim* s, double* fx, double* fy);
 
+/* ---- discrete adjoint of the BGK + IB step (FP64, all states kept) ---- */
+/* J = mean of the body's drag record over steps avg_from..nsteps (1-based);
+ * gradients w.r.t. the per-node base tau (g_tau, N values) and the body
+ * centre (g_c, 2 values). Periodic x, bounce-back walls, one body. */
+int    or_adjoint_drag(int nx, int ny, uint64_t seed, double ux0, double uy0, double noise, const double* tau,
+                       double cx, double cy, double diam, int M, int nsteps, int avg_from, double* J,
+                       double* g_tau, double* g_c);
+double or_drag_objective(int nx, int ny, uint64_t seed, double ux0, double uy0, double noise, const double* tau,
+                         double cx, double cy, double diam, int M, int nsteps, int avg_from);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,10 @@ def lib():
         L.or_mass.argtypes = [vp]
         L.or_mass.restype = d
         L.or_get_force_field.argtypes = [vp, _D, _D]
+        L.or_adjoint_drag.argtypes = [i, i, ctypes.c_uint64, d, d, d, _D, d, d, d, i, i, i, _D, _D, _D]
+        L.or_adjoint_drag.restype = i
+        L.or_drag_objective.argtypes = [i, i, ctypes.c_uint64, d, d, d, _D, d, d, d, i, i, i]
+        L.or_drag_objective.restype = d
         _lib = L
     return _lib
 
@@ -256,3 +260,20 @@ class Sim:
 
     def mass(self):
         return lib().or_mass(self.h)
+
+
+def drag_objective(nx, ny, seed, u0, noise, tau, center, diam, M, nsteps, avg_from):
+    t = np.ascontiguousarray(tau, dtype=np.float64).ravel()
+    return lib().or_drag_objective(nx, ny, seed, u0[0], u0[1], noise, _p(t), center[0], center[1], diam, M,
+                                   nsteps, avg_from)
+
+
+def adjoint_drag(nx, ny, seed, u0, noise, tau, center, diam, M, nsteps, avg_from):
+    """(J, dJ/dtau [ny, nx], dJ/d(cx, cy)) of the FP64 BGK + IB step."""
+    t = np.ascontiguousarray(tau, dtype=np.float64).ravel()
+    J = np.zeros(1)
+    gt = np.zeros(nx * ny)
+    gc = np.zeros(2)
+    lib().or_adjoint_drag(nx, ny, seed, u0[0], u0[1], noise, _p(t), center[0], center[1], diam, M, nsteps,
+                          avg_from, _p(J), _p(gt), _p(gc))
+    return J[0], gt.reshape(ny, nx), gc
