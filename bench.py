@@ -1,33 +1,48 @@
 """Benchmark of the MI355X D2Q9 time step of IPPI-LBM (one JSON line).
 
-Workload (the headline configuration): MRT collision + hybrid Smagorinsky
-LES (Cs = 0.1), 2048 x 1024 nodes, one immersed-boundary cylinder of
-diameter 80 lu (251 markers) at (512, 512), Re = 1e4 (tau0 = 0.5 + 3 U D/Re
-= 0.50192), inflow U = 0.08 with seeded +-1e-3 velocity noise, non-
+Headline workload (`value`): MRT collision + hybrid Smagorinsky LES
+(Cs = 0.1), 2048 x 1024 nodes, one immersed-boundary cylinder of diameter
+80 lu (251 markers) at (512, 512), Re = 1e4 (tau0 = 0.5 + 3 U D / Re =
+0.50192), inflow U = 0.08 with seeded +-1e-3 velocity noise, non-
 equilibrium-extrapolation inlet/outlet and free-slip walls; populations
-stored in FP32 as f - w. A "step" is one full time step (IB marker kernel,
-IB spread kernel, fused stream-collide kernel).
+stored in FP32 as f - w. A "step" is one full time step (IB kernel + fused
+stream-collide kernel), 20,000 steps after 50 warm-up steps.
 
-Reported:
-  value         MLUPS of the FP32 path = nx*ny*steps / seconds / 1e6, with
-                the state resident in HBM (stream synchronised on both
-                sides of the timed region).
-  fp64_mlups    the same workload on the plain FP64 path, and the ratio.
+Reported beside it:
+  fp64_mlups    the same workload on the plain FP64 path, and the ratio;
   accuracy      relative L2 error of velocity and density of the FP32 path
-                against the FP64 GPU path after the timed steps (both start
-                from the same seeded state), and of the drag record.
+                against the FP64 GPU path after the same steps (same seeded
+                start), and of the drag record;
   roofline      achieved HBM bandwidth of the fused stream-collide kernel:
                 72 algorithmic bytes per node (9 FP32 loads + 9 FP32
                 stores) x nodes / average launch time, from HIP events on
-                the library's stream; `traffic` from rocprof counters when
-                available (IPPI_TRAFFIC_BYTES), else null.
+                the library's stream; `traffic` = HBM bytes per launch from
+                the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary,
+                scaled by the same counters on a plain copy of known size;
   cpu_baseline  the FP64 CPU restatement (oracle/, OpenMP) on a bounded
-                sample of the same workload on this host's cores.
+                sample of the same workload on this host's cores;
+  legs          the other workloads, each measured the same way:
+                nn_4096x2048   MRT + LES + IB (D = 160) with the fused
+                               network tau (seeded Xavier weights, guard
+                               active), FP32 vs FP64, 10,000 steps;
+                adjoint_1024x512  BGK + IB discrete adjoint of the mean drag
+                               over steps 2001-4000, checkpoints every 100,
+                               tau field 0.6 +- 0.02: wall time of forward +
+                               reverse sweep, bandwidth of the all-node
+                               adjoint kernel, for FP32 and FP32 forward /
+                               FP64 adjoint;
+                multibody_8192x4096  MRT + LES + network tau, 16 cylinders
+                               D ~ U[40, 120] (seeded placement), FP32 vs
+                               FP64, 5,000 steps, per-body drag error.
+
+`python bench.py --copy-probe MB` runs only a plain device copy of MB
+megabytes (the calibration program for the rocprofv3 traffic counters).
 """
 import argparse
 import importlib.util
 import json
 import os
+import re
 import sys
 import time
 
@@ -37,6 +52,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "machine-learning-based-implicit-physical-parametric-inference-for-cfd-simulation-promotion_amd")
 METRIC = "MLUPS + % HBM roofline, FP32/NN-tau vs plain FP64 at ≤1e-5 rel L2 velocity"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_step_f32_mrt_les_ib_2048x1024.md")
 
 NX, NY = 2048, 1024
 U0 = 0.08
@@ -59,11 +75,8 @@ def load_pkg():
     return m
 
 
-def make_solver(ippi, precision):
-    s = ippi.Solver(NX, NY, "mrt", precision, "inflow", les=True, tau0=TAU0, cs=CS, u_in=(U0, 0.0))
-    s.init_fields(SEED, 1.0, (U0, 0.0), 1e-3)
-    s.add_body(CX, CY, DIAM, NMARK)
-    return s
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def timed_run(s, steps, warmup):
@@ -75,8 +88,40 @@ def timed_run(s, steps, warmup):
     return time.perf_counter() - t0
 
 
+def vel_err(a, b):
+    return float(np.sqrt(np.sum((a[1] - b[1]) ** 2 + (a[2] - b[2]) ** 2) / np.sum(b[1] ** 2 + b[2] ** 2)))
+
+
+def kernel_avg_us(s, steps):
+    """Average fused-kernel and IB-kernel launch times over `steps` event-timed steps."""
+    s.set_timing(True)
+    s.step(steps)
+    step_ms, nstep, ib_ms, nib = s.timing()
+    s.set_timing(False)
+    return step_ms / max(nstep, 1) * 1e3, ib_ms / max(nib, 1) * 1e3
+
+
+# ------------------------------------------------------------------ headline
+def make_headline(ippi, precision):
+    s = ippi.Solver(NX, NY, "mrt", precision, "inflow", les=True, tau0=TAU0, cs=CS, u_in=(U0, 0.0))
+    s.init_fields(SEED, 1.0, (U0, 0.0), 1e-3)
+    s.add_body(CX, CY, DIAM, NMARK)
+    return s
+
+
+def read_traffic():
+    """HBM bytes per launch of the headline fused kernel from the committed
+    rocprofv3 counter summary (None when it is absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def cpu_baseline(seconds_target=12.0):
-    """FP64 oracle (OpenMP) on the same workload, bounded number of steps."""
+    """FP64 oracle (OpenMP) on the headline workload, bounded number of steps."""
     sys.path.insert(0, ROOT)
     from oracle import pyoracle
     threads = min(16, os.cpu_count() or 1)
@@ -95,64 +140,193 @@ def cpu_baseline(seconds_target=12.0):
             "sample": f"oracle/ FP64 MRT+LES+IB, {NX}x{NY}, {n} steps after 1 warm-up step, {threads} OpenMP threads"}
 
 
+# ---------------------------------------------------------------- other legs
+def leg_nn(ippi, steps, warmup, profile_steps):
+    from ippi_amd import workloads
+    nx, ny, D = 4096, 2048, 160.0
+    U, tau0 = 0.08, 0.5 + 3 * 0.08 * 160.0 / 1e4
+    blob = workloads.xavier_blob(7)
+    out = {"workload": f"MRT+LES(Cs=0.1)+IB cylinder D=160 ({workloads.markers_for(D)} markers) + fused network "
+                       "tau (seeded Xavier 6-32-32-1, guard >= 0.501), 4096x2048, inflow 0.08 +-1e-3, Re=1e4",
+           "steps": steps}
+    res = {}
+    for prec in ("fp32", "fp64"):
+        s = ippi.Solver(nx, ny, "mrt", prec, "inflow", les=True, tau_mode="nn", tau0=tau0, cs=CS, u_in=(U, 0.0))
+        s.init_fields(SEED, 1.0, (U, 0.0), 1e-3)
+        s.add_body(nx / 4, ny / 2, D, workloads.markers_for(D))
+        s.set_mlp(blob)
+        log(f"nn 4096x2048 {prec}: {steps} steps")
+        dt = timed_run(s, steps, warmup)
+        res[prec] = (nx * ny * steps / dt / 1e6, s.fields(), s.tau())
+        if prec == "fp32":
+            k_us, ib_us = kernel_avg_us(s, profile_steps)
+            gbs = 72 * nx * ny / (k_us * 1e-6) / 1e9
+            out.update({"fused_kernel_avg_us": round(k_us, 2), "fused_kernel_gbs": round(gbs, 1),
+                        "fused_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "ib_kernel_avg_us": round(ib_us, 2)})
+        s.close()
+    t32, t64 = res["fp32"][2], res["fp64"][2]
+    out.update({"mlups_fp32": round(res["fp32"][0], 1), "mlups_fp64": round(res["fp64"][0], 1),
+                "speedup_vs_fp64": round(res["fp32"][0] / res["fp64"][0], 3),
+                "rel_l2_u_vs_fp64": vel_err(res["fp32"][1], res["fp64"][1]),
+                "network_tau_accepted_frac": float(np.mean(t64 > 0.501)),
+                "rel_l2_tau_vs_fp64": float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))})
+    return out
+
+
+def leg_adjoint(ippi):
+    from ippi_amd import adjoint
+    nx, ny = 1024, 512
+    nu = 0.1 / 3
+    tau = 0.6 + np.random.default_rng(4).uniform(-0.02, 0.02, (ny, nx))
+    c = (256.47, 256.27)
+    out = {"workload": "BGK+IB adjoint, 1024x512, cylinder D=40 (126 markers), tau 0.6+-0.02 per node, Re=200, "
+                       "4000 steps, checkpoints every 100, J = mean drag over steps 2001-4000"}
+    for name, prec, f64 in (("fp32", "fp32", False), ("fp32_fwd_fp64_adj", "fp32", True), ("fp64", "fp64", False)):
+        case = adjoint.DragCase(nx=nx, ny=ny, diameter=40.0, markers=126, nsteps=4000, avg_from=2001, ckpt=100,
+                                precision=prec, fp64_adjoint=f64, seed=20240607, u0=(200 * nu / 40.0, 0.0))
+        log(f"adjoint {name}")
+        with case.solver(tau, c) as w:  # warm-up: first launches of every kernel
+            w.adjoint_drag(20, 1, 10, fp64_adjoint=f64)
+        with case.solver(tau, c) as s:
+            s.set_timing(True)
+            t0 = time.perf_counter()
+            J, gt, gc = s.adjoint_drag(case.nsteps, case.avg_from, case.ckpt, fp64_adjoint=f64)
+            wall = time.perf_counter() - t0
+            ms, n = s.adjoint_timing()
+            s.set_timing(False)
+        per = 36 if prec == "fp32" else 72  # forward state read per node
+        lam = 72 if (prec == "fp64" or f64) else 36
+        tau_b = 4 if prec == "fp32" else 8
+        acc = 8 if (prec == "fp64" or f64) else 4
+        bytes_node = per + 2 * lam + tau_b + 2 * acc
+        us = ms / max(n, 1) * 1e3
+        gbs = bytes_node * nx * ny / (us * 1e-6) / 1e9
+        out[name] = {"J": J, "dJ_dcenter": [float(gc[0][0]), float(gc[0][1])], "wall_s": round(wall, 3),
+                     "adjoint_kernel_avg_us": round(us, 2), "adjoint_kernel_bytes_per_node": bytes_node,
+                     "adjoint_kernel_gbs": round(gbs, 1), "adjoint_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "_gt": gt}
+    g64 = out["fp64"].pop("_gt")
+    for k in ("fp32", "fp32_fwd_fp64_adj"):
+        gt = out[k].pop("_gt")
+        out[k]["tau_grad_rel_l2_vs_fp64"] = float(np.linalg.norm(gt - g64) / np.linalg.norm(g64))
+        out[k]["center_grad_rel_err_vs_fp64"] = [abs(a - b) / abs(b) for a, b in
+                                                 zip(out[k]["dJ_dcenter"], out["fp64"]["dJ_dcenter"])]
+    return out
+
+
+def leg_multibody(ippi, steps, warmup):
+    from ippi_amd import workloads
+    nx, ny, U = 8192, 4096, 0.06
+    bodies = workloads.place_cylinders(nx, ny, 16, 40.0, 120.0, seed=11)
+    nm = sum(b[3] for b in bodies)
+    dmean = float(np.mean([b[2] for b in bodies]))
+    tau0 = 0.5 + 3 * U * dmean / 1e4
+    blob = workloads.xavier_blob(7)
+    out = {"workload": f"MRT+LES(Cs=0.1)+network tau, 8192x4096, 16 cylinders D~U[40,120] ({nm} markers), "
+                       f"inflow 0.06 +-1e-3, tau0={tau0:.5f}", "steps": steps, "markers": nm}
+    res = {}
+    for prec in ("fp32", "fp64"):
+        s = ippi.Solver(nx, ny, "mrt", prec, "inflow", les=True, tau_mode="nn", tau0=tau0, cs=CS, u_in=(U, 0.0),
+                        force_history=steps + warmup)
+        s.init_fields(SEED, 1.0, (U, 0.0), 1e-3)
+        for (cx, cy, d, m) in bodies:
+            s.add_body(cx, cy, d, m)
+        s.set_mlp(blob)
+        log(f"multibody 8192x4096 {prec}: {steps} steps")
+        dt = timed_run(s, steps, warmup)
+        res[prec] = (nx * ny * steps / dt / 1e6, s.fields(), s.body_forces())
+        s.close()
+    f32, f64 = res["fp32"][2], res["fp64"][2]
+    n = min(len(f32), len(f64)) // 2
+    d32, d64 = f32[-n:, :, 0].mean(0), f64[-n:, :, 0].mean(0)
+    err = np.abs(d32 - d64) / np.abs(d64)
+    out.update({"mlups_fp32": round(res["fp32"][0], 1), "mlups_fp64": round(res["fp64"][0], 1),
+                "speedup_vs_fp64": round(res["fp32"][0] / res["fp64"][0], 3),
+                "rel_l2_u_vs_fp64": vel_err(res["fp32"][1], res["fp64"][1]),
+                "drag_rel_err_vs_fp64_per_body": [float(e) for e in err], "drag_rel_err_max": float(err.max()),
+                "drag_window": f"mean over the last {n} records"})
+    return out
+
+
+def copy_probe(mb):
+    import torch
+    n = int(mb * (1 << 20)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 10}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison of the headline")
+    ap.add_argument("--legs", default="nn,adjoint,multibody",
+                    help="comma list of extra workloads (nn, adjoint, multibody) or 'none'")
+    ap.add_argument("--nn-steps", type=int, default=10000)
+    ap.add_argument("--mb-steps", type=int, default=5000)
     ap.add_argument("--profile-steps", type=int, default=500,
                     help="steps of the event-timed pass used for the roofline figure")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
+                    help="only run a plain device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the D2Q9 step runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU x-slab design is described in DESIGN.md but not built).", file=sys.stderr)
+              "(a multi-GPU x-slab split is a design, not built).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     ippi = load_pkg()
+    ippi.library()
     K, W = args.steps, args.warmup
 
-    # FP32 shifted populations (the measured path)
-    s32 = make_solver(ippi, "fp32")
-    print(f"[bench] fp32 MRT+LES+IB {NX}x{NY}: warmup {W}, timed {K} steps", file=sys.stderr, flush=True)
+    s32 = make_headline(ippi, "fp32")
+    log(f"fp32 MRT+LES+IB {NX}x{NY}: warmup {W}, timed {K} steps")
     dt32 = timed_run(s32, K, W)
     mlups32 = NX * NY * K / dt32 / 1e6
+    extra = {}
+    if not args.no_fp64:
+        s64 = make_headline(ippi, "fp64")
+        log(f"fp64 MRT+LES+IB {NX}x{NY}: warmup {W}, timed {K} steps")
+        dt64 = timed_run(s64, K, W)
+        mlups64 = NX * NY * K / dt64 / 1e6
+        r32, r64 = s32.fields(), s64.fields()
+        d32, d64 = s32.body_forces(), s64.body_forces()
+        n = min(len(d32), len(d64))
+        extra = {"fp64_mlups": round(mlups64, 1), "speedup_vs_fp64": round(mlups32 / mlups64, 3),
+                 "rel_l2_u_vs_fp64": vel_err(r32, r64),
+                 "rel_l2_rho_vs_fp64": float(np.linalg.norm(r32[0] - r64[0]) / np.linalg.norm(r64[0])),
+                 "drag_rel_err_vs_fp64": float(np.abs(d32[-n:, 0, 0] - d64[-n:, 0, 0]).mean() /
+                                               np.abs(d64[-n:, 0, 0]).mean())}
+        s64.close()
 
-    # plain FP64 path, same workload
-    s64 = make_solver(ippi, "fp64")
-    print(f"[bench] fp64 MRT+LES+IB {NX}x{NY}: warmup {W}, timed {K} steps", file=sys.stderr, flush=True)
-    dt64 = timed_run(s64, K, W)
-    mlups64 = NX * NY * K / dt64 / 1e6
-
-    # accuracy of FP32 against FP64 after the same W + K steps
-    r32, x32, y32 = s32.fields()
-    r64, x64, y64 = s64.fields()
-    eu = float(np.sqrt(np.sum((x32 - x64) ** 2 + (y32 - y64) ** 2) / np.sum(x64 ** 2 + y64 ** 2)))
-    er = float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))
-    d32, d64 = s32.body_forces(), s64.body_forces()
-    n = min(len(d32), len(d64))
-    drag_err = float(np.abs(d32[-n:, 0, 0] - d64[-n:, 0, 0]).mean() / np.abs(d64[-n:, 0, 0]).mean())
-    s64.close()
-
-    # event-timed pass for the fused kernel's average launch time
-    P = args.profile_steps
-    s32.set_timing(True)
-    s32.step(P)
-    step_ms, nstep, ib_ms, nib = s32.timing()
-    s32.set_timing(False)
+    k_us, ib_us = kernel_avg_us(s32, args.profile_steps)
     s32.close()
-    avg_ms = step_ms / max(nstep, 1)
     algo_bytes = 72 * NX * NY
-    achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
-    traffic = os.environ.get("IPPI_TRAFFIC_BYTES")
+    achieved = algo_bytes / (k_us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": float(traffic) if traffic else None,
-                "kernel": "k_step_f32<MRT,INFLOW,pull>", "avg_launch_us": round(avg_ms * 1e3, 3),
-                "algorithmic_bytes_per_launch": algo_bytes,
-                "ib_kernels_avg_us": round(ib_ms / max(nib, 1) * 1e3, 3)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+                "kernel": "k_step_f32<MRT,INFLOW,pull>", "avg_launch_us": round(k_us, 3),
+                "algorithmic_bytes_per_launch": algo_bytes, "ib_kernel_avg_us": round(ib_us, 3)}
+
+    legs = {}
+    names = [x for x in args.legs.split(",") if x and x != "none"]
+    if "nn" in names:
+        legs["nn_4096x2048"] = leg_nn(ippi, args.nn_steps, W, min(args.profile_steps, 200))
+    if "adjoint" in names:
+        legs["adjoint_1024x512"] = leg_adjoint(ippi)
+    if "multibody" in names:
+        legs["multibody_8192x4096"] = leg_multibody(ippi, args.mb_steps, W)
 
     cpu = None
     if not args.no_cpu:
-        print("[bench] cpu baseline (oracle, OpenMP)", file=sys.stderr, flush=True)
+        log("cpu baseline (oracle, OpenMP)")
         cpu = cpu_baseline()
 
     line = {
@@ -163,9 +337,8 @@ def main():
                                "noise, Re=1e4, FP32 shifted populations (f-w)",
                    "nx": NX, "ny": NY, "collision": "mrt", "les_cs": CS, "tau0": TAU0, "u_in": U0,
                    "bodies": 1, "markers": NMARK, "bc": "inflow"},
-        "fp64_mlups": round(mlups64, 1), "speedup_vs_fp64": round(mlups32 / mlups64, 3),
-        "rel_l2_u_vs_fp64": eu, "rel_l2_rho_vs_fp64": er, "drag_rel_err_vs_fp64": drag_err,
-        "roofline": roofline, "cpu_baseline": cpu,
+        **extra,
+        "roofline": roofline, "cpu_baseline": cpu, "legs": legs,
     }
     print(json.dumps(line), flush=True)
 

@@ -523,8 +523,12 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     }
 }
 
+// At least 4 waves per SIMD: the network variants are VALU-bound and need
+// the occupancy to overlap transcendental latency and the population loads
+// (caps them at 128 VGPRs; the others use far fewer).
 template <int COLL, int BC, bool PULL, int NPT, bool NN>
-__global__ void __launch_bounds__(BX) k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
+__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(4)))
+k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
     const int nx = a.nx, ny = a.ny;
     // all lanes stay resident (the MLP's MFMAs are wave-wide); lanes past
@@ -2151,7 +2155,9 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
     for (int i = 0; i < n; i++)
         if (!std::isfinite(blob[i])) return fail(h, IPPI_ERR_ARG, "MLP blob holds a non-finite value");
     // per-lane MFMA fragments of the FP32 path (layouts described in mlp.hpp)
+    // constants folded as described at MlpFrag
     std::vector<MlpFrag> fr(64);
+    const double L = 1.4426950408889634, LN2 = 0.6931471805599453;
     auto hi = [](double v) { return (__bf16)(float)v; };
     auto lo = [](double v) { return (__bf16)((float)v - (float)(__bf16)(float)v); };
     for (int l = 0; l < 64; l++) {
@@ -2161,7 +2167,7 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
             const int o = mb * 16 + c;
             for (int j = 0; j < 4; j++) {
                 const int k = 4 * g + j;
-                const double w = k < MLP_NIN ? blob[OFF_W1 + o * MLP_NIN + k] : 0.0;
+                const double w = k < MLP_NIN ? -L * blob[OFF_W1 + o * MLP_NIN + k] : 0.0;
                 F.a1h[mb][j] = hi(w);
                 F.a1l[mb][j] = lo(w);
             }
@@ -2172,9 +2178,9 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
             }
             for (int r = 0; r < 4; r++) {
                 const int oo = mb * 16 + 4 * g + r;
-                F.b1[4 * mb + r] = (float)blob[OFF_B1 + oo];
-                F.b2[4 * mb + r] = (float)blob[OFF_B2 + oo];
-                F.w3[4 * mb + r] = (float)blob[OFF_W3 + oo];
+                F.b1[4 * mb + r] = (float)(-L * blob[OFF_B1 + oo]);
+                F.b2[4 * mb + r] = (float)(-L * blob[OFF_B2 + oo]);
+                F.w3[4 * mb + r] = (float)(-LN2 * blob[OFF_W3 + oo]);
             }
         }
     }

@@ -38,10 +38,16 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Per-lane fragments of the FP32/MFMA path (built on the host from the blob).
+// Constants are folded so that every hidden unit costs exp2 + add + rcp +
+// mul on VALU: with L = log2(e), layer 1 computes t1 = -L z1 (W1, b1 scaled
+// by -L) and carries h1' = t1 sigma = -L h1; layer 2 computes
+// t2 = W2 h1' + b2' = -L z2 (b2 scaled by -L, W2 unchanged); the output
+// weights are scaled by -ln(2) so that w3' t2 sigma(t2) = w3 h2, where
+// sigma = 1 / (1 + 2^t). The biases enter as the MFMA accumulator input.
 struct MlpFrag {
     bf16x4 a1h[2], a1l[2];  // layer-1 A tiles (two 16-output blocks), hi / lo
     bf16x8 a2h[2], a2l[2];  // layer-2 A tiles, columns in lane order
-    float b1[8], b2[8], w3[8];  // bias / output weight of this lane's 8 outputs
+    float b1[8], b2[8], w3[8];  // scaled bias / output weight of this lane's 8 outputs
 };
 struct MlpDev {
     const MlpFrag* frag;  // [64 lanes]
@@ -49,7 +55,10 @@ struct MlpDev {
     const double* blob;   // FP64 path
 };
 
-__device__ __forceinline__ float silu_f(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+// 1 / (1 + 2^t) = sigma(z) for t = -log2(e) z
+__device__ __forceinline__ float sig2(float t) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
+}
 __device__ __forceinline__ float softplus_f(float z) { return z > 15.0f ? z : __logf(1.0f + __expf(z)); }
 
 __device__ __forceinline__ void split(float v, __bf16& h, __bf16& l) {
@@ -58,30 +67,29 @@ __device__ __forceinline__ void split(float v, __bf16& h, __bf16& l) {
 }
 
 // tau of this lane's node from its six raw features; every lane of the wave
-// must call it (the MFMAs are wave-wide). lds: 64 x 8 bf16 x 2 (hi, lo) per wave.
+// must call it (the MFMAs are wave-wide). lds: 64 x 8 bf16 x 2 (hi, lo) per
+// wave. The 64 nodes are handled as four blocks of 16; each block runs
+// layer 1, the activation, layer 2 and the output sum before the next, so
+// only one block's tiles are live at a time.
 __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m, __bf16* lds) {
     const int l = threadIdx.x & 63;
     const int g = l >> 4;  // lane group
     const MlpFrag& fr = m.frag[l];
     // normalised features, split hi / lo, to LDS as [node][k] (k 0..7)
-    __bf16 xh[8], xl[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float v = k < 6 ? (feat[k] - m.mu[k]) * m.isig[k] : 0.0f;
-        split(v, xh[k], xl[k]);
-    }
     bf16x8 vh, vl;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        vh[k] = xh[k];
-        vl[k] = xl[k];
+        const float v = k < 6 ? (feat[k] - m.mu[k]) * m.isig[k] : 0.0f;
+        __bf16 h, lo;
+        split(v, h, lo);
+        vh[k] = h;
+        vl[k] = lo;
     }
     *reinterpret_cast<bf16x8*>(lds + 16 * l) = vh;
     *reinterpret_cast<bf16x8*>(lds + 16 * l + 8) = vl;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    // layer 1: D1[mb][nb] = W1[mb] x X[nb]
-    f32x4 d1[2][4];
+    float part[4];
 #pragma unroll
     for (int nb = 0; nb < 4; nb++) {
         const int node = nb * 16 + (l & 15);
@@ -96,40 +104,33 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
                 bl[j] = (__bf16)0.0f;
             }
         }
+        // layer 1 + activation -> layer-2 B operand (8 values per lane)
+        bf16x8 hh, hl;
 #pragma unroll
         for (int mb = 0; mb < 2; mb++) {
-            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+            f32x4 c = {fr.b1[4 * mb], fr.b1[4 * mb + 1], fr.b1[4 * mb + 2], fr.b1[4 * mb + 3]};
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1h[mb], bh, c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1h[mb], bl, c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1l[mb], bh, c, 0, 0, 0);
-            d1[mb][nb] = c;
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    // bias + SiLU -> layer-2 B operand (8 values per lane and node block)
-    float part[4];
-#pragma unroll
-    for (int nb = 0; nb < 4; nb++) {
-        bf16x8 bh, bl;
-#pragma unroll
-        for (int mb = 0; mb < 2; mb++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const float h = silu_f(d1[mb][nb][r] + fr.b1[4 * mb + r]);
-                __bf16 hh, hl;
-                split(h, hh, hl);
-                bh[4 * mb + r] = hh;
-                bl[4 * mb + r] = hl;
+                const float t = c[r];
+                __bf16 h, lo;
+                split(t * sig2(t), h, lo);
+                hh[4 * mb + r] = h;
+                hl[4 * mb + r] = lo;
             }
+        }
+        // layer 2 + activation + output weights
         float acc = 0.0f;
 #pragma unroll
         for (int mb = 0; mb < 2; mb++) {
-            f32x4 c = {0.f, 0.f, 0.f, 0.f};
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], bh, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], bl, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2l[mb], bh, c, 0, 0, 0);
+            f32x4 c = {fr.b2[4 * mb], fr.b2[4 * mb + 1], fr.b2[4 * mb + 2], fr.b2[4 * mb + 3]};
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], hh, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], hl, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2l[mb], hh, c, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 4; r++) acc += fr.w3[4 * mb + r] * silu_f(c[r] + fr.b2[4 * mb + r]);
+            for (int r = 0; r < 4; r++) acc += (fr.w3[4 * mb + r] * c[r]) * sig2(c[r]);
         }
         part[nb] = acc;
     }

@@ -1,0 +1,72 @@
+"""Seeded inputs shared by the benchmark and the tests.
+
+- `xavier_blob`: relaxation-time network 6 -> 32 -> 32 -> 1 with
+  Xavier-uniform weights (numpy PCG64 seeded), in the parameter layout of
+  ippi_set_mlp (W1[32][6], b1[32], W2[32][32], b2[32], W3[32], b3, mu[6],
+  1/sigma[6]);
+- `place_cylinders`: cylinders with diameters uniform in [dmin, dmax] at
+  non-overlapping positions by rejection sampling, each marker support
+  (4x4 delta stencil) kept `gap` lattice units clear of every other body's
+  support and of the domain edges;
+- `markers_for`: marker count for ~1 lu spacing on the circle.
+
+The reference's own cylinder markers follow mesh.cpp:180-181; its network
+is a libtorch model loaded per call (mesh.cpp:2767-3260) and has no
+counterpart here beyond the parameter layout above.
+"""
+import math
+
+import numpy as np
+
+MLP_PARAMS = 6 * 32 + 32 + 32 * 32 + 32 + 32 + 1
+MLP_BLOB = MLP_PARAMS + 12
+
+
+def xavier_blob(seed, b3=-5.0, mu=(1.0, 0.05, 0.0, 2e-3, 0.0, 0.0), sig=(2e-3, 0.02, 0.02, 2e-3, 5e-4, 5e-4),
+                w3_scale=0.3):
+    """Seeded Xavier-uniform network. With the default b3 the output
+    softplus(.) + 0.5 sits a few 1e-3 above 0.5, so the >= 0.501 guard
+    accepts it at typical nodes."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MLP_BLOB)
+
+    def xav(n_in, n_out):
+        a = math.sqrt(6.0 / (n_in + n_out))
+        return rng.uniform(-a, a, (n_out, n_in))
+    p[0:192] = xav(6, 32).ravel()
+    p[224:1248] = xav(32, 32).ravel()
+    p[1280:1312] = xav(32, 1).ravel() * w3_scale
+    p[1312] = b3
+    p[1313:1319] = mu
+    p[1319:1325] = 1.0 / np.asarray(sig, dtype=np.float64)
+    return p
+
+
+def markers_for(diameter):
+    return int(round(math.pi * diameter))
+
+
+def place_cylinders(nx, ny, n, dmin, dmax, seed, gap=4.0, max_tries=100000):
+    """[(cx, cy, D, M)] for n cylinders; raises if they do not fit."""
+    rng = np.random.default_rng(seed)
+    out = []
+    reach = 2.0  # delta support half-width
+    tries = 0
+    while len(out) < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not place the cylinders")
+        d = float(rng.uniform(dmin, dmax))
+        r = 0.5 * d + reach + 1.0
+        cx = float(rng.uniform(r + 1, nx - r - 2))
+        cy = float(rng.uniform(r + 1, ny - r - 2))
+        ok = True
+        for (ox, oy, od, _) in out:
+            # bounding boxes of the supports must be gap apart (patches may not overlap)
+            rr = 0.5 * od + reach + 1.0
+            if abs(cx - ox) < r + rr + gap and abs(cy - oy) < r + rr + gap:
+                ok = False
+                break
+        if ok:
+            out.append((cx, cy, d, markers_for(d)))
+    return out
