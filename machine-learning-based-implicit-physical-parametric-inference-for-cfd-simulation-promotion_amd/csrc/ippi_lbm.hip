@@ -1572,9 +1572,12 @@ int launch_ib(ippi_handle* h, bool pull) {
     return IPPI_OK;
 }
 
+// Timing events only: no system-scope fence (cache writeback / invalidate)
+// when they are recorded, which would slow the kernels being timed; the
+// stream is synchronised before the elapsed times are read.
 hipEvent_t new_event(ippi_handle* h) {
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     (void)hipEventRecord(e, h->s);
     return e;
 }
