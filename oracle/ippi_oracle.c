@@ -23,7 +23,19 @@
  *                by brute force and numpy.roll, decay of a shear wave between
  *                halfway bounce-back walls, interpolation of linear
  *                fields, spread = transpose of interpolation.
+ *   inlet / outlet NEE and free-slip rows: per-node closed forms (rho = 1
+ *                and u = u_in at the inlet, u copied at the outlet, the
+ *                specular mirror at the walls).
+ *   or_noise     the published splitmix64 outputs for seed 0.
  *   MLP          torch autograd (FP64) and central finite differences.
+ *   or_adjoint_drag  central finite differences (h = 1e-4) of the forward
+ *                step for single nodes, random directions and the body
+ *                centre, on horizons of 1 to 50 steps; exact zeros outside
+ *                the domain of dependence; translation equivariance.
+ * Not restated, so no pin applies: the reference's whole step on its own
+ * flat-plate boundary set (mesh.cpp:228-356 with the 'm'/'d' split of
+ * mesh.cpp:1413) and its libtorch models (mesh.cpp:2767-3260): parity
+ * unpinned for those, and the HIP library does not implement them.
  */
 #include "ippi_oracle.h"
 #include <math.h>

@@ -134,3 +134,18 @@ def test_mrt_all_rates_equal_is_bgk(oracle):
     # (energy/energy-square, which use rho*(..) forms of the same quantities)
     for k in (0, 3, 4, 5, 6, 7, 8):
         assert abs((m[k] - mo[k]) - rates[k] * (m[k] - meq[k])) < 1e-15, k
+
+
+def test_noise_is_splitmix64(oracle):
+    """or_noise(seed, node, comp) is output k = 2 node + comp + 1 of the
+    published splitmix64 generator (Steele, Lea, Flood 2014; Vigna's
+    reference code) started from `seed`, mapped to [-1, 1) through its top
+    53 bits. Known answers: the first three outputs for seed 0."""
+    kat = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    to_unit = lambda z: 2.0 * ((z >> 11) * 2.0 ** -53) - 1.0
+    assert oracle.noise(0, 0, 0) == to_unit(kat[0])
+    assert oracle.noise(0, 0, 1) == to_unit(kat[1])
+    assert oracle.noise(0, 1, 0) == to_unit(kat[2])
+    v = np.array([oracle.noise(7, n, c) for n in range(2000) for c in (0, 1)])
+    assert np.all((v >= -1) & (v < 1))
+    assert abs(v.mean()) < 0.05 and abs(v.var() - 1 / 3) < 0.03
