@@ -33,7 +33,10 @@ Reported beside it:
                                FP64 adjoint;
                 multibody_8192x4096  MRT + LES + network tau, 16 cylinders
                                D ~ U[40, 120] (seeded placement), FP32 vs
-                               FP64, 5,000 steps, per-body drag error.
+                               FP64, 5,000 steps, per-body drag error;
+                accuracy_2048x1024  growth of the FP32-vs-FP64 velocity error
+                               on the headline workload over 20,000 steps,
+                               beside FP64 vs FP64 from a perturbed start.
 
 `python bench.py --copy-probe MB` runs only a plain device copy of MB
 megabytes (the calibration program for the rocprofv3 traffic counters).
@@ -248,6 +251,32 @@ def leg_multibody(ippi, steps, warmup):
     return out
 
 
+def leg_accuracy(ippi, marks=(1000, 2000, 5000, 10000, 20000)):
+    """Velocity error growth on the headline workload: FP32 vs FP64, and,
+    as a control, FP64 vs FP64 started from initial noise whose amplitude
+    differs by 1 part in 1e6 (a perturbation of ~1e-9 in u, the size of
+    FP32 rounding). Equal growth rates mean the divergence is the flow's
+    own sensitivity (vortex shedding at Re = 1e4), not a drift of the
+    reduced-precision path."""
+    a32, a64 = make_headline(ippi, "fp32"), make_headline(ippi, "fp64")
+    p64 = ippi.Solver(NX, NY, "mrt", "fp64", "inflow", les=True, tau0=TAU0, cs=CS, u_in=(U0, 0.0))
+    p64.init_fields(SEED, 1.0, (U0, 0.0), 1e-3 * (1 + 1e-6))
+    p64.add_body(CX, CY, DIAM, NMARK)
+    out = {"steps": list(marks), "fp32_vs_fp64": [], "fp64_perturbed_vs_fp64": []}
+    done = 0
+    for m in marks:
+        log(f"accuracy: to step {m}")
+        for s in (a32, a64, p64):
+            s.step(m - done)
+        done = m
+        r32, r64, rp = a32.fields(), a64.fields(), p64.fields()
+        out["fp32_vs_fp64"].append(vel_err(r32, r64))
+        out["fp64_perturbed_vs_fp64"].append(vel_err(rp, r64))
+    for s in (a32, a64, p64):
+        s.close()
+    return out
+
+
 def copy_probe(mb):
     import torch
     n = int(mb * (1 << 20)) // 4
@@ -266,8 +295,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison of the headline")
-    ap.add_argument("--legs", default="nn,adjoint,multibody",
-                    help="comma list of extra workloads (nn, adjoint, multibody) or 'none'")
+    ap.add_argument("--legs", default="nn,adjoint,multibody,accuracy",
+                    help="comma list of extra workloads (nn, adjoint, multibody, accuracy) or 'none'")
     ap.add_argument("--nn-steps", type=int, default=10000)
     ap.add_argument("--mb-steps", type=int, default=5000)
     ap.add_argument("--profile-steps", type=int, default=500,
@@ -323,6 +352,8 @@ def main():
         legs["adjoint_1024x512"] = leg_adjoint(ippi)
     if "multibody" in names:
         legs["multibody_8192x4096"] = leg_multibody(ippi, args.mb_steps, W)
+    if "accuracy" in names:
+        legs["accuracy_2048x1024"] = leg_accuracy(ippi)
 
     cpu = None
     if not args.no_cpu:

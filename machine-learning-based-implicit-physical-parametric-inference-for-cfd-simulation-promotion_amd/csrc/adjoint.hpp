@@ -42,8 +42,16 @@ template <typename T>
 struct NodeFwd {
     T rho, ux, uy;  // macroscopic (before the IB velocity correction)
     T fx, fy;       // IB force of the node
-    T om, ct;       // 1/tb, 1 - 1/(2 tb)
+    T om, ct, dl;   // 1/tb, 1 - 1/(2 tb) = dl/2, dl = 2 - 1/tb
 };
+
+// Relaxation quantities from tx = tb - 1/2 (dl keeps its digits near
+// tb = 1/2, where omega and 1 - omega/2 computed from tb would not).
+template <typename T>
+__device__ __forceinline__ NodeFwd<T> node_fwd(T rho, T ux, T uy, T fx, T fy, T tx) {
+    const T dl = (T(4) * tx) / (T(1) + T(2) * tx);
+    return NodeFwd<T>{rho, ux, uy, fx, fy, T(2) - dl, T(0.5) * dl, dl};
+}
 
 // Adjoint of the corrected velocity us (through feq and S) and of the force
 // through S alone.
@@ -109,9 +117,9 @@ __device__ __forceinline__ void node_adjoint(const T* l, const T* d, const WMom<
     const T ir = T(1) / s.rho;
     const T vx = lux * ir, vy = luy * ir;
     const T base = lr - (vx * s.ux + vy * s.uy);
-    const T keep = T(1) - s.om;
+    // (1 - omega) l = dl l - l
 #pragma unroll
-    for (int k = 0; k < 9; k++) nu[k] = keep * l[k] + base + (T)EX[k] * vx + (T)EY[k] * vy;
+    for (int k = 0; k < 9; k++) nu[k] = (s.dl * l[k] - l[k]) + base + (T)EX[k] * vx + (T)EY[k] * vy;
 }
 
 // Derivative of the 4-point delta (delta4).

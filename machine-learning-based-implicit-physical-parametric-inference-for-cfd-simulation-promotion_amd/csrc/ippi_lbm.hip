@@ -40,16 +40,16 @@ struct Patch {
 struct StepArgs {
     int nx, ny;
     double tau0, cs, uinx, uiny;
-    float tau0f, csf, uinxf, uinyf;
+    float tx0f, csf, uinxf, uinyf;  // FP32 path: tx0f = tau0 - 1/2
     int les;
     int tau_field;          // 1: per-node base tau from tau_ptr
-    const void* tau_ptr;    // double* (FP64) / float* (FP32)
+    const void* tau_ptr;    // double* tau (FP64) / float* tau - 1/2 (FP32)
     const double* patch_f;  // [node][2]
     int* flag;              // divergence flag
     int nbody;
     int nt;                 // non-temporal stores
     double nn_max;          // upper bound of accepted network values
-    float nn_maxf;
+    float nn_maxtxf;        // nn_max - 1/2
     MlpDev mlp;             // relaxation-time network (tau_mode NN)
     Patch patch[IPPI_MAXB];
 };
@@ -440,8 +440,9 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
             o.uy += fy * h;
         }
     }
-    const float tb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tau0f;
-    float tau = tb;
+    // relaxation as tx = tau - 1/2 (see lbm_node.hpp)
+    const float txb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tx0f;
+    float tx = txb;
     const float rho = o.rho, ux = o.ux, uy = o.uy;
     const float rux = rho * ux, ruy = rho * uy;
     // non-equilibrium second moments: sum e e g - (rho u u + drho/3 I)
@@ -458,16 +459,16 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         feat[3] = 1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy));
         feat[4] = nxx - nyy;
         feat[5] = nxy;
-        const float t = mlp_tau_wave(feat, a.mlp, lds);
-        nn_ok = t >= 0.501f && t <= a.nn_maxf;
-        if (nn_ok) tau = t;
+        const float t = mlp_tau_wave(feat, a.mlp, lds);  // tau - 1/2
+        nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
+        if (nn_ok) tx = t;
     }
-    if (a.les && !nn_ok) tau = sh::les_tau(nxx, nxy, nyy, rho, tb, a.csf);
-    const float om = sh::rcp(tau);
+    if (a.les && !nn_ok) tx = sh::les_tx(nxx, nxy, nyy, rho, txb, a.csf);
+    const float dl = sh::dl_of(tx);  // 2 - omega
     if (COLL == IPPI_COLL_RR) {
         float q[9];
         sh::geq(o.drho, rho, ux, uy, q);
-        sh::rr(g, q, nxx, nxy, nyy, tau, fx, fy);
+        sh::rr(g, q, nxx, nxy, nyy, dl, fx, fy);
     } else {
         // moments of g (Lallemand-Luo basis, NodeAndLattice.cpp:18-27)
         const float s4 = o.s13 + o.s24, s8 = o.s57 + o.s68;
@@ -480,29 +481,31 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         const float t1 = ux * ux + uy * uy, t2 = ux * ux - uy * uy;
         const float e1 = -2.0f * o.drho + 3.0f * rho * t1;
         const float e2 = o.drho - 3.0f * rho * t1;
-        float r1, r2, r4;
-        if (COLL == IPPI_COLL_MRT) {  // NodeAndLattice.cpp:306
-            r1 = 1.2f;
-            r2 = 1.1f;
-            r4 = 1.2f;
+        // c_l = s_l (m_l - meq_l) / |M_l|^2; a rate s_l = omega = 2 - dl is
+        // applied as 2 x - dl x (one rounding, no rounded omega)
+        auto om_x = [dl](float x) { return 2.0f * x - dl * x; };
+        const float x1 = (m1 - e1) * (1.0f / 36.0f), x2 = (m2 - e2) * (1.0f / 36.0f);
+        const float x4 = (m4 + rux) * (1.0f / 12.0f), x6 = (m6 + ruy) * (1.0f / 12.0f);
+        float c1, c2, c4, c6;
+        if (COLL == IPPI_COLL_MRT) {  // fixed rates, NodeAndLattice.cpp:306
+            c1 = 1.2f * x1;
+            c2 = 1.1f * x2;
+            c4 = 1.2f * x4;
+            c6 = 1.2f * x6;
         } else {
-            r1 = om;
-            r2 = om;
-            r4 = om;
+            c1 = om_x(x1);
+            c2 = om_x(x2);
+            c4 = om_x(x4);
+            c6 = om_x(x6);
         }
-        // c_l = s_l (m_l - meq_l) / |M_l|^2
-        const float c1 = r1 * (m1 - e1) * (1.0f / 36.0f);
-        const float c2 = r2 * (m2 - e2) * (1.0f / 36.0f);
-        const float c4 = r4 * (m4 + rux) * (1.0f / 12.0f);
-        const float c6 = r4 * (m6 + ruy) * (1.0f / 12.0f);
-        const float c7 = om * (m7 - rho * t2) * 0.25f;
-        const float c8 = om * (m8 - rux * uy) * 0.25f;
+        const float c7 = om_x((m7 - rho * t2) * 0.25f);
+        const float c8 = om_x((m8 - rux * uy) * 0.25f);
         // BGK relaxes the momentum towards rho u (shifted by F/2 under IB
         // forcing): exact target -omega F/2; MRT does not relax it.
         float c3 = 0.f, c5 = 0.f;
         if (COLL == IPPI_COLL_BGK) {
-            c3 = -0.5f * om * fx * (1.0f / 6.0f);
-            c5 = -0.5f * om * fy * (1.0f / 6.0f);
+            c3 = om_x(fx * (-1.0f / 12.0f));
+            c5 = om_x(fy * (-1.0f / 12.0f));
         }
         const float A = -c1 - 2.0f * c2, B = 2.0f * c1 + c2;
         const float p46 = c4 + c6, q64 = c6 - c4;
@@ -516,7 +519,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         g[6] -= B + q64 - c8 - c3 + c5;
         g[8] -= B - q64 - c8 + c3 - c5;
     }
-    if (forced) sh::guo(ux, uy, fx, fy, tb, g);
+    if (forced) sh::guo(ux, uy, fx, fy, sh::dl_of(txb), g);
     if (active) {
 #pragma unroll
         for (int k = 0; k < 9; k++) st(dst + k * N, n, g[k], a.nt);
@@ -572,22 +575,22 @@ __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, dou
     float g[9];
     Mo o;
     node_state32<BC>(src, x, y, nx, ny, N, a.uinxf, a.uinyf, g, o);
-    const float tb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tau0f;
+    const float txb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tx0f;
     const float d3 = o.drho * (1.0f / 3.0f);
     const float nxx = (o.s13 + (o.s57 + o.s68)) - (o.rho * o.ux * o.ux + d3);
     const float nyy = (o.s24 + (o.s57 + o.s68)) - (o.rho * o.uy * o.uy + d3);
     const float nxy = (o.s57 - o.s68) - o.rho * o.ux * o.uy;
-    float tau = tb;
+    float tx = txb;
     bool nn_ok = false;
     if (NN) {
         float feat[6] = {o.rho, o.ux, o.uy,
                          1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy)), nxx - nyy, nxy};
         const float t = mlp_tau_wave(feat, a.mlp, lds_mlp + (threadIdx.x & ~63) * 16);
-        nn_ok = t >= 0.501f && t <= a.nn_maxf;
-        if (nn_ok) tau = t;
+        nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
+        if (nn_ok) tx = t;
     }
-    if (a.les && !nn_ok) tau = sh::les_tau(nxx, nxy, nyy, o.rho, tb, a.csf);
-    if (active) out[n] = tau;
+    if (a.les && !nn_ok) tx = sh::les_tx(nxx, nxy, nyy, o.rho, txb, a.csf);
+    if (active) out[n] = 0.5 + (double)tx;
 }
 
 template <int BC, bool NN>
@@ -1168,11 +1171,16 @@ __device__ __forceinline__ void adj_fwd(const T* __restrict__ src, const double*
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T adj_tau(const AdjArgs& a, unsigned n) {
-    return a.tau_field ? ld(static_cast<const T*>(a.tau_ptr), n) : (T)a.tau0;
+// Base tau - 1/2 of node n in the adjoint's precision: the FP64 state
+// keeps tau, the FP32 state tau - 1/2.
+template <typename S, typename A>
+__device__ __forceinline__ A adj_tx(const AdjArgs& a, unsigned n) {
+    if (!a.tau_field) return (A)(a.tau0 - 0.5);
+    if constexpr (sizeof(S) == 8)
+        return (A)(ld(static_cast<const double*>(a.tau_ptr), n) - 0.5);
+    else
+        return (A)ld(static_cast<const float*>(a.tau_ptr), n);
 }
-
 // S: type of the stored forward populations, A: adjoint arithmetic and
 // storage.
 template <typename S, typename A, bool PULL>
@@ -1191,9 +1199,8 @@ __global__ void __launch_bounds__(256) k_adj_act(const A* __restrict__ lin, cons
 #pragma unroll
     for (int k = 0; k < 9; k++) l[k] -= c0;
     adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, f, drho, rho, ux, uy);
-    const A om = A(1) / (A)adj_tau<S>(a, (unsigned)node);
-    const adj::NodeFwd<A> s{(A)rho, (A)ux, (A)uy, (A)a.patch_f[2 * r], (A)a.patch_f[2 * r + 1], om,
-                            A(1) - A(0.5) * om};
+    const adj::NodeFwd<A> s = adj::node_fwd<A>((A)rho, (A)ux, (A)uy, (A)a.patch_f[2 * r],
+                                                (A)a.patch_f[2 * r + 1], adj_tx<S, A>(a, (unsigned)node));
     const adj::WMom<A> m = adj::wmoments(l);
     const adj::UsAdj<A> ua = adj::us_adjoint(m, s, true);
     A lfx, lfy;
@@ -1289,8 +1296,7 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
         lix = (A)a.patch_l[2 * r];
         liy = (A)a.patch_l[2 * r + 1];
     }
-    const A om = A(1) / (A)adj_tau<S>(a, n);
-    const adj::NodeFwd<A> s{(A)rho, (A)ux, (A)uy, fx, fy, om, A(1) - A(0.5) * om};
+    const adj::NodeFwd<A> s = adj::node_fwd<A>((A)rho, (A)ux, (A)uy, fx, fy, adj_tx<S, A>(a, n));
     const adj::WMom<A> m = adj::wmoments(l);
     const adj::UsAdj<A> ua = adj::us_adjoint(m, s, forced);
     // non-equilibrium part in the representation of the stored populations
@@ -1423,7 +1429,7 @@ StepArgs make_args(ippi_handle* h) {
     a.cs = h->c.cs;
     a.uinx = h->c.u_in_x;
     a.uiny = h->c.u_in_y;
-    a.tau0f = (float)h->c.tau0;
+    a.tx0f = (float)(h->c.tau0 - 0.5);
     a.csf = (float)h->c.cs;
     a.uinxf = (float)h->c.u_in_x;
     a.uinyf = (float)h->c.u_in_y;
@@ -1435,7 +1441,7 @@ StepArgs make_args(ippi_handle* h) {
     a.nbody = (int)h->bodies.size();
     a.nt = 2 * 9 * h->N * h->esz > ((size_t)192 << 20);
     a.nn_max = h->c.tau_nn_max > 0 ? h->c.tau_nn_max : 1e30;
-    a.nn_maxf = (float)a.nn_max;
+    a.nn_maxtxf = (float)(a.nn_max - 0.5);
     a.mlp.frag = h->d_frag;
     a.mlp.blob = h->d_blob;
     for (int i = 0; i < 6; i++) {
@@ -2146,7 +2152,7 @@ int ippi_set_tau_field(ippi_handle* h, const double* tau) {
         HIPC(h, hipMemcpy(h->tau, tau, sizeof(double) * h->N, hipMemcpyHostToDevice));
     } else {
         std::vector<float> t(h->N);
-        for (size_t n = 0; n < h->N; n++) t[n] = (float)tau[n];
+        for (size_t n = 0; n < h->N; n++) t[n] = (float)(tau[n] - 0.5);  // FP32 path keeps tau - 1/2
         HIPC(h, hipMemcpy(h->tau, t.data(), sizeof(float) * h->N, hipMemcpyHostToDevice));
     }
     return IPPI_OK;

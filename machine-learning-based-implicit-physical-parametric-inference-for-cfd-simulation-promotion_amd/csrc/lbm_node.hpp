@@ -273,6 +273,15 @@ IPPI_HD double les_tau(const double* f, double rho, double ux, double uy, double
 
 // ------------------------------------------------------------------ FP32
 // g_k = f_k - w_k; rho = 1 + sum g; rho u = sum g e (sum w e = 0).
+//
+// Relaxation times are carried as tx = tau - 1/2, the quantity the
+// viscosity is proportional to (nu = tx / 3). Near tau = 1/2 (high
+// Reynolds numbers) tau and omega = 1/tau hold only a few significant
+// digits of tx in FP32: rounding tau0 = 0.50192 to FP32 alone lowers the
+// viscosity by 8e-6 relative, a bias that the flow integrates step after
+// step. Every FP32 formula below takes tx and uses
+//     dl = 2 - omega = 4 tx / (1 + 2 tx)   and   1 - omega / 2 = dl / 2,
+// both with full relative precision.
 namespace sh {
 
 // hardware reciprocal / square root (1 ulp) on the device, exact on the host
@@ -328,14 +337,22 @@ IPPI_HD void geq(float drho, float rho, float ux, float uy, float* q) {
     q[6] = WF[6] * (drho + ad - bd);
 }
 
-IPPI_HD float les_tau(float pxx, float pxy, float pyy, float rho, float tau0, float cs) {
+// 2 - omega for tau = 1/2 + tx (correctly rounded division)
+IPPI_HD float dl_of(float tx) { return (4.0f * tx) / (1.0f + 2.0f * tx); }
+
+// Hybrid Smagorinsky tau (mesh.cpp:877-910) as tx = tau - 1/2:
+// tau = (sqrt(tau0^2 + X) + tau0) / 2 with X = 18 Cs^2 Q / rho, written
+// tx = tx0 + X / (2 (sqrt(tau0^2 + X) + tau0)) so that tx0 and the eddy
+// part keep their digits.
+IPPI_HD float les_tx(float pxx, float pxy, float pyy, float rho, float tx0, float cs) {
 #pragma clang fp contract(fast)
+    const float tau0 = 0.5f + tx0;
     float ir = rcp(rho);
     float c = 1.5f * ir * rcp(tau0);
     float S2 = c * c * (pxx * pxx + 2.0f * pxy * pxy + pyy * pyy);
     float Q = fsqrt(2.0f * S2);
-    float g1 = tau0 * tau0 + 18.0f * cs * cs * Q * ir;
-    return 0.5f * (fsqrt(g1) + tau0);
+    float X = 18.0f * cs * cs * Q * ir;
+    return tx0 + X * rcp(2.0f * (fsqrt(tau0 * tau0 + X) + tau0));
 }
 
 // Set the mass and momentum of a collision increment d to their exact
@@ -357,9 +374,10 @@ IPPI_HD void conserve(float* d, float tx, float ty) {
     for (int k = 0; k < 9; k++) d[k] -= WF[k] * (m + (float)EX[k] * jx + (float)EY[k] * jy);
 }
 
-IPPI_HD void rr(float* g, const float* q, float pxx, float pxy, float pyy, float tau, float fx, float fy) {
+// Regularized collision (NodeAndLattice.cpp:228-286): fcol = feq +
+// (1 - omega) f1 with 1 - omega = dl - 1, taken as dl f1 - f1.
+IPPI_HD void rr(float* g, const float* q, float pxx, float pxy, float pyy, float dl, float fx, float fy) {
 #pragma clang fp contract(fast)
-    float a = 1.0f - rcp(tau);
     const float third = 1.0f / 3.0f;
     float d[9];
 #pragma unroll
@@ -368,16 +386,19 @@ IPPI_HD void rr(float* g, const float* q, float pxx, float pxy, float pyy, float
         float q3 = (float)(EY[k] * EY[k]) - third;
         float s = q0 * pxx + q3 * pyy;
         if (EX[k] * EY[k] != 0) s += 2.0f * (float)(EX[k] * EY[k]) * pxy;
-        d[k] = (q[k] - g[k]) + a * (4.5f * WF[k]) * s;
+        const float f1 = (4.5f * WF[k]) * s;
+        d[k] = (q[k] - g[k]) + (dl * f1 - f1);
     }
     conserve(d, 0.5f * fx, 0.5f * fy);
 #pragma unroll
     for (int k = 0; k < 9; k++) g[k] += d[k];
 }
 
-IPPI_HD void guo(float ux, float uy, float fx, float fy, float tau_begin, float* g) {
+// Guo source (NodeAndLattice.cpp:162-195) with 1 - 1/(2 TauAtBeginning)
+// = dl_begin / 2.
+IPPI_HD void guo(float ux, float uy, float fx, float fy, float dl_begin, float* g) {
 #pragma clang fp contract(fast)
-    float omega = 1.0f - 0.5f * rcp(tau_begin);
+    float omega = 0.5f * dl_begin;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         float ex = (float)EX[k], ey = (float)EY[k];

@@ -59,14 +59,16 @@ struct MlpDev {
 __device__ __forceinline__ float sig2(float t) {
     return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
-__device__ __forceinline__ float softplus_f(float z) { return z > 15.0f ? z : __logf(1.0f + __expf(z)); }
+// softplus with log1p: the output is tau - 1/2, often ~1e-3, and
+// log(1 + e^z) would lose its digits to the rounding of 1 + e^z
+__device__ __forceinline__ float softplus_f(float z) { return z > 15.0f ? z : log1pf(__expf(z)); }
 
 __device__ __forceinline__ void split(float v, __bf16& h, __bf16& l) {
     h = (__bf16)v;
     l = (__bf16)(v - (float)h);
 }
 
-// tau of this lane's node from its six raw features; every lane of the wave
+// tau - 1/2 of this lane's node from its six raw features; every lane of the wave
 // must call it (the MFMAs are wave-wide). lds: 64 x 8 bf16 x 2 (hi, lo) per
 // wave. The 64 nodes are handled as four blocks of 16; each block runs
 // layer 1, the activation, layer 2 and the output sum before the next, so
@@ -141,7 +143,7 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
         part[nb] += __shfl_xor(part[nb], 32);
     }
     const float o = (g == 0 ? part[0] : g == 1 ? part[1] : g == 2 ? part[2] : part[3]) + m.b3;
-    return softplus_f(o) + 0.5f;
+    return softplus_f(o);
 }
 
 // FP64 network on VALU, the oracle's loop order (weights read wave-uniformly).
