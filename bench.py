@@ -36,7 +36,10 @@ Reported beside it:
                                FP64, 5,000 steps, per-body drag error;
                 accuracy_2048x1024  growth of the FP32-vs-FP64 velocity error
                                on the headline workload over 20,000 steps,
-                               beside FP64 vs FP64 from a perturbed start.
+                               beside FP64 vs FP64 from a perturbed start;
+                flatplate_2401x501  the reference's own run (flat plate,
+                               BGK + LES, Re = 5e5): FP32 and FP64 MLUPS and
+                               the same error comparison.
 
 `python bench.py --copy-probe MB` runs only a plain device copy of MB
 megabytes (the calibration program for the rocprofv3 traffic counters).
@@ -277,6 +280,45 @@ def leg_accuracy(ippi, marks=(1000, 2000, 5000, 10000, 20000)):
     return out
 
 
+def leg_flatplate(ippi, steps=5000, marks=(1000, 5000)):
+    """The reference's own run (IPPI-LBM/main.cpp): flat plate on
+    2401 x 501, BGK + hybrid LES (Cs = 0.173), U = 0.1, Re = 5e5
+    (tau = 0.5 + 6 Ny U / Re = 0.5006), free slip ahead of x = 400 and no
+    slip beyond, uniform start. MLUPS of FP32 and FP64 over `steps`
+    steps, and the FP32-vs-FP64 velocity error next to FP64 vs FP64 from a
+    start perturbed by 1e-9 noise: the flow amplifies either perturbation."""
+    nx, ny, u = 2401, 501, 0.1
+    tau = 0.5 + 6.0 * (ny * u) / 5e5
+    out = {"workload": f"flat plate 2401x501 (the reference's main.cpp run), BGK+LES(Cs=0.173), U=0.1, Re=5e5, "
+                       f"tau={tau:.7f}, no-slip from x=400", "steps": steps}
+
+    def make(prec, noise):
+        s = ippi.Solver(nx, ny, "bgk", prec, "flatplate", les=True, tau0=tau, cs=0.173, u_in=(u, 0.0), plate_x=400)
+        s.init_fields(1, 1.0, (u, 0.0), noise)
+        return s
+    for prec in ("fp32", "fp64"):
+        s = make(prec, 0.0)
+        log(f"flat plate {prec}: {steps} steps")
+        dt = timed_run(s, steps, 20)
+        out[f"mlups_{prec}"] = round(nx * ny * steps / dt / 1e6, 1)
+        s.close()
+    out["speedup_vs_fp64"] = round(out["mlups_fp32"] / out["mlups_fp64"], 3)
+    runs = {"fp32": make("fp32", 0.0), "fp64": make("fp64", 0.0), "fp64_noise1e-9": make("fp64", 1e-9)}
+    out["error_steps"] = list(marks)
+    out["rel_l2_u_fp32_vs_fp64"], out["rel_l2_u_fp64_noise1e-9_vs_fp64"] = [], []
+    done = 0
+    for mk in marks:
+        for r in runs.values():
+            r.step(mk - done)
+        done = mk
+        f = {k: r.fields() for k, r in runs.items()}
+        out["rel_l2_u_fp32_vs_fp64"].append(vel_err(f["fp32"], f["fp64"]))
+        out["rel_l2_u_fp64_noise1e-9_vs_fp64"].append(vel_err(f["fp64_noise1e-9"], f["fp64"]))
+    for r in runs.values():
+        r.close()
+    return out
+
+
 def copy_probe(mb):
     import torch
     n = int(mb * (1 << 20)) // 4
@@ -295,8 +337,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison of the headline")
-    ap.add_argument("--legs", default="nn,adjoint,multibody,accuracy",
-                    help="comma list of extra workloads (nn, adjoint, multibody, accuracy) or 'none'")
+    ap.add_argument("--legs", default="nn,adjoint,multibody,accuracy,flatplate",
+                    help="comma list of extra workloads (nn, adjoint, multibody, accuracy, flatplate) or 'none'")
     ap.add_argument("--nn-steps", type=int, default=10000)
     ap.add_argument("--mb-steps", type=int, default=5000)
     ap.add_argument("--profile-steps", type=int, default=500,
@@ -354,6 +396,8 @@ def main():
         legs["multibody_8192x4096"] = leg_multibody(ippi, args.mb_steps, W)
     if "accuracy" in names:
         legs["accuracy_2048x1024"] = leg_accuracy(ippi)
+    if "flatplate" in names:
+        legs["flatplate_2401x501"] = leg_flatplate(ippi)
 
     cpu = None
     if not args.no_cpu:

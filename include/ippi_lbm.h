@@ -50,7 +50,22 @@ extern "C" {
 
 enum ippi_collision { IPPI_COLL_BGK = 0, IPPI_COLL_MRT = 1, IPPI_COLL_RR = 2 };
 enum ippi_precision { IPPI_FP64 = 0, IPPI_FP32_SHIFTED = 1 };
-enum ippi_bc { IPPI_BC_PERIODIC_BB = 0, IPPI_BC_INFLOW = 1 };
+enum ippi_bc { IPPI_BC_PERIODIC_BB = 0, IPPI_BC_INFLOW = 1, IPPI_BC_FLATPLATE = 2 };
+/* Boundary kinds:
+ *   IPPI_BC_PERIODIC_BB  periodic x (the reference's ring halo), halfway
+ *                        bounce-back walls at y = 0 and y = ny-1.
+ *   IPPI_BC_INFLOW       NEE inlet (rho = 1, u = u_in) and outlet (rho = 1,
+ *                        u copied from x = nx-2) at x = 0 and nx-1
+ *                        (mesh.cpp:239-286); specular free slip at both walls
+ *                        (the 'm' rule, mesh.cpp:325-337, mirrored on top).
+ *   IPPI_BC_FLATPLATE    the reference's own set (Mesh::BoundaryCondition,
+ *                        mesh.cpp:228-356, styles mesh.cpp:1391-1454), x being
+ *                        the reference's global column: NEE inlet and outlet on
+ *                        1 <= y <= ny-2; NEE top (rho = 1, u from y = ny-2);
+ *                        bottom free slip ('m') for x < plate_x and NEE no-slip
+ *                        ('d': rho from y = 1, u = 0) beyond; the corners
+ *                        (0, 0) and (nx-1, 0), which no rule reaches, keep
+ *                        their state. */
 enum ippi_tau_mode { IPPI_TAU_CONST = 0, IPPI_TAU_FIELD = 1, IPPI_TAU_NN = 2 };
 
 typedef struct ippi_config {
@@ -62,12 +77,14 @@ typedef struct ippi_config {
     int tau_mode;        /* ippi_tau_mode */
     double tau0;         /* startTau = TauAtBeginning */
     double cs;           /* Smagorinsky constant Cs */
-    double u_in_x;       /* inlet velocity for IPPI_BC_INFLOW */
+    double u_in_x;       /* inlet velocity for IPPI_BC_INFLOW and IPPI_BC_FLATPLATE */
     double u_in_y;
     int device;          /* HIP device ordinal */
     int force_history;   /* body-force records kept (ring); 0 -> 65536 */
     double tau_nn_max;   /* IPPI_TAU_NN: network values above this are rejected
                             like those below 0.501 (0 = no upper bound) */
+    int plate_x;         /* IPPI_BC_FLATPLATE: first no-slip column of the bottom
+                            wall (the reference hard-codes 400, mesh.cpp:1413) */
 } ippi_config;
 
 typedef struct ippi_handle ippi_handle;

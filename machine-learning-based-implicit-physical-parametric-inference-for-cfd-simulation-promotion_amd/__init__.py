@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libippi_lbm.so")
 IPPI_OK, IPPI_ERR_ARG, IPPI_ERR_HIP, IPPI_ERR_DIVERGED, IPPI_ERR_STATE, IPPI_ERR_NOMEM = range(6)
 COLLISION = {"bgk": 0, "mrt": 1, "rr": 2}
 PRECISION = {"fp64": 0, "fp32": 1}
-BC = {"periodic_bb": 0, "inflow": 1}
+BC = {"periodic_bb": 0, "inflow": 1, "flatplate": 2}
 TAU_MODE = {"const": 0, "field": 1, "nn": 2}
 MLP_BLOB = 6 * 32 + 32 + 32 * 32 + 32 + 32 + 1 + 12
 IPPI_ADJ_FP64 = 1
@@ -40,7 +40,7 @@ class Config(ctypes.Structure):
         ("tau0", ctypes.c_double), ("cs", ctypes.c_double),
         ("u_in_x", ctypes.c_double), ("u_in_y", ctypes.c_double),
         ("device", ctypes.c_int), ("force_history", ctypes.c_int),
-        ("tau_nn_max", ctypes.c_double),
+        ("tau_nn_max", ctypes.c_double), ("plate_x", ctypes.c_int),
     ]
 
 
@@ -128,13 +128,13 @@ class Solver:
 
     def __init__(self, nx, ny, collision="bgk", precision="fp64", bc="periodic_bb", les=False,
                  tau_mode="const", tau0=0.6, cs=0.0, u_in=(0.0, 0.0), device=0, force_history=0,
-                 tau_nn_max=0.0):
+                 tau_nn_max=0.0, plate_x=0):
         L = library()
         self.nx, self.ny, self.N = int(nx), int(ny), int(nx) * int(ny)
         self.precision = precision
         cfg = Config(self.nx, self.ny, COLLISION[collision], PRECISION[precision], BC[bc], int(bool(les)),
                      TAU_MODE[tau_mode], float(tau0), float(cs), float(u_in[0]), float(u_in[1]), int(device),
-                     int(force_history), float(tau_nn_max))
+                     int(force_history), float(tau_nn_max), int(plate_x))
         h = ctypes.c_void_p()
         rc = L.ippi_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != IPPI_OK:

@@ -51,6 +51,9 @@ struct StepArgs {
     double nn_max;          // upper bound of accepted network values
     float nn_maxtxf;        // nn_max - 1/2
     MlpDev mlp;             // relaxation-time network (tau_mode NN)
+    int plate_x;            // IPPI_BC_FLATPLATE: first no-slip column of the bottom wall
+    const double* corner;   // IPPI_BC_FLATPLATE: populations of (0, 0) and (nx-1, 0) ([2][9], full f)
+    const double* mac;      // macro arrays (rho, ux, uy)
     Patch patch[IPPI_MAXB];
 };
 
@@ -226,6 +229,104 @@ __device__ __forceinline__ void node_state32(const float* __restrict__ s, int x,
     }
 }
 
+// ------------------------------------------------------------- flat plate
+// IPPI_BC_FLATPLATE, the reference's own boundary set (Mesh::BoundaryCondition,
+// mesh.cpp:228-356, styles of Mesh::JudgeStyle, mesh.cpp:1391-1454), x being
+// the reference's global column. State of node (x, y) after streaming and
+// the boundary rules, with the macros its collision uses:
+//   1 <= x <= nx-2, 1 <= y <= ny-2: pull, macros from the populations;
+//   x = 0 (inlet), 1 <= y <= ny-2: rho = 1, u = u_in, non-equilibrium
+//     extrapolation (NEE) from (1, y);
+//   x = nx-1 (outlet), 1 <= y <= ny-2: rho = 1, u = u(nx-2, y), NEE from (nx-2, y);
+//   y = ny-1 (top), all x: rho = 1, u = u(x, ny-2), NEE from (x, ny-2), where
+//     (0, ny-2) and (nx-1, ny-2) are the inlet and outlet nodes just built;
+//   y = 0, 1 <= x < plate_x: specular free slip ('m'), macros from the populations;
+//   y = 0, plate_x <= x <= nx-2: rho = rho(x, 1), u = 0, NEE from (x, 1) ('d');
+//   (0, 0) and (nx-1, 0): no rule reaches them, so their populations and
+//     macros never change after initialisation. Returns false for them, with
+//     their stored post-collision populations in f (copied through).
+// FP64: the reference's arithmetic (NEE as feq_b + f_n - feq_n).
+__device__ __forceinline__ bool fp_state64(const double* __restrict__ s, int x, int y, int nx, int ny, unsigned N,
+                                           double uinx, double uiny, int plate_x, double* f, double& rho, double& ux,
+                                           double& uy) {
+#pragma clang fp contract(off)
+    auto nee = [&](double rb, double bx, double by) {
+        double qb[9], qn[9];
+        ref::feq(rb, bx, by, qb);
+        ref::feq(rho, ux, uy, qn);
+#pragma unroll
+        for (int k = 0; k < 9; k++) f[k] = qb[k] + f[k] - qn[k];
+        rho = rb;
+        ux = bx;
+        uy = by;
+    };
+    if (y == 0) {
+        if (x == 0 || x == nx - 1) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) f[k] = ld(s + k * N, (unsigned)x);
+            rho = 1.0;
+            ux = 0.0;
+            uy = 0.0;
+            return false;
+        }
+        pull<double, IPPI_BC_INFLOW>(s, x, x < plate_x ? 0 : 1, nx, ny, N, f);
+        ref::macro(f, rho, ux, uy);
+        if (x >= plate_x) nee(rho, 0.0, 0.0);
+        return true;
+    }
+    const int xs = x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x);
+    pull<double, IPPI_BC_INFLOW>(s, xs, y == ny - 1 ? ny - 2 : y, nx, ny, N, f);
+    ref::macro(f, rho, ux, uy);
+    if (x == 0)
+        nee(1.0, uinx, uiny);
+    else if (x == nx - 1)
+        nee(1.0, ux, uy);
+    if (y == ny - 1) nee(1.0, ux, uy);
+    return true;
+}
+
+// FP32 (shifted populations) counterpart of fp_state64.
+__device__ __forceinline__ bool fp_state32(const float* __restrict__ s, int x, int y, int nx, int ny, unsigned N,
+                                           float uinx, float uiny, int plate_x, float* g, Mo& o) {
+#pragma clang fp contract(fast)
+    auto nee = [&](float drb, float rb, float bx, float by) {
+        float qb[9], qn[9];
+        sh::geq(drb, rb, bx, by, qb);
+        sh::geq(o.drho, o.rho, o.ux, o.uy, qn);
+#pragma unroll
+        for (int k = 0; k < 9; k++) g[k] = qb[k] + (g[k] - qn[k]);
+        moments32(g, o);  // partial sums of the new populations
+        o.drho = drb;
+        o.rho = rb;
+        o.ir = sh::rcp(rb);
+        o.ux = bx;
+        o.uy = by;
+        o.jx = rb * bx;
+        o.jy = rb * by;
+    };
+    if (y == 0) {
+        if (x == 0 || x == nx - 1) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) g[k] = ld(s + k * N, (unsigned)x);
+            moments32(g, o);
+            return false;
+        }
+        pull<float, IPPI_BC_INFLOW>(s, x, x < plate_x ? 0 : 1, nx, ny, N, g);
+        moments32(g, o);
+        if (x >= plate_x) nee(o.drho, o.rho, 0.f, 0.f);
+        return true;
+    }
+    const int xs = x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x);
+    pull<float, IPPI_BC_INFLOW>(s, xs, y == ny - 1 ? ny - 2 : y, nx, ny, N, g);
+    moments32(g, o);
+    if (x == 0)
+        nee(0.f, 1.f, uinx, uiny);
+    else if (x == nx - 1)
+        nee(0.f, 1.f, o.ux, o.uy);
+    if (y == ny - 1) nee(0.f, 1.f, o.ux, o.uy);
+    return true;
+}
+
 // IB force of node (x, y) from the per-body patches (the block-level test
 // keeps the per-node test off all blocks that touch no patch).
 __device__ __forceinline__ bool patch_force(const StepArgs& a, int x, int y, int bx0, int bx1, int by0, int by1,
@@ -294,13 +395,17 @@ __device__ __forceinline__ double nn_tau_f64(const double* f, const double* q, d
 }
 
 template <int COLL, int BC, bool PULL, bool NN>
-__device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, const double* __restrict__ mac,
-                                         double* __restrict__ dst, const StepArgs& a, bool active) {
+__device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, const double* __restrict__ src,
+                                         const double* __restrict__ mac, double* __restrict__ dst, const StepArgs& a,
+                                         bool active) {
 #pragma clang fp contract(off)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
     double rho, ux, uy;
-    if (PULL) {
+    bool fresh = true;  // false: flat-plate corner, stored populations copied through
+    if (PULL && BC == IPPI_BC_FLATPLATE) {
+        fresh = fp_state64(src, x, y, nx, a.ny, N, a.uinx, a.uiny, a.plate_x, f, rho, ux, uy);
+    } else if (PULL) {
         ref::macro(f, rho, ux, uy);
         if (BC == IPPI_BC_INFLOW && (x == 0 || x == nx - 1)) {  // NEE, mesh.cpp:239-286
             double bx = x == 0 ? a.uinx : ux, by = x == 0 ? a.uiny : uy;
@@ -358,7 +463,7 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
     }
     if (active) {
 #pragma unroll
-        for (int k = 0; k < 9; k++) st(dst + k * N, n, o[k], a.nt);
+        for (int k = 0; k < 9; k++) st(dst + k * N, n, fresh ? o[k] : f[k], a.nt);
     }
 }
 
@@ -372,16 +477,17 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
     const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
     const int y0 = blockIdx.y * NPT;
     const unsigned N = (unsigned)nx * ny;
+    constexpr bool RAW = !(PULL && BC == IPPI_BC_FLATPLATE);  // the flat-plate rules load for themselves
     double cur[9];
-    load_raw<double, BC, PULL>(src, x, y0, nx, ny, N, cur);
+    if (RAW) load_raw<double, BC, PULL>(src, x, y0, nx, ny, N, cur);
 #pragma unroll
     for (int j = 0; j < NPT; j++) {
         const int y = y0 + j;
         if (y >= ny) break;
         const bool more = j + 1 < NPT && y + 1 < ny;
         double nxt[9];
-        if (j + 1 < NPT) load_raw<double, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f64<COLL, BC, PULL, NN>(cur, x, y, N, mac, dst, a, active);
+        if (RAW && j + 1 < NPT) load_raw<double, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
+        node_f64<COLL, BC, PULL, NN>(cur, x, y, N, src, mac, dst, a, active);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -396,14 +502,23 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
 // evaluated on populations with the increment projected onto exact mass and
 // momentum. The LES stress comes from the second moments directly.
 template <int COLL, int BC, bool PULL, bool NN>
-__device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const double* __restrict__ mac,
-                                         float* __restrict__ dst, const StepArgs& a, bool active, __bf16* lds) {
+__device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const float* __restrict__ src,
+                                         const double* __restrict__ mac, float* __restrict__ dst, const StepArgs& a,
+                                         bool active, __bf16* lds) {
 #pragma clang fp contract(fast)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
     Mo o;
-    moments32(g, o);
-    if (PULL) {
+    bool fresh = true;  // false: flat-plate corner, stored populations copied through
+    float keep[BC == IPPI_BC_FLATPLATE ? 9 : 1];
+    if (PULL && BC == IPPI_BC_FLATPLATE) {
+        fresh = fp_state32(src, x, y, nx, a.ny, N, a.uinxf, a.uinyf, a.plate_x, g, o);
+#pragma unroll
+        for (int k = 0; k < (BC == IPPI_BC_FLATPLATE ? 9 : 1); k++) keep[k] = g[k];
+    } else {
+        moments32(g, o);
+    }
+    if (PULL && BC != IPPI_BC_FLATPLATE) {
         if (BC == IPPI_BC_INFLOW && (x == 0 || x == nx - 1)) {  // NEE, mesh.cpp:239-286
             float bx = x == 0 ? a.uinxf : o.ux, by = x == 0 ? a.uinyf : o.uy;
             float qb[9], qn[9];
@@ -418,7 +533,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
             o.ux = bx;
             o.uy = by;
         }
-    } else {
+    } else if (!PULL) {
         o.rho = (float)mac[n];
         o.drho = (float)(mac[n] - 1.0);
         o.ir = sh::rcp(o.rho);
@@ -520,6 +635,10 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         g[8] -= B - q64 - c8 + c3 - c5;
     }
     if (forced) sh::guo(ux, uy, fx, fy, sh::dl_of(txb), g);
+    if (BC == IPPI_BC_FLATPLATE && !fresh) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) g[k] = keep[BC == IPPI_BC_FLATPLATE ? k : 0];
+    }
     if (active) {
 #pragma unroll
         for (int k = 0; k < 9; k++) st(dst + k * N, n, g[k], a.nt);
@@ -542,16 +661,17 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
     __shared__ __bf16 lds_mlp[NN ? BX * 16 : 1];
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
     const unsigned N = (unsigned)nx * ny;
+    constexpr bool RAW = !(PULL && BC == IPPI_BC_FLATPLATE);  // the flat-plate rules load for themselves
     float cur[9];
-    load_raw<float, BC, PULL>(src, x, y0, nx, ny, N, cur);
+    if (RAW) load_raw<float, BC, PULL>(src, x, y0, nx, ny, N, cur);
 #pragma unroll
     for (int j = 0; j < NPT; j++) {
         const int y = y0 + j;
         if (y >= ny) break;
         const bool more = j + 1 < NPT && y + 1 < ny;
         float nxt[9];
-        if (j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f32<COLL, BC, PULL, NN>(cur, x, y, N, mac, dst, a, active, lds);
+        if (RAW && j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
+        node_f32<COLL, BC, PULL, NN>(cur, x, y, N, src, mac, dst, a, active, lds);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -574,7 +694,22 @@ __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, dou
     __shared__ __bf16 lds_mlp[NN ? BX * 16 : 1];
     float g[9];
     Mo o;
-    node_state32<BC>(src, x, y, nx, ny, N, a.uinxf, a.uinyf, g, o);
+    if constexpr (BC == IPPI_BC_FLATPLATE) {
+        if (y == 0 && (x == 0 || x == nx - 1)) {  // corner: initial populations, macros from mac
+#pragma unroll
+            for (int k = 0; k < 9; k++) g[k] = (float)(a.corner[9 * (x != 0) + k] - WD[k]);
+            moments32(g, o);
+            o.rho = (float)a.mac[n];
+            o.drho = (float)(a.mac[n] - 1.0);
+            o.ir = sh::rcp(o.rho);
+            o.ux = (float)a.mac[N + n];
+            o.uy = (float)a.mac[2 * N + n];
+        } else {
+            fp_state32(src, x, y, nx, ny, N, a.uinxf, a.uinyf, a.plate_x, g, o);
+        }
+    } else {
+        node_state32<BC>(src, x, y, nx, ny, N, a.uinxf, a.uinyf, g, o);
+    }
     const float txb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tx0f;
     const float d3 = o.drho * (1.0f / 3.0f);
     const float nxx = (o.s13 + (o.s57 + o.s68)) - (o.rho * o.ux * o.ux + d3);
@@ -602,7 +737,19 @@ __global__ void __launch_bounds__(BX) k_tau64(const double* __restrict__ src, do
     if (x >= nx) return;
     const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
     double f[9], rho, ux, uy;
-    node_state(src, x, y, nx, ny, N, BC, a.uinx, a.uiny, f, rho, ux, uy);
+    if constexpr (BC == IPPI_BC_FLATPLATE) {
+        if (y == 0 && (x == 0 || x == nx - 1)) {  // corner: initial populations, macros from mac
+#pragma unroll
+            for (int k = 0; k < 9; k++) f[k] = a.corner[9 * (x != 0) + k];
+            rho = a.mac[n];
+            ux = a.mac[N + n];
+            uy = a.mac[2 * N + n];
+        } else {
+            fp_state64(src, x, y, nx, ny, N, a.uinx, a.uiny, a.plate_x, f, rho, ux, uy);
+        }
+    } else {
+        node_state(src, x, y, nx, ny, N, BC, a.uinx, a.uiny, f, rho, ux, uy);
+    }
     const double tb = a.tau_field ? ld(static_cast<const double*>(a.tau_ptr), n) : a.tau0;
     double q[9];
     ref::feq(rho, ux, uy, q);
@@ -795,9 +942,22 @@ __global__ void k_init(T* __restrict__ dst, double* __restrict__ mac, size_t N, 
 // Post-streaming populations and moments of the current state (reads only);
 // f_out may be null. FP32 moments are accumulated in FP64 from the stored
 // deviations.
+// Saves the populations of the flat-plate corners (0, 0) and (nx-1, 0)
+// from the stored initial state (full f, FP64): no rule updates them, so
+// they are the populations these nodes keep.
+template <typename T>
+__global__ void k_corner(const T* __restrict__ src, double* __restrict__ corner, int nx, unsigned N) {
+    const int t = threadIdx.x;
+    if (t >= 18) return;
+    const int k = t % 9;
+    double v = (double)src[(size_t)k * N + (t < 9 ? 0u : (unsigned)(nx - 1))];
+    if constexpr (sizeof(T) == 4) v += WD[k];
+    corner[t] = v;
+}
+
 template <typename T, int BC>
 __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, double* __restrict__ f_out, int nx,
-                        int ny, double uinx, double uiny) {
+                        int ny, double uinx, double uiny, int plate_x, const double* __restrict__ corner) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * BX + threadIdx.x;
     const int y = blockIdx.y;
@@ -805,7 +965,34 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
     const unsigned N = (unsigned)nx * ny;
     const unsigned n = (unsigned)y * nx + x;
     double f[9], rho, ux, uy;
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (BC == IPPI_BC_FLATPLATE) {
+        if (y == 0 && (x == 0 || x == nx - 1)) {  // corner: state of the initialisation, macros kept in mac
+            if (f_out) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) f_out[(size_t)k * N + n] = corner[9 * (x != 0) + k];
+            }
+            return;
+        }
+        if constexpr (sizeof(T) == 8) {
+            fp_state64(src, x, y, nx, ny, N, uinx, uiny, plate_x, f, rho, ux, uy);
+        } else {
+            float g[9];
+            Mo o;
+            fp_state32(src, x, y, nx, ny, N, (float)uinx, (float)uiny, plate_x, g, o);
+            const bool assigned = x == 0 || x == nx - 1 || y == ny - 1 || (y == 0 && x >= plate_x);
+            double gd[9], sm = 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) gd[k] = (double)g[k];
+            for (int k = 0; k < 9; k++) sm += gd[k];
+            const double jx = ((gd[1] - gd[3]) + (gd[5] - gd[7])) + (gd[8] - gd[6]);
+            const double jy = ((gd[2] - gd[4]) + (gd[5] - gd[7])) + (gd[6] - gd[8]);
+            rho = assigned ? (double)o.rho : 1.0 + sm;
+            ux = assigned ? (double)o.ux : jx / rho;
+            uy = assigned ? (double)o.uy : jy / rho;
+#pragma unroll
+            for (int k = 0; k < 9; k++) f[k] = gd[k] + WD[k];
+        }
+    } else if constexpr (sizeof(T) == 8) {
         node_state(src, x, y, nx, ny, N, BC, uinx, uiny, f, rho, ux, uy);
     } else {
         float g[9];
@@ -1212,32 +1399,55 @@ __global__ void __launch_bounds__(256) k_adj_act(const A* __restrict__ lin, cons
 }
 
 // One block per body. cj = weight of this step's force record in the
-// objective.
+// objective. One lane per (marker, support node) pair, in chunks of IB_T
+// pairs (32 markers) held in flight together; the sum over a marker's 16
+// supports is a shuffle tree inside its 16-lane group, the centre gradient
+// a fixed-order block tree, so the result is deterministic.
+constexpr int ADJ_CH = 4;  // chunks in flight per lane (IB_T * ADJ_CH / 16 = 128 markers)
 __global__ void __launch_bounds__(IB_T) k_adj_mk(AdjIb d, double cj) {
     __shared__ double LU[2 * IB_MAXM];
     __shared__ double red[2 * IB_T];
     const AdjBody b = d.body[blockIdx.x];
     const int t = threadIdx.x;
     double gx = 0.0, gy = 0.0;
-    for (int ml = t; ml < b.M; ml += IB_T) {
-        const int m = b.m0 + ml;
-        double ax = -cj * b.wx, ay = -cj * b.wy;  // record = - sum of marker forces
+    for (int c0 = 0; c0 < 16 * b.M; c0 += ADJ_CH * IB_T) {
+        double w[ADJ_CH], wxs[ADJ_CH], wys[ADJ_CH], lfx[ADJ_CH], lfy[ADJ_CH], ux[ADJ_CH], uy[ADJ_CH];
+        // all loads of the lane's pairs first (pairs past the body's end read pair 0, weight 0)
 #pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const int i = b.a0 + d.pidx[16 * m + s];
-            ax += d.w[16 * m + s] * d.lf[2 * i];
-            ay += d.w[16 * m + s] * d.lf[2 * i + 1];
+        for (int j = 0; j < ADJ_CH; j++) {
+            const int item = c0 + j * IB_T + t;
+            const bool ok = item < 16 * b.M;
+            const int q = 16 * b.m0 + (ok ? item : 0);
+            const int i = b.a0 + d.pidx[q];
+            w[j] = ok ? d.w[q] : 0.0;
+            wxs[j] = ok ? d.wx[q] : 0.0;
+            wys[j] = ok ? d.wy[q] : 0.0;
+            lfx[j] = d.lf[2 * i];
+            lfy[j] = d.lf[2 * i + 1];
+            ux[j] = d.u[2 * i];
+            uy[j] = d.u[2 * i + 1];
         }
-        const double lux = -2.0 * ax, luy = -2.0 * ay;  // F = 2 (U_d - U)
-        LU[2 * ml] = lux;
-        LU[2 * ml + 1] = luy;
-        const double Fx = d.F[2 * m], Fy = d.F[2 * m + 1];
 #pragma unroll
-        for (int s = 0; s < 16; s++) {
-            const int i = b.a0 + d.pidx[16 * m + s];
-            const double c = lux * d.u[2 * i] + luy * d.u[2 * i + 1] + d.lf[2 * i] * Fx + d.lf[2 * i + 1] * Fy;
-            gx += d.wx[16 * m + s] * c;
-            gy += d.wy[16 * m + s] * c;
+        for (int j = 0; j < ADJ_CH; j++) {
+            const int item = c0 + j * IB_T + t;
+            const int ml = min(item, 16 * b.M - 1) >> 4;  // marker of the pair (16-lane aligned)
+            double ax = w[j] * lfx[j], ay = w[j] * lfy[j];
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) {
+                ax += __shfl_xor(ax, o, 16);
+                ay += __shfl_xor(ay, o, 16);
+            }
+            // record = - sum of marker forces; F = 2 (U_d - U)
+            const double lux = -2.0 * (ax - cj * b.wx), luy = -2.0 * (ay - cj * b.wy);
+            if (item < 16 * b.M && (item & 15) == 0) {
+                LU[2 * ml] = lux;
+                LU[2 * ml + 1] = luy;
+            }
+            const int m = b.m0 + ml;
+            const double Fx = d.F[2 * m], Fy = d.F[2 * m + 1];
+            const double cc = lux * ux[j] + luy * uy[j] + lfx[j] * Fx + lfy[j] * Fy;
+            gx += wxs[j] * cc;
+            gy += wys[j] * cc;
         }
     }
     red[2 * t] = gx;
@@ -1336,6 +1546,7 @@ struct ippi_handle {
     int phase = 0;  // 0: buf[cur] holds fnew + macro arrays; 1: buf[cur] holds fcol
     int64_t steps = 0;
     double* mac = nullptr;  // rho, ux, uy (3N)
+    double* corner = nullptr;  // IPPI_BC_FLATPLATE corner populations [2][9]
     void* tau = nullptr;
     int* flag = nullptr;
     double* part = nullptr;
@@ -1417,6 +1628,9 @@ StepKernel<T> pick_step(int coll, int bc) {
     IPPI_PICK(IPPI_COLL_BGK, IPPI_BC_INFLOW)
     IPPI_PICK(IPPI_COLL_MRT, IPPI_BC_INFLOW)
     IPPI_PICK(IPPI_COLL_RR, IPPI_BC_INFLOW)
+    IPPI_PICK(IPPI_COLL_BGK, IPPI_BC_FLATPLATE)
+    IPPI_PICK(IPPI_COLL_MRT, IPPI_BC_FLATPLATE)
+    IPPI_PICK(IPPI_COLL_RR, IPPI_BC_FLATPLATE)
 #undef IPPI_PICK
     return nullptr;
 }
@@ -1449,6 +1663,9 @@ StepArgs make_args(ippi_handle* h) {
         a.mlp.isig[i] = h->mlp_isig[i];
     }
     a.mlp.b3 = h->mlp_b3;
+    a.plate_x = h->c.plate_x;
+    a.corner = h->corner;
+    a.mac = h->mac;
     for (size_t b = 0; b < h->bodies.size(); b++) a.patch[b] = h->bodies[b].p;
     return a;
 }
@@ -1630,6 +1847,11 @@ int run_steps(ippi_handle* h, int nsteps) {
     StepArgs a = make_args(h);
     for (int t = 0; t < nsteps; t++) {
         const bool pull = h->phase == 1;
+        if (!pull && h->c.bc == IPPI_BC_FLATPLATE) {
+            hipLaunchKernelGGL(k_corner<T>, dim3(1), dim3(64), 0, h->s, (const T*)h->buf[h->cur], h->corner, nx,
+                               (unsigned)h->N);
+            HIPC(h, hipGetLastError());
+        }
         if (ib) {
             if (h->timing) h->ev_ib.push_back(new_event(h));
             int r = launch_ib<T>(h, pull);
@@ -1670,12 +1892,18 @@ int run_macro(ippi_handle* h, double* f_out) {
     double* d_f = nullptr;
     if (f_out) HIPC(h, hipMalloc(&d_f, sizeof(double) * 9 * h->N));
     dim3 grid((h->c.nx + BX - 1) / BX, h->c.ny);
+    const T* src = (const T*)h->buf[h->cur];
+    const int nx = h->c.nx, ny = h->c.ny, px = h->c.plate_x;
+    const double ux = h->c.u_in_x, uy = h->c.u_in_y;
     if (h->c.bc == IPPI_BC_PERIODIC_BB)
-        hipLaunchKernelGGL((k_macro<T, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, (const T*)h->buf[h->cur], h->mac,
-                           d_f, h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y);
+        hipLaunchKernelGGL((k_macro<T, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
+                           px, h->corner);
+    else if (h->c.bc == IPPI_BC_INFLOW)
+        hipLaunchKernelGGL((k_macro<T, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
+                           px, h->corner);
     else
-        hipLaunchKernelGGL((k_macro<T, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, (const T*)h->buf[h->cur], h->mac,
-                           d_f, h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y);
+        hipLaunchKernelGGL((k_macro<T, IPPI_BC_FLATPLATE>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
+                           px, h->corner);
     HIPC(h, hipGetLastError());
     if (f_out) {
         HIPC(h, hipMemcpyAsync(f_out, d_f, sizeof(double) * 9 * h->N, hipMemcpyDeviceToHost, h->s));
@@ -1801,16 +2029,20 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     }
     r = check_flag(h);
     if (r) return r;
-    // objective from the force records of the window
+    // objective from the force records of the window (a contiguous range of
+    // the record ring, copied in at most two pieces)
     {
-        std::vector<double> rec(2 * (size_t)nb);
-        double acc = 0.0;
-        for (int t = avg_from - 1; t < nsteps; t++) {
-            const int64_t slot = (rec0 + t) % h->hist_cap;
-            HIPC(h, hipMemcpy(rec.data(), h->hist + 2 * (size_t)nb * slot, sizeof(double) * 2 * nb,
+        const size_t rb = 2 * (size_t)nb;
+        std::vector<double> rec(rb * (size_t)nwin);
+        const int64_t s0 = (rec0 + avg_from - 1) % h->hist_cap;
+        const int64_t n1 = std::min<int64_t>(nwin, h->hist_cap - s0);
+        HIPC(h, hipMemcpy(rec.data(), h->hist + rb * s0, sizeof(double) * rb * n1, hipMemcpyDeviceToHost));
+        if (n1 < nwin)
+            HIPC(h, hipMemcpy(rec.data() + rb * n1, h->hist, sizeof(double) * rb * (nwin - n1),
                               hipMemcpyDeviceToHost));
-            for (int b = 0; b < nb; b++) acc += tab[b].wx * rec[2 * b] + tab[b].wy * rec[2 * b + 1];
-        }
+        double acc = 0.0;
+        for (int t = 0; t < nwin; t++)
+            for (int b = 0; b < nb; b++) acc += tab[b].wx * rec[rb * t + 2 * b] + tab[b].wy * rec[rb * t + 2 * b + 1];
         if (J) *J = acc / nwin;
     }
 
@@ -1895,7 +2127,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (!cfg || !out) return IPPI_ERR_ARG;
     *out = nullptr;
     if (cfg->nx < 4 || cfg->ny < 4 || cfg->collision < 0 || cfg->collision > 2 || cfg->precision < 0 ||
-        cfg->precision > 1 || cfg->bc < 0 || cfg->bc > 1 || cfg->tau_mode < 0 || cfg->tau_mode > 2 ||
+        cfg->precision > 1 || cfg->bc < 0 || cfg->bc > 2 || cfg->tau_mode < 0 || cfg->tau_mode > 2 ||
         !(cfg->tau0 > 0.5) || cfg->tau_nn_max < 0)
         return IPPI_ERR_ARG;
     if (9 * (uint64_t)cfg->nx * (uint64_t)cfg->ny >= ((uint64_t)1 << 32)) return IPPI_ERR_ARG;
@@ -1912,6 +2144,8 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     for (int i = 0; i < 2; i++)
         if (hipMalloc(&h->buf[i], h->esz * 9 * h->N) != hipSuccess) return bail(IPPI_ERR_NOMEM);
     if (hipMalloc(&h->mac, sizeof(double) * 3 * h->N) != hipSuccess) return bail(IPPI_ERR_NOMEM);
+    if (hipMalloc(&h->corner, sizeof(double) * 18) != hipSuccess) return bail(IPPI_ERR_NOMEM);
+    if (hipMemset(h->corner, 0, sizeof(double) * 18) != hipSuccess) return bail(IPPI_ERR_HIP);
     if (hipMalloc(&h->flag, sizeof(int)) != hipSuccess) return bail(IPPI_ERR_NOMEM);
     if (hipMalloc(&h->part, sizeof(double) * 1024) != hipSuccess) return bail(IPPI_ERR_NOMEM);
     if (hipMemset(h->flag, 0, sizeof(int)) != hipSuccess) return bail(IPPI_ERR_HIP);
@@ -1925,7 +2159,7 @@ int ippi_destroy(ippi_handle* h) {
     clear_events(h->ev_step);
     clear_events(h->ev_ib);
     clear_events(h->ev_adj);
-    dfree(h->buf[0]); dfree(h->buf[1]); dfree(h->mac); dfree(h->tau); dfree(h->flag); dfree(h->part);
+    dfree(h->buf[0]); dfree(h->buf[1]); dfree(h->mac); dfree(h->corner); dfree(h->tau); dfree(h->flag); dfree(h->part);
     free_ib(h);
     dfree(h->d_frag);
     dfree(h->d_blob);
@@ -1975,6 +2209,8 @@ int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, f
     if (h->c.nx % 2 || h->c.ny % 2 || h->c.nx < 16 || h->c.ny < 16) return fail(h, IPPI_ERR_ARG, "grid must be even and >= 16");
     if (coarse_collision != IPPI_COLL_BGK && coarse_collision != IPPI_COLL_RR)
         return fail(h, IPPI_ERR_ARG, "labels are defined for BGK and RR coarse collisions");
+    if (h->c.bc == IPPI_BC_FLATPLATE)
+        return fail(h, IPPI_ERR_ARG, "training pairs are built for the periodic and inflow boundary kinds");
     if (!(tau_coarse > 0.5)) return IPPI_ERR_ARG;
     if (h->phase == 0) {
         int r = ippi_step(h, 1);
@@ -2285,20 +2521,25 @@ int ippi_get_tau(ippi_handle* h, double* tau) {
     double* d_out = nullptr;
     HIPC(h, hipMalloc(&d_out, sizeof(double) * h->N));
     dim3 grid((h->c.nx + BX - 1) / BX, h->c.ny);
-    const bool nn = h->c.tau_mode == IPPI_TAU_NN, pb = h->c.bc == IPPI_BC_PERIODIC_BB;
-    if (h->esz == 4) {
-        const float* src = (const float*)h->buf[h->cur];
-        if (pb && nn) hipLaunchKernelGGL((k_tau32<IPPI_BC_PERIODIC_BB, true>), grid, dim3(BX), 0, h->s, src, d_out, a);
-        if (pb && !nn) hipLaunchKernelGGL((k_tau32<IPPI_BC_PERIODIC_BB, false>), grid, dim3(BX), 0, h->s, src, d_out, a);
-        if (!pb && nn) hipLaunchKernelGGL((k_tau32<IPPI_BC_INFLOW, true>), grid, dim3(BX), 0, h->s, src, d_out, a);
-        if (!pb && !nn) hipLaunchKernelGGL((k_tau32<IPPI_BC_INFLOW, false>), grid, dim3(BX), 0, h->s, src, d_out, a);
-    } else {
-        const double* src = (const double*)h->buf[h->cur];
-        if (pb && nn) hipLaunchKernelGGL((k_tau64<IPPI_BC_PERIODIC_BB, true>), grid, dim3(BX), 0, h->s, src, d_out, a);
-        if (pb && !nn) hipLaunchKernelGGL((k_tau64<IPPI_BC_PERIODIC_BB, false>), grid, dim3(BX), 0, h->s, src, d_out, a);
-        if (!pb && nn) hipLaunchKernelGGL((k_tau64<IPPI_BC_INFLOW, true>), grid, dim3(BX), 0, h->s, src, d_out, a);
-        if (!pb && !nn) hipLaunchKernelGGL((k_tau64<IPPI_BC_INFLOW, false>), grid, dim3(BX), 0, h->s, src, d_out, a);
+    const bool nn = h->c.tau_mode == IPPI_TAU_NN;
+    const int bc = h->c.bc;
+#define IPPI_TAU_LAUNCH(K, T, B)                                                                          \
+    if (bc == B) {                                                                                        \
+        if (nn)                                                                                           \
+            hipLaunchKernelGGL((K<B, true>), grid, dim3(BX), 0, h->s, (const T*)h->buf[h->cur], d_out, a);  \
+        else                                                                                              \
+            hipLaunchKernelGGL((K<B, false>), grid, dim3(BX), 0, h->s, (const T*)h->buf[h->cur], d_out, a); \
     }
+    if (h->esz == 4) {
+        IPPI_TAU_LAUNCH(k_tau32, float, IPPI_BC_PERIODIC_BB)
+        IPPI_TAU_LAUNCH(k_tau32, float, IPPI_BC_INFLOW)
+        IPPI_TAU_LAUNCH(k_tau32, float, IPPI_BC_FLATPLATE)
+    } else {
+        IPPI_TAU_LAUNCH(k_tau64, double, IPPI_BC_PERIODIC_BB)
+        IPPI_TAU_LAUNCH(k_tau64, double, IPPI_BC_INFLOW)
+        IPPI_TAU_LAUNCH(k_tau64, double, IPPI_BC_FLATPLATE)
+    }
+#undef IPPI_TAU_LAUNCH
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(tau, d_out, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s);
     if (e == hipSuccess) e = hipStreamSynchronize(h->s);
@@ -2379,35 +2620,39 @@ struct SaveHdr {
 
 int ippi_save(ippi_handle* h, void* buf, size_t* size) {
     if (!h || !size) return IPPI_ERR_ARG;
-    const size_t need = sizeof(SaveHdr) + h->esz * 9 * h->N + sizeof(double) * 3 * h->N;
+    const size_t need = sizeof(SaveHdr) + h->esz * 9 * h->N + sizeof(double) * (3 * h->N + 18);
     if (!buf) {
         *size = need;
         return IPPI_OK;
     }
     if (*size < need) return fail(h, IPPI_ERR_ARG, "checkpoint buffer too small");
-    SaveHdr hd{0x4950504931ull, h->c.nx, h->c.ny, h->c.precision, h->phase, h->steps, h->nrec};
+    SaveHdr hd{0x4950504932ull, h->c.nx, h->c.ny, h->c.precision, h->phase, h->steps, h->nrec};
     char* p = (char*)buf;
     std::memcpy(p, &hd, sizeof(hd));
     HIPC(h, hipStreamSynchronize(h->s));
     HIPC(h, hipMemcpy(p + sizeof(hd), h->buf[h->cur], h->esz * 9 * h->N, hipMemcpyDeviceToHost));
     HIPC(h, hipMemcpy(p + sizeof(hd) + h->esz * 9 * h->N, h->mac, sizeof(double) * 3 * h->N, hipMemcpyDeviceToHost));
+    HIPC(h, hipMemcpy(p + sizeof(hd) + h->esz * 9 * h->N + sizeof(double) * 3 * h->N, h->corner, sizeof(double) * 18,
+                      hipMemcpyDeviceToHost));
     *size = need;
     return IPPI_OK;
 }
 
 int ippi_load(ippi_handle* h, const void* buf, size_t size) {
     if (!h || !buf) return IPPI_ERR_ARG;
-    const size_t need = sizeof(SaveHdr) + h->esz * 9 * h->N + sizeof(double) * 3 * h->N;
+    const size_t need = sizeof(SaveHdr) + h->esz * 9 * h->N + sizeof(double) * (3 * h->N + 18);
     if (size < need) return fail(h, IPPI_ERR_ARG, "checkpoint too small");
     SaveHdr hd;
     std::memcpy(&hd, buf, sizeof(hd));
-    if (hd.magic != 0x4950504931ull || hd.nx != h->c.nx || hd.ny != h->c.ny || hd.precision != h->c.precision)
+    if (hd.magic != 0x4950504932ull || hd.nx != h->c.nx || hd.ny != h->c.ny || hd.precision != h->c.precision)
         return fail(h, IPPI_ERR_ARG, "checkpoint does not match this configuration");
     const char* p = (const char*)buf;
     HIPC(h, hipStreamSynchronize(h->s));
     h->cur = 0;
     HIPC(h, hipMemcpy(h->buf[0], p + sizeof(hd), h->esz * 9 * h->N, hipMemcpyHostToDevice));
     HIPC(h, hipMemcpy(h->mac, p + sizeof(hd) + h->esz * 9 * h->N, sizeof(double) * 3 * h->N, hipMemcpyHostToDevice));
+    HIPC(h, hipMemcpy(h->corner, p + sizeof(hd) + h->esz * 9 * h->N + sizeof(double) * 3 * h->N, sizeof(double) * 18,
+                      hipMemcpyHostToDevice));
     HIPC(h, hipMemset(h->flag, 0, sizeof(int)));
     h->phase = hd.phase;
     h->steps = hd.steps;

@@ -1,14 +1,23 @@
-// Driver written like IPPI-LBM/main.cpp:3-229, on the Mesh facade: one GPU
-// holds the whole domain (RankID 0, CoreNum 1). Workload: a cylinder of
-// diameter 80 lu (251 markers) in a 2048 x 1024 channel, MRT + hybrid
-// Smagorinsky LES, Re = 1e4, inflow 0.08; prints cd/cl every SaveStep.
-//   usage: ippi_main [NumStep] [SaveStep]
+// Drivers written like IPPI-LBM/main.cpp:3-229, on the Mesh facade: one GPU
+// holds the whole domain (RankID 0, CoreNum 1).
+//   ippi_main [NumStep] [SaveStep]
+//       cylinder of diameter 80 lu (251 markers) in a 2048 x 1024 channel,
+//       MRT + hybrid Smagorinsky LES, Re = 1e4, inflow 0.08, FP32 shifted
+//       populations; appends cd/cl to particle/CD_CL.dat every SaveStep.
+//   ippi_main plate [NumStep] [SaveStep]
+//       the reference's own run (main.cpp:13-48): flat plate on 2401 x 501,
+//       BGK + hybrid Smagorinsky (Cs = 0.173), U = 0.1, Re = 5e5,
+//       tau = 0.5 + 6 Ny U / Re, FP64, the reference's boundary set
+//       (free slip ahead of x = 400, no slip beyond); Tecplot fields every
+//       SaveStep (WriteDatFluid + MergeDatFluid), running means from
+//       startAvg = 30000 (ComputeAvgVelocity).
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
 
 #include "mesh.h"
 
-int main(int argc, char* argv[]) {
+static int run_cylinder(int NumStep, int SaveStep) {
     int RankID = 0, CoreNum = 1;
     REAL Cs = 0.1;
     REAL Pr = 1;
@@ -21,8 +30,6 @@ int main(int argc, char* argv[]) {
     Point2D StartVel{0.08, 0.0};
     const REAL Rho = 1.0;
     const REAL tau = 0.5 + 3.0 * StartVel.x * (2 * Radius) / Re;
-    const int NumStep = argc > 1 ? std::atoi(argv[1]) : 2000;
-    const int SaveStep = argc > 2 ? std::atoi(argv[2]) : 500;
 
     Mesh mesh;
     mesh.startAvg = NumStep;
@@ -55,4 +62,60 @@ int main(int argc, char* argv[]) {
     }
     mesh.ErrorCatch(0, 0);
     return 0;
+}
+
+static int run_plate(int NumStep, int SaveStep) {
+    int RankID = 0, CoreNum = 1;
+    REAL Cs = 0.173;
+    REAL Pr = 1;
+    int Nx = 2401;
+    int Ny = 501;
+    int StepH = 0, StepL = 0;
+    const REAL Radius = 20;
+    Point2D CylinderCenter{400, 620};
+    double Re = 5e5;
+    Point2D StartVel{0.1, 0.0};
+    const REAL Rho = 1.0;
+    const REAL tau = 0.5 + 6.0 * ((Ny * StartVel.x) / Re);
+
+    Mesh mesh;
+    mesh.startAvg = 30000;
+    mesh.center = CylinderCenter;
+    mesh.radius = Radius;
+    mesh.startTau = tau;
+    mesh.startU = StartVel;
+    mesh.MyrankID = RankID;
+    mesh.MyCoreNum = CoreNum;
+    mesh.stepH = StepH;
+    mesh.stepL = StepL;
+    mesh.StartCs = Cs;
+    mesh.LatmeshSizeNx = Nx;
+    mesh.LatmeshSizeNy = Ny;
+    mesh.Collision = IPPI_COLL_BGK;
+    mesh.Precision = IPPI_FP64;
+    mesh.Boundary = IPPI_BC_FLATPLATE;
+    mesh.PlateX = 400;
+    mesh.InitialFile();
+    mesh.InitialLatMesh(Nx, Ny, tau, StartVel, Rho, RankID, CoreNum, 0);
+    mesh.JudgeStyle(CylinderCenter, Radius, RankID, CoreNum);
+    mesh.ParallelSize = Nx;
+    std::cout << "Initial Completed" << std::endl;
+    std::cout << "U = " << StartVel.x << " Re = " << Re << " Tau = " << tau << std::endl;
+    for (int t = 1; t <= NumStep; ++t) {
+        mesh.OneLBMStep(RankID, CoreNum, Cs, Pr, t);
+        if (t % SaveStep == 0) {
+            mesh.WriteDatFluid(t, RankID, Nx, CoreNum);
+            mesh.MergeDatFluid(t, CoreNum);
+            std::cout << "Compute Step: " << t << std::endl;
+        }
+        mesh.ComputeAvgVelocity(t);
+    }
+    mesh.ErrorCatch(0, 0);
+    return 0;
+}
+
+int main(int argc, char* argv[]) {
+    if (argc > 1 && std::strcmp(argv[1], "plate") == 0)
+        return run_plate(argc > 2 ? std::atoi(argv[2]) : 200000, argc > 3 ? std::atoi(argv[3]) : 20000);
+    return run_cylinder(argc > 1 ? std::atoi(argv[1]) : 2000, argc > 2 ? std::atoi(argv[2]) : 500);
 }

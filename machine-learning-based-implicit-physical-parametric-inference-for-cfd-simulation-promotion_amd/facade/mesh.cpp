@@ -60,6 +60,7 @@ void Mesh::EnsureHandle(REAL Cs) {
     c.cs = Cs;
     c.u_in_x = startU.x;
     c.u_in_y = startU.y;
+    c.plate_x = PlateX;
     Check(ippi_create(&c, &h_), "ippi_create");
     cs_ = Cs;
     Check(ippi_init_fields(h_, Seed, rho0_, u0_.x, u0_.y, NoiseAmplitude), "ippi_init_fields");

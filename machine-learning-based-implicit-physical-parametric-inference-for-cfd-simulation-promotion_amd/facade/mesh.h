@@ -37,8 +37,9 @@ public:
     // runtime equivalents of Parameter.h:8-16
     int Collision = IPPI_COLL_BGK;   // SRTLBM / MRT / RRLBM
     int Precision = IPPI_FP64;       // REAL = double, or FP32 shifted populations
-    int Boundary = IPPI_BC_INFLOW;   // inlet/outlet NEE + free-slip walls, or periodic + bounce-back
+    int Boundary = IPPI_BC_INFLOW;   // IPPI_BC_FLATPLATE (the reference's own set), inflow + free slip, periodic
     int UseLES = 1;                  // SmagorinskyByHybrid
+    int PlateX = 400;                // IPPI_BC_FLATPLATE: 'm' / 'd' split of the bottom wall (mesh.cpp:1413)
     uint64_t Seed = 0;
     double NoiseAmplitude = 0.0;
 

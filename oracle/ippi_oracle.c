@@ -629,8 +629,92 @@ static void pull_node(const or_sim* s, int x, int y, double f[9]) {
     }
 }
 
+/* fnew and macros of node nb by non-equilibrium extrapolation from node nn
+ * with boundary density rb and velocity (uxb, uyb): f = feq_b + f_n - feq_n
+ * (the form of mesh.cpp:245-250, 279-284, 311-318, 343-349). */
+static void nee_node(or_sim* s, int nb, int nn, double rb, double uxb, double uyb) {
+    const size_t N = (size_t)s->N;
+    double fb[9], fq[9];
+    or_feq(rb, uxb, uyb, fb);
+    or_feq(s->rho[nn], s->ux[nn], s->uy[nn], fq);
+    for (int k = 0; k < 9; k++) s->fnew[k * N + nb] = fb[k] + s->fnew[k * N + nn] - fq[k];
+    s->rho[nb] = rb;
+    s->ux[nb] = uxb;
+    s->uy[nb] = uyb;
+}
+
+/* Flat plate: the reference's own boundary set for one domain
+ * (Mesh::BoundaryCondition, mesh.cpp:228-356, with the styles of
+ * Mesh::JudgeStyle, mesh.cpp:1391-1454); x is the reference's global
+ * column pos.x (mesh.cpp:105). Pull streaming on the 'f' nodes
+ * 1 <= x <= nx-2, 1 <= y <= ny-2 (mesh.cpp:192-226), then in the
+ * reference's order:
+ *   inlet  x = 0,    1 <= y <= ny-2: rho = 1, u = u_in, NEE from (1, y)
+ *          (mesh.cpp:239-253; the reference writes u = (0.1, 0))
+ *   outlet x = nx-1, 1 <= y <= ny-2: rho = 1, u = u(nx-2, y), NEE from (nx-2, y)
+ *          (mesh.cpp:273-287)
+ *   top    y = ny-1, 0 <= x <= nx-1: rho = 1, u = u(x, ny-2), NEE from (x, ny-2)
+ *          (mesh.cpp:305-319; at x = 0 and nx-1 the source is the inlet /
+ *          outlet node just set)
+ *   bottom y = 0,    1 <= x <= nx-2: x < plate_x: specular 'm' rule
+ *          (mesh.cpp:325-337); else 'd': rho = rho(x, 1), u = 0, NEE from
+ *          (x, 1) (mesh.cpp:338-350)
+ * The corners (0, 0) and (nx-1, 0) carry the inlet / outlet style, which
+ * no rule of the bottom loop handles: their fnew and macros never change
+ * (they are still collided every step). */
+static void stream_flatplate(or_sim* s) {
+    const int nx = s->c.nx, ny = s->c.ny;
+    const size_t N = (size_t)s->N;
+#pragma omp parallel for schedule(static) if (N > 4096) num_threads(g_threads > 0 ? g_threads : 1)
+    for (int y = 1; y < ny - 1; y++)
+        for (int x = 1; x < nx - 1; x++) {
+            double f[9], m[3];
+            for (int k = 0; k < 9; k++) f[k] = fc(s, k, x - EX[k], y - EY[k]);
+            or_macro(f, m);
+            const int n = y * nx + x;
+            for (int k = 0; k < 9; k++) s->fnew[k * N + n] = f[k];
+            s->rho[n] = m[0];
+            s->ux[n] = m[1];
+            s->uy[n] = m[2];
+        }
+    for (int y = 1; y < ny - 1; y++) nee_node(s, y * nx, y * nx + 1, 1.0, s->c.u_in_x, s->c.u_in_y);
+    for (int y = 1; y < ny - 1; y++) {
+        const int nn = y * nx + nx - 2;
+        nee_node(s, y * nx + nx - 1, nn, 1.0, s->ux[nn], s->uy[nn]);
+    }
+    for (int x = 0; x < nx; x++) {
+        const int nn = (ny - 2) * nx + x;
+        nee_node(s, (ny - 1) * nx + x, nn, 1.0, s->ux[nn], s->uy[nn]);
+    }
+    for (int x = 1; x < nx - 1; x++) {
+        if (x < s->c.plate_x) {
+            double f[9], m[3];
+            f[0] = fc(s, 0, x, 0);
+            f[1] = fc(s, 1, x - 1, 0);
+            f[2] = fc(s, 4, x, 1);
+            f[3] = fc(s, 3, x + 1, 0);
+            f[4] = fc(s, 4, x, 1);
+            f[5] = fc(s, 8, x - 1, 1);
+            f[6] = fc(s, 7, x + 1, 1);
+            f[7] = fc(s, 7, x + 1, 1);
+            f[8] = fc(s, 8, x - 1, 1);
+            or_macro(f, m);
+            for (int k = 0; k < 9; k++) s->fnew[k * N + x] = f[k];
+            s->rho[x] = m[0];
+            s->ux[x] = m[1];
+            s->uy[x] = m[2];
+        } else {
+            nee_node(s, x, nx + x, s->rho[nx + x], 0.0, 0.0);
+        }
+    }
+}
+
 static void stream_phase(or_sim* s) {
     const int nx = s->c.nx, ny = s->c.ny, N = s->N;
+    if (s->c.bc == OR_BC_FLATPLATE) {
+        stream_flatplate(s);
+        return;
+    }
     const int inflow = s->c.bc == OR_BC_INFLOW;
 #pragma omp parallel for schedule(static) if (N > 4096) num_threads(g_threads > 0 ? g_threads : 1)
     for (int y = 0; y < ny; y++) {
@@ -746,7 +830,7 @@ static double dphi(double x) { /* derivative of or_delta */
 int or_adjoint_drag(int nx, int ny, uint64_t seed, double ux0, double uy0, double noise, const double* tau,
                     double cx, double cy, double diam, int M, int nsteps, int avg_from, double* J, double* g_tau,
                     double* g_c) {
-    or_config c = {nx, ny, OR_COLL_BGK, OR_BC_PERIODIC_BB, 0, OR_TAU_FIELD, 0.6, 0.0, 0.0, 0.0, 0.0};
+    or_config c = {nx, ny, OR_COLL_BGK, OR_BC_PERIODIC_BB, 0, OR_TAU_FIELD, 0.6, 0.0, 0.0, 0.0, 0.0, 0};
     or_sim* s = or_create(&c);
     or_init_fields(s, seed, 1.0, ux0, uy0, noise);
     or_set_tau_field(s, tau);
@@ -911,7 +995,7 @@ int or_adjoint_drag(int nx, int ny, uint64_t seed, double ux0, double uy0, doubl
 double or_drag_objective(int nx, int ny, uint64_t seed, double ux0, double uy0, double noise, const double* tau,
                          double cx, double cy, double diam, int M, int nsteps, int avg_from) {
     double J = 0.0;
-    or_config c = {nx, ny, OR_COLL_BGK, OR_BC_PERIODIC_BB, 0, OR_TAU_FIELD, 0.6, 0.0, 0.0, 0.0, 0.0};
+    or_config c = {nx, ny, OR_COLL_BGK, OR_BC_PERIODIC_BB, 0, OR_TAU_FIELD, 0.6, 0.0, 0.0, 0.0, 0.0, 0};
     or_sim* s = or_create(&c);
     or_init_fields(s, seed, 1.0, ux0, uy0, noise);
     or_set_tau_field(s, tau);

@@ -25,7 +25,7 @@ extern "C" {
 #endif
 
 enum { OR_COLL_BGK = 0, OR_COLL_MRT = 1, OR_COLL_RR = 2 };
-enum { OR_BC_PERIODIC_BB = 0, OR_BC_INFLOW = 1 };
+enum { OR_BC_PERIODIC_BB = 0, OR_BC_INFLOW = 1, OR_BC_FLATPLATE = 2 };
 enum { OR_TAU_CONST = 0, OR_TAU_FIELD = 1, OR_TAU_NN = 2 };
 
 typedef struct {
@@ -39,6 +39,7 @@ typedef struct {
     double u_in_x;   /* inlet velocity for OR_BC_INFLOW */
     double u_in_y;
     double tau_nn_max;  /* network values above this fall back too (0 = none) */
+    int plate_x;        /* OR_BC_FLATPLATE: bottom free-slip for x < plate_x, no-slip beyond */
 } or_config;
 
 /* ---- per-node kernels (NodeAndLattice.cpp, mesh.cpp) ---- */

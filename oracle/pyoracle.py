@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "libippi_oracle.so")
 REF_LIB_PATH = os.path.join(_HERE, "_ref", "libippi_refnode.so")
 
 COLL = {"bgk": 0, "mrt": 1, "rr": 2}
-BC = {"periodic_bb": 0, "inflow": 1}
+BC = {"periodic_bb": 0, "inflow": 1, "flatplate": 2}
 TAU = {"const": 0, "field": 1, "nn": 2}
 MLP_NPARAM = 6 * 32 + 32 + 32 * 32 + 32 + 32 + 1
 MLP_BLOB = MLP_NPARAM + 12
@@ -26,7 +26,7 @@ class OrConfig(ctypes.Structure):
         ("bc", ctypes.c_int), ("les", ctypes.c_int), ("tau_mode", ctypes.c_int),
         ("tau0", ctypes.c_double), ("cs", ctypes.c_double),
         ("u_in_x", ctypes.c_double), ("u_in_y", ctypes.c_double),
-        ("tau_nn_max", ctypes.c_double),
+        ("tau_nn_max", ctypes.c_double), ("plate_x", ctypes.c_int),
     ]
 
 
@@ -168,11 +168,11 @@ class Sim:
     """FP64 CPU solver; same configuration vocabulary as the HIP library."""
 
     def __init__(self, nx, ny, collision="bgk", bc="periodic_bb", les=False, tau_mode="const",
-                 tau0=0.6, cs=0.0, u_in=(0.0, 0.0), threads=1, tau_nn_max=0.0):
+                 tau0=0.6, cs=0.0, u_in=(0.0, 0.0), threads=1, tau_nn_max=0.0, plate_x=0):
         self.nx, self.ny = nx, ny
         self.N = nx * ny
         cfg = OrConfig(nx, ny, COLL[collision], BC[bc], int(bool(les)), TAU[tau_mode],
-                       tau0, cs, u_in[0], u_in[1], tau_nn_max)
+                       tau0, cs, u_in[0], u_in[1], tau_nn_max, int(plate_x))
         lib().or_set_threads(int(threads))
         self.h = lib().or_create(ctypes.byref(cfg))
 

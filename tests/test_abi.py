@@ -3,6 +3,7 @@
 import ctypes
 import os
 import re
+import subprocess
 
 from tests import _pkg
 
@@ -31,8 +32,18 @@ def test_binding_covers_header():
     assert set(declared()) == set(ippi._SIGS)
 
 
-def test_config_struct_layout():
+def test_config_struct_layout(tmp_path):
+    """The ctypes mirror of struct ippi_config has the C layout: sizes and
+    field offsets from a C program compiled against the header."""
     ippi = _pkg.load()
-    # 7 ints (28 B, padded to 32), 4 doubles, 2 ints, 1 double
-    assert ctypes.sizeof(ippi.Config) == 80
+    names = [f[0] for f in ippi.Config._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"ippi_lbm.h\"\nint main(void) {\n"
+                   "  printf(\"%zu\\n\", sizeof(ippi_config));\n" +
+                   "".join(f"  printf(\"%zu\\n\", offsetof(ippi_config, {n}));\n" for n in names) + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(_pkg.ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(ippi.Config)
+    assert out[1:] == [getattr(ippi.Config, n).offset for n in names]
     assert ippi.library().ippi_version() == 1

@@ -1,0 +1,41 @@
+"""The reference's own run driven through the Mesh facade (ippi_main plate,
+written like IPPI-LBM/main.cpp): the Tecplot field file that WriteDatFluid
+and MergeDatFluid produce (layout of mesh.cpp:1180-1243, 1320-1353) holds
+the fields of the binding's solver on the same configuration."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import _pkg
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(_pkg.PKG_DIR, "facade", "ippi_main")
+
+
+def test_facade_flatplate_tecplot(tmp_path):
+    assert os.path.exists(BIN), "facade driver not built (run __graft_entry__.build())"
+    print("[facade] ippi_main plate 200 200", flush=True)
+    r = subprocess.run([BIN, "plate", "200", "200"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    fn = tmp_path / "fluid" / "fluid_Merged_200.dat"
+    with open(fn) as f:
+        head = [next(f) for _ in range(3)]
+    assert head[1].startswith('VARIABLES = "X" "Y" "Density" "U" "V"')
+    assert "I=501, J=2401" in head[2]
+    d = np.loadtxt(fn, skiprows=3)
+    assert d.shape == (2401 * 501, 9)
+    ippi = _pkg.load()
+    tau = 0.5 + 6.0 * (501 * 0.1) / 5e5
+    with ippi.Solver(2401, 501, "bgk", "fp64", "flatplate", les=True, tau0=tau, cs=0.173, u_in=(0.1, 0.0),
+                     plate_x=400) as s:
+        s.init_fields(0, 1.0, (0.1, 0.0), 0.0)
+        s.step(200)
+        rho, ux, uy = s.fields()
+    # rows are x-major with y fastest (I = Ny)
+    X, Y = d[:, 0].astype(int), d[:, 1].astype(int)
+    assert np.array_equal(X, np.repeat(np.arange(2401), 501)) and np.array_equal(Y, np.tile(np.arange(501), 2401))
+    np.testing.assert_allclose(d[:, 2], rho[Y, X], rtol=1e-5)
+    np.testing.assert_allclose(d[:, 3], ux[Y, X], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(d[:, 4], uy[Y, X], rtol=1e-5, atol=1e-9)

@@ -44,7 +44,12 @@ CASES = {
     "bgk_inflow": (96, 48, "bgk", "inflow", False, "const", 0.7, 0.0, (0.05, 0.0), None, 300),
     "rr_les_inflow_cyl": (128, 64, "rr", "inflow", True, "const", 0.53, 0.1, (0.07, 0.0), (40, 32, 16, 51), 300),
     "bgk_les_channel_cyl": (128, 64, "bgk", "periodic_bb", True, "const", 0.51, 0.173, (0.08, 0.0), (40, 32, 20, 63), 300),
+    # the reference's own boundary set (NEE inlet/outlet/top, free-slip then no-slip bottom)
+    "bgk_les_flatplate": (160, 48, "bgk", "flatplate", True, "const", 0.52, 0.173, (0.1, 0.0), None, 400),
+    "mrt_les_flatplate_cyl": (160, 64, "mrt", "flatplate", True, "const", 0.53, 0.173, (0.08, 0.0), (60, 32, 16, 51), 300),
+    "rr_flatplate": (128, 48, "rr", "flatplate", False, "const", 0.56, 0.0, (0.06, 0.0), None, 300),
 }
+PLATE_X = {"bgk_les_flatplate": 40, "mrt_les_flatplate_cyl": 30, "rr_flatplate": 20}
 
 
 def tau_field(nx, ny, seed=7):
@@ -54,8 +59,9 @@ def tau_field(nx, ny, seed=7):
 
 def run_pair(oracle, ippi, case, precision):
     nx, ny, coll, bc, les, tm, tau0, cs, u0, body, steps = CASES[case]
-    o = oracle.Sim(nx, ny, coll, bc, les=les, tau_mode=tm, tau0=tau0, cs=cs, u_in=u0, threads=4)
-    g = ippi.Solver(nx, ny, coll, precision, bc, les=les, tau_mode=tm, tau0=tau0, cs=cs, u_in=u0)
+    px = PLATE_X.get(case, 0)
+    o = oracle.Sim(nx, ny, coll, bc, les=les, tau_mode=tm, tau0=tau0, cs=cs, u_in=u0, threads=4, plate_x=px)
+    g = ippi.Solver(nx, ny, coll, precision, bc, les=les, tau_mode=tm, tau0=tau0, cs=cs, u_in=u0, plate_x=px)
     o.init_fields(42, 1.0, u0, 1e-3)
     g.init_fields(42, 1.0, u0, 1e-3)
     if tm == "field":
@@ -71,6 +77,17 @@ def run_pair(oracle, ippi, case, precision):
     return o, g, steps
 
 
+def next_tau(oracle, o, case):
+    """tau the next collision uses, from the oracle's streamed state."""
+    _, _, _, _, les, _, tau0, cs, _, _, _ = CASES[case]
+    f = o.fnew()
+    r, ux, uy = o.fields()
+    if not les:
+        return np.full(r.shape, tau0)
+    return np.array([[oracle.les_tau(f[:, j, i], r[j, i], ux[j, i], uy[j, i], tau0, cs)
+                      for i in range(r.shape[1])] for j in range(r.shape[0])])
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_fp64_matches_oracle(oracle, ippi, case):
     o, g, steps = run_pair(oracle, ippi, case, "fp64")
@@ -82,6 +99,8 @@ def test_fp64_matches_oracle(oracle, ippi, case):
     assert rel_l2(f1, f0) <= FP64_TOL
     ndiff = int(np.count_nonzero(f0 != f1))
     print(f"{case}: populations differing bitwise: {ndiff} of {f0.size}")
+    if CASES[case][3] == "flatplate":  # relaxation time of the next collision, corners included
+        assert rel_l2(g.tau(), next_tau(oracle, o, case)) <= FP64_TOL
     if CASES[case][9] is not None:
         d0, d1 = o.forces(), g.body_forces()
         assert d1.shape == d0.shape == (steps, 1, 2)

@@ -157,3 +157,83 @@ def test_shear_wave_decay(oracle, coll):
     expect = U * math.exp(-nu * (math.pi / ny) ** 2 * steps)
     assert abs(amp / expect - 1) < 5e-3
     s.close()
+
+
+def _flatplate(oracle, nx=40, ny=12, plate_x=15, steps=7, u_in=0.06, noise=1e-3, coll="bgk"):
+    s = oracle.Sim(nx, ny, coll, "flatplate", les=True, tau0=0.56, cs=0.173, u_in=(u_in, 0.0), plate_x=plate_x)
+    s.init_fields(4, 1.0, (u_in, 0.0), noise)
+    return s
+
+
+def test_flatplate_uniform_inflow_is_steady(oracle):
+    """No plate (plate_x = nx) and no noise: the uniform inflow is a fixed
+    point of every rule (NEE inlet/outlet/top, specular bottom, untouched
+    corners that hold feq of the same flow)."""
+    nx, ny = 30, 10
+    s = _flatplate(oracle, nx, ny, plate_x=nx, noise=0.0)
+    s.step(40)
+    rho, ux, uy = s.fields()
+    assert np.abs(rho - 1).max() < 1e-14
+    assert np.abs(ux - 0.06).max() < 1e-14 and np.abs(uy).max() < 1e-15
+    s.close()
+
+
+def test_flatplate_rules(oracle):
+    """Each boundary rule of the reference's flat plate (mesh.cpp:228-356)
+    checked on the oracle's state after a few steps of a noisy flow."""
+    nx, ny, px, uin = 40, 12, 15, 0.06
+    s = _flatplate(oracle, nx, ny, plate_x=px)
+    s.step(1)
+    f0, (r0, x0, y0) = s.fnew(), s.fields()
+    corners = [(0, 0), (0, nx - 1)]
+    c0 = {c: (f0[:, c[0], c[1]].copy(), r0[c], x0[c], y0[c]) for c in corners}
+    s.step(6)
+    f, fc = s.fnew(), s.fcol()
+    rho, ux, uy = s.fields()
+
+    def nee_ok(yb, xb, yn, xn, rb, ubx, uby):
+        fb = oracle.feq(rb, ubx, uby)
+        fq = oracle.feq(rho[yn, xn], ux[yn, xn], uy[yn, xn])
+        assert np.allclose(f[:, yb, xb], fb + f[:, yn, xn] - fq, rtol=0, atol=1e-16), (yb, xb)
+        assert rho[yb, xb] == rb and ux[yb, xb] == ubx and uy[yb, xb] == uby, (yb, xb)
+
+    for y in range(1, ny - 1):
+        nee_ok(y, 0, y, 1, 1.0, uin, 0.0)                               # inlet
+        nee_ok(y, nx - 1, y, nx - 2, 1.0, ux[y, nx - 2], uy[y, nx - 2])  # outlet
+    for x in range(nx):
+        nee_ok(ny - 1, x, ny - 2, x, 1.0, ux[ny - 2, x], uy[ny - 2, x])  # top
+    for x in range(px, nx - 1):
+        nee_ok(0, x, 1, x, rho[1, x], 0.0, 0.0)                         # no-slip plate ('d')
+    # specular free slip ('m') from the post-collision populations
+    for x in range(1, px):
+        exp = [fc[0, 0, x], fc[1, 0, x - 1], fc[4, 1, x], fc[3, 0, x + 1], fc[4, 1, x],
+               fc[8, 1, x - 1], fc[7, 1, x + 1], fc[7, 1, x + 1], fc[8, 1, x - 1]]
+        assert np.array_equal(f[:, 0, x], np.array(exp)), x
+        assert abs(uy[0, x]) < 1e-16
+    # interior pull is a plain shift of the post-collision populations
+    for k in range(9):
+        exp = fc[k, 1 - EY[k]:ny - 1 - EY[k], 1 - EX[k]:nx - 1 - EX[k]]
+        assert np.array_equal(f[k, 1:ny - 1, 1:nx - 1], exp), k
+    # the two bottom corners keep their state
+    for c in corners:
+        fc0, rc, xc, yc = c0[c]
+        assert np.array_equal(f[:, c[0], c[1]], fc0)
+        assert rho[c] == rc and ux[c] == xc and uy[c] == yc
+    s.close()
+
+
+def test_flatplate_boundary_layer_grows(oracle):
+    """Physics: far upstream of the leading edge the free-slip wall leaves
+    the inflow nearly undisturbed (the plate's upstream influence decays);
+    behind it the no-slip plate slows the first row of fluid, more so
+    further downstream (boundary-layer growth)."""
+    nx, ny, px = 120, 24, 30
+    s = oracle.Sim(nx, ny, "bgk", "flatplate", tau0=0.6, u_in=(0.05, 0.0), plate_x=px)
+    s.init_fields(1, 1.0, (0.05, 0.0), 0.0)
+    assert s.step(600) == 0
+    _, ux, _ = s.fields()
+    assert np.abs(ux[1, 3:9] - 0.05).max() < 1e-3
+    assert np.all(np.diff(ux[1, 3:px]) < 0)              # slowing towards the leading edge
+    assert ux[1, px + 20] < 0.04 and ux[1, px + 60] < ux[1, px + 20] + 1e-3
+    assert np.all(ux[0, px:nx - 1] == 0.0)
+    s.close()
