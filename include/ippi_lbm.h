@@ -28,6 +28,8 @@
  *                      (+ InterpolateVel / CalParticleForce / SpreadForce /
  *                       UpdateVelocity, mesh.cpp:514-832,1244-1281)
  *   ippi_get_fields    reads of LatMesh[i][j].rho/u  (WriteDatFluid, mesh.cpp:1180-1243)
+ *   ippi_accumulate_mean / ippi_get_mean
+ *                      Mesh::ComputeAvgVelocity      mesh.cpp:1954-1974 (avg_u/avg_v/avg_density)
  *   ippi_get_body_forces  OutputCd                   mesh.cpp:1461-1526
  *   ippi_mass          ErrorCatch / global checks    mesh.cpp:2101-2111
  *   ippi_destroy       Mesh::~Mesh                   mesh.cpp:52-55
@@ -182,6 +184,13 @@ int ippi_trainer_adam(ippi_trainer* t, int iters, int batch, double lr, uint64_t
 int ippi_set_timing(ippi_handle* h, int enable);
 int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, double* ib_ms,
                     int64_t* ib_launches);
+
+/* Running means of rho, ux, uy (Mesh::ComputeAvgVelocity, mesh.cpp:1954-1974):
+ * updates mean = (mean * k + current) / (k + 1) at every node on the device
+ * from the current state (k = t - startAvg in the reference; k = 0 starts
+ * a new mean). ippi_get_mean synchronises and copies the three fields. */
+int ippi_accumulate_mean(ippi_handle* h, int64_t k);
+int ippi_get_mean(ippi_handle* h, double* rho, double* ux, double* uy);
 
 /* Discrete adjoint of the time-averaged body force (no counterpart in the
  * reference, which has no gradient code; it differentiates the step that

@@ -88,6 +88,8 @@ _SIGS = {
     "ippi_adjoint_drag": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D],
                           ctypes.c_int),
     "ippi_get_adjoint_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "ippi_accumulate_mean": ([ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "ippi_get_mean": ([ctypes.c_void_p, _D, _D, _D], ctypes.c_int),
     "ippi_save": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "ippi_load": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
 }
@@ -219,6 +221,16 @@ class Solver:
     def fields(self):
         r, ux, uy = (np.empty(self.N) for _ in range(3))
         self._check(library().ippi_get_fields(self.h, _p(r), _p(ux), _p(uy)))
+        s = (self.ny, self.nx)
+        return r.reshape(s), ux.reshape(s), uy.reshape(s)
+
+    def accumulate_mean(self, k):
+        """mean = (mean * k + current) / (k + 1) on the device (ComputeAvgVelocity)."""
+        self._check(library().ippi_accumulate_mean(self.h, int(k)))
+
+    def mean(self):
+        r, ux, uy = (np.empty(self.N) for _ in range(3))
+        self._check(library().ippi_get_mean(self.h, _p(r), _p(ux), _p(uy)))
         s = (self.ny, self.nx)
         return r.reshape(s), ux.reshape(s), uy.reshape(s)
 

@@ -942,6 +942,14 @@ __global__ void k_init(T* __restrict__ dst, double* __restrict__ mac, size_t N, 
 // Post-streaming populations and moments of the current state (reads only);
 // f_out may be null. FP32 moments are accumulated in FP64 from the stored
 // deviations.
+// Running means from the macro arrays (state before the first step).
+__global__ void k_mean_phase0(const double* __restrict__ mac, double* __restrict__ mean, size_t N, double kk) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+#pragma unroll
+    for (int c = 0; c < 3; c++) mean[c * N + i] = (mean[c * N + i] * kk + mac[c * N + i]) / (kk + 1.0);
+}
+
 // Saves the populations of the flat-plate corners (0, 0) and (nx-1, 0)
 // from the stored initial state (full f, FP64): no rule updates them, so
 // they are the populations these nodes keep.
@@ -957,7 +965,8 @@ __global__ void k_corner(const T* __restrict__ src, double* __restrict__ corner,
 
 template <typename T, int BC>
 __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, double* __restrict__ f_out, int nx,
-                        int ny, double uinx, double uiny, int plate_x, const double* __restrict__ corner) {
+                        int ny, double uinx, double uiny, int plate_x, const double* __restrict__ corner,
+                        double* __restrict__ mean, double kk) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * BX + threadIdx.x;
     const int y = blockIdx.y;
@@ -970,6 +979,10 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
             if (f_out) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) f_out[(size_t)k * N + n] = corner[9 * (x != 0) + k];
+            }
+            if (mean) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) mean[c * N + n] = (mean[c * N + n] * kk + mac[c * N + n]) / (kk + 1.0);
             }
             return;
         }
@@ -1015,6 +1028,11 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
     mac[n] = rho;
     mac[N + n] = ux;
     mac[2 * N + n] = uy;
+    if (mean) {  // running means, ComputeAvgVelocity (mesh.cpp:1954-1974)
+        mean[n] = (mean[n] * kk + rho) / (kk + 1.0);
+        mean[N + n] = (mean[N + n] * kk + ux) / (kk + 1.0);
+        mean[2 * N + n] = (mean[2 * N + n] * kk + uy) / (kk + 1.0);
+    }
     if (f_out) {
 #pragma unroll
         for (int k = 0; k < 9; k++) f_out[(size_t)k * N + n] = f[k];
@@ -1547,6 +1565,7 @@ struct ippi_handle {
     int64_t steps = 0;
     double* mac = nullptr;  // rho, ux, uy (3N)
     double* corner = nullptr;  // IPPI_BC_FLATPLATE corner populations [2][9]
+    double* mean = nullptr;    // running means of rho, ux, uy (3N), allocated on first use
     void* tau = nullptr;
     int* flag = nullptr;
     double* part = nullptr;
@@ -1877,7 +1896,13 @@ int run_steps(ippi_handle* h, int nsteps) {
 }
 
 template <typename T>
-int run_macro(ippi_handle* h, double* f_out) {
+int run_macro(ippi_handle* h, double* f_out, double* mean = nullptr, double kk = 0.0) {
+    if (h->phase == 0 && mean) {  // before the first step: the stored initial state
+        hipLaunchKernelGGL(k_mean_phase0, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0, h->s, h->mac, mean,
+                           h->N, kk);
+        HIPC(h, hipGetLastError());
+        return IPPI_OK;
+    }
     if (h->phase == 0) {
         if (f_out) {  // fnew state stored directly
             std::vector<T> tmp(9 * h->N);
@@ -1897,13 +1922,13 @@ int run_macro(ippi_handle* h, double* f_out) {
     const double ux = h->c.u_in_x, uy = h->c.u_in_y;
     if (h->c.bc == IPPI_BC_PERIODIC_BB)
         hipLaunchKernelGGL((k_macro<T, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
-                           px, h->corner);
+                           px, h->corner, mean, kk);
     else if (h->c.bc == IPPI_BC_INFLOW)
         hipLaunchKernelGGL((k_macro<T, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
-                           px, h->corner);
+                           px, h->corner, mean, kk);
     else
         hipLaunchKernelGGL((k_macro<T, IPPI_BC_FLATPLATE>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
-                           px, h->corner);
+                           px, h->corner, mean, kk);
     HIPC(h, hipGetLastError());
     if (f_out) {
         HIPC(h, hipMemcpyAsync(f_out, d_f, sizeof(double) * 9 * h->N, hipMemcpyDeviceToHost, h->s));
@@ -2159,7 +2184,7 @@ int ippi_destroy(ippi_handle* h) {
     clear_events(h->ev_step);
     clear_events(h->ev_ib);
     clear_events(h->ev_adj);
-    dfree(h->buf[0]); dfree(h->buf[1]); dfree(h->mac); dfree(h->corner); dfree(h->tau); dfree(h->flag); dfree(h->part);
+    dfree(h->buf[0]); dfree(h->buf[1]); dfree(h->mac); dfree(h->corner); dfree(h->mean); dfree(h->tau); dfree(h->flag); dfree(h->part);
     free_ib(h);
     dfree(h->d_frag);
     dfree(h->d_blob);
@@ -2508,6 +2533,29 @@ int ippi_get_fields(ippi_handle* h, double* rho, double* ux, double* uy) {
     if (rho) HIPC(h, hipMemcpyAsync(rho, h->mac, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s));
     if (ux) HIPC(h, hipMemcpyAsync(ux, h->mac + h->N, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s));
     if (uy) HIPC(h, hipMemcpyAsync(uy, h->mac + 2 * h->N, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s));
+    HIPC(h, hipStreamSynchronize(h->s));
+    return check_flag(h);
+}
+
+int ippi_accumulate_mean(ippi_handle* h, int64_t k) {
+    if (!h || k < 0) return IPPI_ERR_ARG;
+    if (!h->mean) {
+        HIPC(h, hipMalloc(&h->mean, sizeof(double) * 3 * h->N));
+        HIPC(h, hipMemsetAsync(h->mean, 0, sizeof(double) * 3 * h->N, h->s));
+    }
+    int r = h->esz == 8 ? run_macro<double>(h, nullptr, h->mean, (double)k)
+                        : run_macro<float>(h, nullptr, h->mean, (double)k);
+    if (r) return r;
+    return IPPI_OK;
+}
+
+int ippi_get_mean(ippi_handle* h, double* rho, double* ux, double* uy) {
+    if (!h || !rho || !ux || !uy) return IPPI_ERR_ARG;
+    if (!h->mean) return fail(h, IPPI_ERR_STATE, "no running mean accumulated");
+    const size_t B = sizeof(double) * h->N;
+    HIPC(h, hipMemcpyAsync(rho, h->mean, B, hipMemcpyDeviceToHost, h->s));
+    HIPC(h, hipMemcpyAsync(ux, h->mean + h->N, B, hipMemcpyDeviceToHost, h->s));
+    HIPC(h, hipMemcpyAsync(uy, h->mean + 2 * h->N, B, hipMemcpyDeviceToHost, h->s));
     HIPC(h, hipStreamSynchronize(h->s));
     return check_flag(h);
 }

@@ -4,6 +4,8 @@
 //       cylinder of diameter 80 lu (251 markers) in a 2048 x 1024 channel,
 //       MRT + hybrid Smagorinsky LES, Re = 1e4, inflow 0.08, FP32 shifted
 //       populations; appends cd/cl to particle/CD_CL.dat every SaveStep.
+//   ippi_main check
+//       every writer on a small case (see run_check).
 //   ippi_main plate [NumStep] [SaveStep]
 //       the reference's own run (main.cpp:13-48): flat plate on 2401 x 501,
 //       BGK + hybrid Smagorinsky (Cs = 0.173), U = 0.1, Re = 5e5,
@@ -114,7 +116,44 @@ static int run_plate(int NumStep, int SaveStep) {
     return 0;
 }
 
+// Every writer of the facade on a small case (128 x 64, cylinder D = 16,
+// BGK, periodic x / bounce-back walls, FP64): running means from step 20,
+// then at step 50 WriteDatFluid + MergeDatFluid, WriteAvgu + MergeAvgu,
+// WriteDatParticle and OutputCd.
+static int run_check() {
+    int RankID = 0, CoreNum = 1;
+    const int Nx = 128, Ny = 64, NumStep = 50;
+    Point2D StartVel{0.05, 0.0};
+    Mesh mesh;
+    mesh.startAvg = 20;
+    mesh.startTau = 0.6;
+    mesh.startU = StartVel;
+    mesh.Collision = IPPI_COLL_BGK;
+    mesh.Precision = IPPI_FP64;
+    mesh.Boundary = IPPI_BC_PERIODIC_BB;
+    mesh.UseLES = 0;
+    mesh.Seed = 7;
+    mesh.NoiseAmplitude = 1e-3;
+    mesh.InitialFile();
+    mesh.InitialLatMesh(Nx, Ny, 0.6, StartVel, 1.0, RankID, CoreNum, 0);
+    mesh.InitialIBParticle(51, 8.0, 40.0, 32.0, Point2D{0.0, 0.0});
+    for (int t = 1; t <= NumStep; ++t) {
+        mesh.OneLBMStep(RankID, CoreNum, 0.0, 1.0, t);
+        mesh.ComputeAvgVelocity(t);
+    }
+    mesh.WriteDatFluid(NumStep, RankID, Nx, CoreNum);
+    mesh.MergeDatFluid(NumStep, CoreNum);
+    mesh.WriteAvgu(NumStep, RankID, Nx, CoreNum);
+    mesh.MergeAvgu(NumStep, CoreNum);
+    mesh.WriteDatParticle(NumStep);
+    mesh.OutputCd(RankID, CoreNum, NumStep, 8.0);
+    mesh.ErrorCatch(0, 0);
+    std::cout << "Check Completed" << std::endl;
+    return 0;
+}
+
 int main(int argc, char* argv[]) {
+    if (argc > 1 && std::strcmp(argv[1], "check") == 0) return run_check();
     if (argc > 1 && std::strcmp(argv[1], "plate") == 0)
         return run_plate(argc > 2 ? std::atoi(argv[2]) : 200000, argc > 3 ? std::atoi(argv[3]) : 20000);
     return run_cylinder(argc > 1 ? std::atoi(argv[1]) : 2000, argc > 2 ? std::atoi(argv[2]) : 500);

@@ -112,31 +112,34 @@ void Mesh::InitialFile() {
     mkdir("cp", 0755);
 }
 
-// ComputeAvgVelocity, mesh.cpp:1954-1974 (running means from t = startAvg)
+// ComputeAvgVelocity, mesh.cpp:1954-1974: running means from t = startAvg,
+// avg = (avg (t - startAvg) + x) / (t - startAvg + 1), kept on the device.
 void Mesh::ComputeAvgVelocity(int t) {
     if (t < startAvg || !h_) return;
-    std::vector<double> r, u, v;
-    Check(Fields(r, u, v), "ippi_get_fields");
-    const size_t N = r.size();
-    if (avg_u_.size() != N) {
-        avg_u_.assign(N, 0.0);
-        avg_v_.assign(N, 0.0);
-        avg_rho_.assign(N, 0.0);
-    }
-    for (size_t n = 0; n < N; n++) {
-        avg_u_[n] = (avg_u_[n] * (t - startAvg) + u[n]) / (t - startAvg + 1);
-        avg_v_[n] = (avg_v_[n] * (t - startAvg) + v[n]) / (t - startAvg + 1);
-        avg_rho_[n] = (avg_rho_[n] * (t - startAvg) + r[n]) / (t - startAvg + 1);
-    }
+    Check(ippi_accumulate_mean(h_, (int64_t)(t - startAvg)), "ippi_accumulate_mean");
+    have_mean_ = true;
+}
+
+// Means for the writers (zeros before the first ComputeAvgVelocity, as the
+// reference's avg fields start at zero).
+int Mesh::Means(std::vector<double>& rho, std::vector<double>& u, std::vector<double>& v) {
+    const size_t N = (size_t)nx_ * ny_;
+    rho.assign(N, 0.0);
+    u.assign(N, 0.0);
+    v.assign(N, 0.0);
+    if (!have_mean_) return IPPI_OK;
+    return ippi_get_mean(h_, rho.data(), u.data(), v.data());
 }
 
 // WriteDatFluid, mesh.cpp:1180-1243: Tecplot ASCII, point packing, x-major
-// (I = Ny fastest), variables X Y Density U V avgu avgv tau avgRho. The
-// relaxation time column holds tau0 (the step kernel keeps tau in registers).
+// (I = Ny fastest), variables X Y Density U V avgu avgv tau avgRho; tau is
+// the relaxation time of each node's next collision.
 void Mesh::WriteDatFluid(int t, int RankID, int NX, int CoreNum) {
     (void)CoreNum;
-    std::vector<double> r, u, v;
+    std::vector<double> r, u, v, ar, au, av, tau((size_t)nx_ * ny_, startTau);
     Check(Fields(r, u, v), "ippi_get_fields");
+    Check(Means(ar, au, av), "ippi_get_mean");
+    if (ippi_get_tau(h_, tau.data()) != IPPI_OK) tau.assign(tau.size(), startTau);  // before the first step
     std::stringstream fn;
     fn << "fluid/fluid_t" << t << "_" << RankID << ".dat";
     std::ofstream out(fn.str());
@@ -145,13 +148,67 @@ void Mesh::WriteDatFluid(int t, int RankID, int NX, int CoreNum) {
     out << "ZONE T=\"Time " << t << "\", ";
     out << "I=" << ny_ << ", J=" << NX << ", K=1, ";
     out << "DATAPACKING=POINT, ZONETYPE=Ordered\n";
-    const bool avg = avg_u_.size() == r.size();
     for (int X = 0; X < nx_; ++X)
         for (int Y = 0; Y < ny_; ++Y) {
             const size_t n = (size_t)Y * nx_ + X;
-            out << X << " " << Y << " " << r[n] << " " << u[n] << " " << v[n] << " " << (avg ? avg_u_[n] : 0.0)
-                << " " << (avg ? avg_v_[n] : 0.0) << " " << startTau << " " << (avg ? avg_rho_[n] : 0.0) << "\n";
+            out << X << " " << Y << " " << r[n] << " " << u[n] << " " << v[n] << " " << au[n] << " " << av[n] << " "
+                << tau[n] << " " << ar[n] << "\n";
         }
+}
+
+// WriteAvgu, mesh.cpp:2009-2067: rows 1 .. Ny-2 only (I = Ny-2), variables
+// X Y AvgdDenstiy U V avgy avgv Style; the density column holds rho (as the
+// reference writes it); Style 0 for fluid nodes, 3 for inlet / outlet.
+void Mesh::WriteAvgu(int t, int RankID, int NX, int CoreNum) {
+    (void)CoreNum;
+    std::vector<double> r, u, v, ar, au, av;
+    Check(Fields(r, u, v), "ippi_get_fields");
+    Check(Means(ar, au, av), "ippi_get_mean");
+    std::stringstream fn;
+    fn << "fluid/Avgu_" << t << "_" << RankID << ".dat";
+    std::ofstream out(fn.str());
+    out << "TITLE = \"Simulation of Fluid State at Time " << t << "\"\n";
+    out << "VARIABLES = \"X\" \"Y\" \"AvgdDenstiy\" \"U\" \"V\" \"avgy\" \"avgv\" \"Style\" \n";
+    out << "ZONE T=\"Time " << t << "\", ";
+    out << "I=" << ny_ - 2 << ", J=" << NX << ", K=1, ";
+    out << "DATAPACKING=POINT, ZONETYPE=Ordered\n";
+    const bool nee_cols = Boundary != IPPI_BC_PERIODIC_BB;
+    for (int X = 0; X < nx_; ++X)
+        for (int Y = 1; Y < ny_ - 1; ++Y) {
+            const size_t n = (size_t)Y * nx_ + X;
+            const int style = nee_cols && (X == 0 || X == nx_ - 1) ? 3 : 0;
+            out << X << " " << Y << " " << r[n] << " " << u[n] << " " << v[n] << " " << au[n] << " " << av[n] << " "
+                << style << "\n";
+        }
+}
+
+// MergeAvgu, mesh.cpp:2069-2099: one part file with one GPU.
+void Mesh::MergeAvgu(int t, int CoreNum) {
+    (void)CoreNum;
+    std::stringstream src, dst;
+    src << "fluid/Avgu_" << t << "_0.dat";
+    dst << "fluid/fluid_avg_" << t << ".dat";
+    std::rename(src.str().c_str(), dst.str().c_str());
+}
+
+// WriteDatParticle, mesh.cpp:1283-1319: Tecplot FE line segments through the
+// markers of every body, closed by an element from node 1 to the last node.
+void Mesh::WriteDatParticle(int t) {
+    int n = 0;
+    Check(ippi_get_markers(h_, nullptr, 1 << 30, &n), "ippi_get_markers");  // count
+    std::vector<double> mk(6 * (size_t)(n > 0 ? n : 1));
+    Check(ippi_get_markers(h_, mk.data(), n, &n), "ippi_get_markers");
+    std::stringstream fn;
+    fn << "particle/particle_t" << t << ".dat";
+    std::ofstream out(fn.str());
+    out << "TITLE = \"Particle State at Time " << t << "\"\n";
+    out << "VARIABLES = \"X\" \"Y\" \"Z\"\n";
+    out << "ZONE T=\"Particle\", N=" << n << ", E=" << n;
+    out << ", DATAPACKING=POINT, ZONETYPE=FELINESEG\n";
+    out << "Nodes=" << n << ", Elements=" << n << "\n";
+    for (int m = 0; m < n; ++m) out << mk[6 * m] << " " << mk[6 * m + 1] << " 0\n";
+    for (int m = 0; m + 1 < n; ++m) out << m + 1 << " " << m + 2 << "\n";
+    out << 1 << " " << n << "\n";
 }
 
 // MergeDatFluid, mesh.cpp:1320-1353: with one GPU there is one part file.

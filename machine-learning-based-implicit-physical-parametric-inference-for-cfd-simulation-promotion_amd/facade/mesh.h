@@ -54,6 +54,9 @@ public:
     void MergeDatFluid(int t, int CoreNum);
     void OutputCd(int RankID, int CoreNum, int time, double Radius);
     void ComputeAvgVelocity(int t);
+    void WriteAvgu(int t, int RankID, int NX, int CoreNum);
+    void MergeAvgu(int t, int CoreNum);
+    void WriteDatParticle(int t);
     void ErrorCatch(int i, int j);
 
     // direct access for callers that read the fields back
@@ -75,6 +78,7 @@ private:
         Point2D v;
     };
     std::vector<PendingBody> pending_;
-    std::vector<double> avg_u_, avg_v_, avg_rho_;
+    bool have_mean_ = false;
+    int Means(std::vector<double>& rho, std::vector<double>& u, std::vector<double>& v);
     int last_force_rec_ = 0;
 };
