@@ -35,6 +35,7 @@ namespace {
 
 struct Patch {
     int x0, y0, w, h, off;
+    int roff;  // offset of the body's region (patch grown by one node) in the region numbering
 };
 
 struct StepArgs {
@@ -54,6 +55,11 @@ struct StepArgs {
     int plate_x;            // IPPI_BC_FLATPLATE: first no-slip column of the bottom wall
     const double* corner;   // IPPI_BC_FLATPLATE: populations of (0, 0) and (nx-1, 0) ([2][9], full f)
     const double* mac;      // macro arrays (rho, ux, uy)
+    // populations the next IB kernel pulls, copied into slot[k][active node]
+    // by the node they come from (scat = nullptr: none)
+    const int* scat;        // [9][region node] active node pulling population k from it, or -1
+    void* slot;             // [9][active] (the population type)
+    int nreg, nact;
     Patch patch[IPPI_MAXB];
 };
 
@@ -346,6 +352,37 @@ __device__ __forceinline__ bool patch_force(const StepArgs& a, int x, int y, int
 
 constexpr int BX = 256;  // nodes per block along x (one row segment)
 
+// Slots of node (x, y) for the next IB kernel: sl[k] is the slot of the
+// active node that pulls population k from here (-1: none). Loaded at the
+// start of the node's work, so the loads overlap the population loads. A
+// node lies in at most one body's region for each k that has a slot.
+__device__ __forceinline__ void ib_slot_ids(const StepArgs& a, int x, int y, int* sl) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) sl[k] = -1;
+    if (!a.scat) return;
+    const int bx0 = blockIdx.x * BX, bx1 = bx0 + BX - 1;
+    for (int b = 0; b < a.nbody; ++b) {
+        const Patch p = a.patch[b];
+        if (y < p.y0 - 1 || y > p.y0 + p.h || bx1 < p.x0 - 1 || bx0 > p.x0 + p.w) continue;
+        if (x >= p.x0 - 1 && x <= p.x0 + p.w) {
+            const int ri = p.roff + (y - p.y0 + 1) * (p.w + 2) + (x - p.x0 + 1);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const int v = a.scat[k * a.nreg + ri];
+                sl[k] = v >= 0 ? v : sl[k];
+            }
+        }
+    }
+}
+// Population k of the stored state v to slot sl[k].
+template <typename T>
+__device__ __forceinline__ void ib_slots(const StepArgs& a, const int* sl, const T* v) {
+    T* slot = static_cast<T*>(a.slot);
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        if (sl[k] >= 0) slot[(size_t)k * a.nact + sl[k]] = v[k];
+}
+
 // Rows per thread of the fused step kernels: each thread walks NPT rows of
 // one column and issues the loads of row j+1 before the arithmetic of row j,
 // so every wave keeps two rows of populations in flight.
@@ -508,6 +545,8 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
 #pragma clang fp contract(fast)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
+    int sl[9];
+    ib_slot_ids(a, x, y, sl);
     Mo o;
     bool fresh = true;  // false: flat-plate corner, stored populations copied through
     float keep[BC == IPPI_BC_FLATPLATE ? 9 : 1];
@@ -642,6 +681,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     if (active) {
 #pragma unroll
         for (int k = 0; k < 9; k++) st(dst + k * N, n, g[k], a.nt);
+        if (a.scat) ib_slots<float>(a, sl, g);
     }
 }
 
@@ -1121,9 +1161,27 @@ __device__ __forceinline__ void support_u(const T* __restrict__ src, const doubl
     }
 }
 
-template <typename T, bool PULL, int E>
+// SLOT: the pulled populations of the active nodes were copied into
+// slot[k][active node] by the step kernel that stored them (one coalesced
+// load each instead of the node list and the stencil gathers).
+// Velocity from the nine pulled populations, as support_u computes it.
+template <typename T>
+__device__ __forceinline__ void slot_u(const T* f, double& ux, double& uy) {
+#pragma clang fp contract(off)
+    if constexpr (sizeof(T) == 8) {
+        double r;
+        ref::macro(f, r, ux, uy);
+    } else {
+        Mo o;
+        moments32(f, o);
+        ux = o.ux;
+        uy = o.uy;
+    }
+}
+
+template <typename T, bool PULL, int E, bool SLOT>
 __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const double* __restrict__ mac, IbDev d,
-                                             int nx, int ny) {
+                                             int nx, int ny, const T* __restrict__ slot, int nact_all) {
 #pragma clang fp contract(off)
     __shared__ double U[2 * IB_MAXACT];
     __shared__ double FB[2 * IB_MAXM];
@@ -1151,7 +1209,28 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
         ew[e] = d.ecw[(size_t)E * ia + e];
     }
     // A. velocity at every active node of the body, four nodes in flight per lane
-    for (int c0 = 0; c0 < bk.nact; c0 += 4 * IB_T) {
+    if (SLOT) {
+        for (int c0 = 0; c0 < bk.nact; c0 += 4 * IB_T) {
+            double ux[4], uy[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int i = bk.act0 + min(c0 + j * IB_T + t, bk.nact - 1);
+                T f[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) f[k] = slot[(size_t)k * nact_all + i];
+                slot_u(f, ux[j], uy[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int i = c0 + j * IB_T + t;
+                if (i < bk.nact) {
+                    U[2 * i] = ux[j];
+                    U[2 * i + 1] = uy[j];
+                }
+            }
+        }
+    }
+    for (int c0 = 0; c0 < (SLOT ? 0 : bk.nact); c0 += 4 * IB_T) {
         int nd[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) nd[j] = d.act[bk.act0 + min(c0 + j * IB_T + t, bk.nact - 1)];
@@ -1584,6 +1663,10 @@ struct ippi_handle {
     int ell = 8;
     double *d_w = nullptr, *d_ud = nullptr, *d_ui = nullptr, *d_F = nullptr, *d_ecw = nullptr, *d_pf = nullptr;
     int npn = 0, nact = 0, nblk_ib = 0;
+    int* d_scat = nullptr;   // [9][region node] active node pulling population k from it, or -1
+    void* d_slot = nullptr;  // [9][active] populations the next IB kernel pulls
+    int nreg = 0;            // region nodes (patches grown by one node)
+    bool slots_ok = false;   // d_slot holds the stored state's populations
     std::vector<int> h_act;      // active node -> patch node (host copy)
     std::vector<int> body_act0;  // first active node of each body (+ total)
     double* hist = nullptr;
@@ -1695,6 +1778,9 @@ void free_ib(ippi_handle* h) {
     dfree(h->d_pidx); dfree(h->d_ecnt); dfree(h->d_emk); dfree(h->d_act); dfree(h->d_blk);
     dfree(h->d_w); dfree(h->d_ud); dfree(h->d_ui); dfree(h->d_F); dfree(h->d_ecw); dfree(h->d_pf);
     dfree(h->hist);
+    dfree(h->d_scat);
+    dfree(h->d_slot);
+    h->slots_ok = false;
 }
 
 int upload_ib(ippi_handle* h) {
@@ -1785,9 +1871,57 @@ int upload_ib(ippi_handle* h) {
     HIPC(h, hipMemcpy(h->d_emk, emk.data(), sizeof(int) * emk.size(), hipMemcpyHostToDevice));
     HIPC(h, hipMemcpy(h->d_ecw, ecw.data(), sizeof(double) * ecw.size(), hipMemcpyHostToDevice));
     HIPC(h, hipMemcpy(h->d_blk, blk.data(), sizeof(IbBlk) * blk.size(), hipMemcpyHostToDevice));
+    // slot map: active node D pulls population k from D - e_k, a node of D's
+    // body region (patch grown by one node)
+    {
+        int roff = 0;
+        for (auto& b : h->bodies) {
+            b.p.roff = roff;
+            roff += (b.p.w + 2) * (b.p.h + 2);
+        }
+        h->nreg = roff;
+        std::vector<int> scat(9 * (size_t)roff, -1);
+        for (int bi = 0; bi < nb; bi++) {
+            const Patch& p = h->bodies[bi].p;
+            for (int i = act0s[bi]; i < act0s[bi + 1]; i++) {
+                const int r = act[i] - p.off, X = p.x0 + r % p.w, Y = p.y0 + r / p.w;
+                for (int k = 0; k < 9; k++) {
+                    const int sx = X - EX[k], sy = Y - EY[k];
+                    const int ri = p.roff + (sy - p.y0 + 1) * (p.w + 2) + (sx - p.x0 + 1);
+                    scat[(size_t)k * roff + ri] = i;
+                }
+            }
+        }
+        HIPC(h, hipMalloc(&h->d_scat, sizeof(int) * scat.size()));
+        HIPC(h, hipMemcpy(h->d_scat, scat.data(), sizeof(int) * scat.size(), hipMemcpyHostToDevice));
+        HIPC(h, hipMalloc(&h->d_slot, h->esz * 9 * act.size()));
+        h->slots_ok = false;
+    }
     h->nrec = 0;
     h->ib_dirty = false;
     return IPPI_OK;
+}
+
+// IB kernel on the handle's stream; slot: read the pulled populations from
+// the slots the previous step kernel wrote (pull steps only).
+template <typename T>
+void launch_ib_to(ippi_handle* h, const T* src, bool pull, const IbDev& d, bool slot = false) {
+    dim3 g(h->nblk_ib), b(IB_T);
+    const int nx = h->c.nx, ny = h->c.ny, na = h->nact;
+    const T* sl = (const T*)h->d_slot;
+#define IPPI_IB_LAUNCH(E)                                                                                     \
+    if (slot)                                                                                                 \
+        hipLaunchKernelGGL((k_ib<T, true, E, true>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na);        \
+    else if (pull)                                                                                            \
+        hipLaunchKernelGGL((k_ib<T, true, E, false>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na);       \
+    else                                                                                                      \
+        hipLaunchKernelGGL((k_ib<T, false, E, false>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na);
+    if (h->ell == 8) {
+        IPPI_IB_LAUNCH(8)
+    } else {
+        IPPI_IB_LAUNCH(16)
+    }
+#undef IPPI_IB_LAUNCH
 }
 
 template <typename T>
@@ -1796,19 +1930,7 @@ int launch_ib(ippi_handle* h, bool pull) {
     IbDev d{h->d_w, h->d_pidx, h->d_ud, h->d_ui, h->d_F, h->d_blk, h->d_act, h->d_ecnt, h->d_emk, h->d_ecw,
             h->d_pf, h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap)};
     const T* src = (const T*)h->buf[h->cur];
-    dim3 g(h->nblk_ib), b(IB_T);
-    const int nx = h->c.nx, ny = h->c.ny;
-    if (h->ell == 8) {
-        if (pull)
-            hipLaunchKernelGGL((k_ib<T, true, 8>), g, b, 0, h->s, src, h->mac, d, nx, ny);
-        else
-            hipLaunchKernelGGL((k_ib<T, false, 8>), g, b, 0, h->s, src, h->mac, d, nx, ny);
-    } else {
-        if (pull)
-            hipLaunchKernelGGL((k_ib<T, true, 16>), g, b, 0, h->s, src, h->mac, d, nx, ny);
-        else
-            hipLaunchKernelGGL((k_ib<T, false, 16>), g, b, 0, h->s, src, h->mac, d, nx, ny);
-    }
+    launch_ib_to<T>(h, src, pull, d, pull && h->slots_ok);
     HIPC(h, hipGetLastError());
     h->nrec++;
     return IPPI_OK;
@@ -1864,6 +1986,12 @@ int run_steps(ippi_handle* h, int nsteps) {
         if (r) return r;
     }
     StepArgs a = make_args(h);
+    if (ib && sizeof(T) == 4) {  // the FP32 step kernels copy what the next IB kernel pulls
+        a.scat = h->d_scat;
+        a.slot = h->d_slot;
+        a.nreg = h->nreg;
+        a.nact = h->nact;
+    }
     for (int t = 0; t < nsteps; t++) {
         const bool pull = h->phase == 1;
         if (!pull && h->c.bc == IPPI_BC_FLATPLATE) {
@@ -1888,6 +2016,7 @@ int run_steps(ippi_handle* h, int nsteps) {
                            (T*)h->buf[h->cur ^ 1], h->mac, a);
         HIPC(h, hipGetLastError());
         if (h->timing && pull) h->ev_step.push_back(new_event(h));
+        h->slots_ok = a.scat != nullptr;
         h->cur ^= 1;
         h->phase = 1;
         h->steps++;
@@ -1959,22 +2088,6 @@ struct AdjBufs {
     }
 };
 
-template <typename T>
-void launch_ib_to(ippi_handle* h, const T* src, bool pull, const IbDev& d) {
-    dim3 g(h->nblk_ib), b(IB_T);
-    const int nx = h->c.nx, ny = h->c.ny;
-    if (h->ell == 8) {
-        if (pull)
-            hipLaunchKernelGGL((k_ib<T, true, 8>), g, b, 0, h->s, src, h->mac, d, nx, ny);
-        else
-            hipLaunchKernelGGL((k_ib<T, false, 8>), g, b, 0, h->s, src, h->mac, d, nx, ny);
-    } else {
-        if (pull)
-            hipLaunchKernelGGL((k_ib<T, true, 16>), g, b, 0, h->s, src, h->mac, d, nx, ny);
-        else
-            hipLaunchKernelGGL((k_ib<T, false, 16>), g, b, 0, h->s, src, h->mac, d, nx, ny);
-    }
-}
 
 template <typename T, typename A>
 int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double* bw, double* J, double* g_tau,
@@ -2205,6 +2318,7 @@ static int init_common(ippi_handle* h, int mode, uint64_t seed, double rho0, dou
     HIPC(h, hipMemsetAsync(h->flag, 0, sizeof(int), h->s));
     h->cur = 0;
     h->phase = 0;
+    h->slots_ok = false;
     h->steps = 0;
     h->nrec = 0;
     return IPPI_OK;
@@ -2338,6 +2452,7 @@ int ippi_set_populations(ippi_handle* h, const double* f) {
     HIPC(h, hipMemset(h->flag, 0, sizeof(int)));
     h->cur = 0;
     h->phase = 0;
+    h->slots_ok = false;
     h->steps = 0;
     h->nrec = 0;
     return IPPI_OK;
@@ -2378,7 +2493,7 @@ int ippi_add_body(ippi_handle* h, double cx, double cy, double diameter, int M, 
     }
     if (x0 < 1 || y0 < 1 || x1 > h->c.nx - 2 || y1 > h->c.ny - 2)
         return fail(h, IPPI_ERR_ARG, "immersed-boundary support must lie in the interior");
-    b.p = Patch{x0, y0, x1 - x0 + 1, y1 - y0 + 1, 0};
+    b.p = Patch{x0, y0, x1 - x0 + 1, y1 - y0 + 1, 0, 0};
     for (const auto& o : h->bodies) {
         const Patch& q = o.p;
         if (!(b.p.x0 + b.p.w <= q.x0 || q.x0 + q.w <= b.p.x0 || b.p.y0 + b.p.h <= q.y0 || q.y0 + q.h <= b.p.y0))
@@ -2509,9 +2624,15 @@ int ippi_adjoint_drag(ippi_handle* h, int nsteps, int avg_from, int ckpt, int fl
                                      "constant or per-node tau");
     if (h->bodies.empty()) return fail(h, IPPI_ERR_STATE, "the drag objective needs an immersed body");
     if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
-    if (h->esz == 8) return adjoint_run<double, double>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
-    if (flags & IPPI_ADJ_FP64) return adjoint_run<float, double>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
-    return adjoint_run<float, float>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
+    int r;
+    if (h->esz == 8)
+        r = adjoint_run<double, double>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
+    else if (flags & IPPI_ADJ_FP64)
+        r = adjoint_run<float, double>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
+    else
+        r = adjoint_run<float, float>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
+    h->slots_ok = false;  // the recomputation ran step kernels that write no slots
+    return r;
 }
 
 int ippi_synchronize(ippi_handle* h) {
@@ -2704,6 +2825,7 @@ int ippi_load(ippi_handle* h, const void* buf, size_t size) {
     HIPC(h, hipMemset(h->flag, 0, sizeof(int)));
     h->phase = hd.phase;
     h->steps = hd.steps;
+    h->slots_ok = false;
     return IPPI_OK;
 }
 
