@@ -178,7 +178,7 @@ int ippi_trainer_adam(ippi_trainer* t, int iters, int batch, double lr, uint64_t
 
 /* Kernel timing (no counterpart in the reference: measurement only). While
  * enabled, every fused stream-collide launch and every immersed-boundary
- * kernel pair is bracketed by HIP events on the handle's stream;
+ * kernel launch is bracketed by HIP events on the handle's stream;
  * ippi_get_timing synchronises, returns the summed durations (ms) and
  * launch counts since the last call, and clears them. */
 int ippi_set_timing(ippi_handle* h, int enable);

@@ -10,9 +10,10 @@
 // i.e. the streaming of step t and the collision of step t+1, reading and
 // writing 9 populations per node once (A/B buffers, SoA f[k][y][x]).
 // The immersed boundary (mesh.cpp:514-832, 1244-1281, wired before the
-// collision) runs as two small kernels before each fused step: marker
-// interpolation + direct forcing, then a gather-style spread into a dense
-// per-body patch that the fused kernel reads.
+// collision) runs as one small kernel before each fused step: velocity at
+// the marker supports, marker interpolation + direct forcing, then a
+// gather-style spread into a dense per-body patch that the fused kernel
+// reads.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
