@@ -546,8 +546,8 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
 #pragma clang fp contract(fast)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
-    int sl[9];
-    ib_slot_ids(a, x, y, sl);
+    int sl[9];  // not in the network variants: VALU-bound, they need every register
+    if (!NN) ib_slot_ids(a, x, y, sl);
     Mo o;
     bool fresh = true;  // false: flat-plate corner, stored populations copied through
     float keep[BC == IPPI_BC_FLATPLATE ? 9 : 1];
@@ -682,7 +682,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     if (active) {
 #pragma unroll
         for (int k = 0; k < 9; k++) st(dst + k * N, n, g[k], a.nt);
-        if (a.scat) ib_slots<float>(a, sl, g);
+        if (!NN && a.scat) ib_slots<float>(a, sl, g);
     }
 }
 
@@ -1987,7 +1987,8 @@ int run_steps(ippi_handle* h, int nsteps) {
         if (r) return r;
     }
     StepArgs a = make_args(h);
-    if (ib && sizeof(T) == 4) {  // the FP32 step kernels copy what the next IB kernel pulls
+    if (ib && sizeof(T) == 4 && h->c.tau_mode != IPPI_TAU_NN) {  // FP32 step kernels without the network
+                                                                 // copy what the next IB kernel pulls
         a.scat = h->d_scat;
         a.slot = h->d_slot;
         a.nreg = h->nreg;
