@@ -330,6 +330,23 @@ def copy_probe(mb):
     print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 10}), flush=True)
 
 
+def pattern_probe(ippi):
+    """Bandwidth ceiling of the fused step's memory pattern: the same nine
+    pulled loads and nine stores with no arithmetic, next to a plain
+    nine-array copy, at the headline and network-leg sizes."""
+    out = {}
+    for (nx, ny) in ((NX, NY), (4096, 2048)):
+        for prec in ("fp32", "fp64"):
+            bpn = 72 if prec == "fp32" else 144
+            for mode in ("pull", "copy"):
+                us = ippi.pattern_probe(nx, ny, prec, mode, reps=300)
+                gbs = bpn * nx * ny / (us * 1e-6) / 1e9
+                out[f"{nx}x{ny}_{prec}_{mode}"] = {"us": round(us, 3), "gbs": round(gbs, 1),
+                                                   "frac_of_8tbs": round(gbs / 8000.0, 4)}
+                log(f"pattern probe {nx}x{ny} {prec} {mode}: {us:.2f} us, {gbs:.0f} GB/s")
+    print(json.dumps({"pattern_probe": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,6 +362,8 @@ def main():
                     help="steps of the event-timed pass used for the roofline figure")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (counter calibration)")
+    ap.add_argument("--pattern-probe", action="store_true",
+                    help="only time the fused step's memory pattern with no arithmetic (bandwidth ceiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the D2Q9 step runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
@@ -355,6 +374,9 @@ def main():
         return
     ippi = load_pkg()
     ippi.library()
+    if args.pattern_probe:
+        pattern_probe(ippi)
+        return
     K, W = args.steps, args.warmup
 
     s32 = make_headline(ippi, "fp32")
@@ -381,10 +403,13 @@ def main():
     s32.close()
     algo_bytes = 72 * NX * NY
     achieved = algo_bytes / (k_us * 1e-6) / 1e9
+    # the same memory pattern with no arithmetic: the ceiling the kernel's traffic can reach here
+    probe_us = ippi.pattern_probe(NX, NY, "fp32", "pull", reps=300)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
                 "kernel": "k_step_f32<MRT,INFLOW,pull>", "avg_launch_us": round(k_us, 3),
-                "algorithmic_bytes_per_launch": algo_bytes, "ib_kernel_avg_us": round(ib_us, 3)}
+                "algorithmic_bytes_per_launch": algo_bytes, "ib_kernel_avg_us": round(ib_us, 3),
+                "pattern_probe_us": round(probe_us, 3), "frac_of_pattern_probe": round(probe_us / k_us, 4)}
 
     legs = {}
     names = [x for x in args.legs.split(",") if x and x != "none"]

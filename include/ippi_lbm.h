@@ -226,6 +226,14 @@ int ippi_adjoint_drag(ippi_handle* h, int nsteps, int avg_from, int ckpt, int fl
  * recorded while timing was enabled; clears them. */
 int ippi_get_adjoint_timing(ippi_handle* h, double* ms, int64_t* launches);
 
+/* Bandwidth probe (no counterpart in the reference: measurement only). On
+ * its own buffers and stream, times `reps` launches of a kernel with the
+ * fused step's memory pattern and no arithmetic on an nx x ny lattice:
+ * mode 1 = the nine pulled loads of an inflow step and nine stores, mode 0
+ * = a nine-array copy; precision IPPI_FP64 or IPPI_FP32_SHIFTED. Writes the
+ * mean time per launch (microseconds). */
+int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch);
+
 /* Checkpoint: the full device state as a flat byte image. *size gives the
  * required size when buf is NULL. */
 int ippi_save(ippi_handle* h, void* buf, size_t* size);

@@ -88,6 +88,8 @@ _SIGS = {
     "ippi_adjoint_drag": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D],
                           ctypes.c_int),
     "ippi_get_adjoint_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "ippi_pattern_probe": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D],
+                           ctypes.c_int),
     "ippi_accumulate_mean": ([ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "ippi_get_mean": ([ctypes.c_void_p, _D, _D, _D], ctypes.c_int),
     "ippi_save": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
@@ -369,3 +371,15 @@ class Trainer:
 def drag_coefficient(force_x, u_ref, radius):
     """C = F / (0.5 U^2 2R), the normalisation of OutputCd (mesh.cpp:1501-1502)."""
     return force_x / (0.5 * u_ref * u_ref * 2 * radius)
+
+
+def pattern_probe(nx, ny, precision="fp32", mode="pull", reps=200, device=0):
+    """Microseconds per launch of a kernel with the fused step's memory
+    pattern and no arithmetic (mode "pull": the nine shifted loads of an
+    inflow step; "copy": a nine-array copy) on an nx x ny lattice."""
+    us = ctypes.c_double()
+    rc = library().ippi_pattern_probe(int(device), int(nx), int(ny), PRECISION[precision],
+                                      {"copy": 0, "pull": 1}[mode], int(reps), ctypes.byref(us))
+    if rc != 0:
+        raise IppiError(rc, "ippi_pattern_probe failed")
+    return us.value

@@ -72,16 +72,27 @@ template <typename T>
 __device__ __forceinline__ T ld(const T* __restrict__ base, unsigned idx) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(idx * (unsigned)sizeof(T)));
 }
-// Stores: while both population buffers fit in the 256 MB MALL (Infinity
-// Cache) the next step reads what this one writes, so the stores allocate
-// there; beyond that they stream past the caches (non-temporal).
-template <typename T>
-__device__ __forceinline__ void st(T* __restrict__ base, unsigned idx, T v, bool nt) {
+// Stores: up to twice the 256 MB MALL (Infinity Cache) for both population
+// buffers the next step still finds much of what this one writes there, so
+// the stores allocate; beyond that they stream past the caches
+// (non-temporal). Measured: non-temporal costs 43% at 151 MB and 4% at
+// 302 MB, and gains 2% at 604 MB and above.
+// The rule is a template parameter: with a runtime flag the compiler merges
+// the two identical stores and drops the non-temporal hint.
+template <bool NT, typename T>
+__device__ __forceinline__ void st(T* __restrict__ base, unsigned idx, T v) {
     T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(idx * (unsigned)sizeof(T)));
-    if (nt)
+    if (NT)
         __builtin_nontemporal_store(v, p);
     else
         *p = v;
+}
+
+// The nine populations of node n.
+template <bool NT, typename T>
+__device__ __forceinline__ void st9(T* __restrict__ dst, unsigned N, unsigned n, const T* v) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) st<NT>(dst + k * N, n, v[k]);
 }
 
 // Pull of the post-collision populations into node (x, y), with the
@@ -432,7 +443,7 @@ __device__ __forceinline__ double nn_tau_f64(const double* f, const double* q, d
     return mlp_tau_f64(in, blob);
 }
 
-template <int COLL, int BC, bool PULL, bool NN>
+template <int COLL, int BC, bool PULL, bool NN, bool NT>
 __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, const double* __restrict__ src,
                                          const double* __restrict__ mac, double* __restrict__ dst, const StepArgs& a,
                                          bool active) {
@@ -500,14 +511,16 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
         for (int k = 0; k < 9; k++) o[k] = o[k] + g[k];
     }
     if (active) {
+        double v[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) st(dst + k * N, n, fresh ? o[k] : f[k], a.nt);
+        for (int k = 0; k < 9; k++) v[k] = fresh ? o[k] : f[k];
+        st9<NT>(dst, N, n, v);
     }
 }
 
 // PULL = false: collide the fnew state stored at the node itself with the
 // moments from the macro arrays (first collision after initialisation).
-template <int COLL, int BC, bool PULL, int NPT, bool NN>
+template <int COLL, int BC, bool PULL, int NPT, bool NN, bool NT>
 __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src, double* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
     const int nx = a.nx, ny = a.ny;
@@ -525,7 +538,7 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
         const bool more = j + 1 < NPT && y + 1 < ny;
         double nxt[9];
         if (RAW && j + 1 < NPT) load_raw<double, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f64<COLL, BC, PULL, NN>(cur, x, y, N, src, mac, dst, a, active);
+        node_f64<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -539,7 +552,7 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
 // never touched, so FP32 rounding cannot drift mass or momentum. RR is
 // evaluated on populations with the increment projected onto exact mass and
 // momentum. The LES stress comes from the second moments directly.
-template <int COLL, int BC, bool PULL, bool NN>
+template <int COLL, int BC, bool PULL, bool NN, bool NT>
 __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const float* __restrict__ src,
                                          const double* __restrict__ mac, float* __restrict__ dst, const StepArgs& a,
                                          bool active, __bf16* lds) {
@@ -680,8 +693,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         for (int k = 0; k < 9; k++) g[k] = keep[BC == IPPI_BC_FLATPLATE ? k : 0];
     }
     if (active) {
-#pragma unroll
-        for (int k = 0; k < 9; k++) st(dst + k * N, n, g[k], a.nt);
+        st9<NT>(dst, N, n, g);
         if (!NN && a.scat) ib_slots<float>(a, sl, g);
     }
 }
@@ -689,7 +701,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
 // At least 4 waves per SIMD: the network variants are VALU-bound and need
 // the occupancy to overlap transcendental latency and the population loads
 // (caps them at 128 VGPRs; the others use far fewer).
-template <int COLL, int BC, bool PULL, int NPT, bool NN>
+template <int COLL, int BC, bool PULL, int NPT, bool NN, bool NT>
 __global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(4)))
 k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
@@ -712,7 +724,7 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
         const bool more = j + 1 < NPT && y + 1 < ny;
         float nxt[9];
         if (RAW && j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f32<COLL, BC, PULL, NN>(cur, x, y, N, src, mac, dst, a, active, lds);
+        node_f32<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active, lds);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -720,6 +732,29 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
     }
 }
 
+
+// Bandwidth probe: the fused step's memory pattern with no arithmetic.
+// MODE 1 loads the nine populations exactly as the pull of an inflow step
+// does (shifted rows and columns, SGPR base + 32-bit offsets) and stores
+// them with the step's store rule; MODE 0 copies them in place. The time
+// per launch is the ceiling the fused kernel's memory traffic can reach.
+template <typename T, int MODE, bool NT>
+__global__ void __launch_bounds__(BX) k_probe(const T* __restrict__ src, T* __restrict__ dst, int nx, int ny) {
+    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
+    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1), y = blockIdx.y;
+    const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
+    T f[9];
+    if (MODE == 1) {
+        load_raw<T, IPPI_BC_INFLOW, true>(src, x, y, nx, ny, N, f);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) f[k] = ld(src + k * N, n);
+    }
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) st<NT>(dst + k * N, n, f[k]);
+    }
+}
 
 // Relaxation time the next collision uses at each node, from the current
 // post-streaming state (IB velocity correction not applied): base value,
@@ -1716,14 +1751,19 @@ int check_flag(ippi_handle* h) {
 template <typename T>
 using StepKernel = void (*)(const T*, T*, const double*, StepArgs);
 
+// nt: non-temporal stores (pull steps of lattices whose two population
+// buffers exceed 512 MB)
 template <typename T, bool PULL, bool NN>
-StepKernel<T> pick_step(int coll, int bc) {
-#define IPPI_PICK(C, B)                                                                   \
-    if (coll == C && bc == B) {                                                           \
-        if constexpr (sizeof(T) == 8)                                                     \
-            return (StepKernel<T>)k_step_f64<C, B, PULL, PULL ? IPPI_NPT : 1, NN>;        \
-        else                                                                              \
-            return (StepKernel<T>)k_step_f32<C, B, PULL, PULL ? IPPI_NPT : 1, NN>;        \
+StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
+#define IPPI_PICK(C, B)                                                                         \
+    if (coll == C && bc == B) {                                                                 \
+        if constexpr (sizeof(T) == 8) {                                                         \
+            if (PULL && nt) return (StepKernel<T>)k_step_f64<C, B, PULL, IPPI_NPT, NN, PULL>;   \
+            return (StepKernel<T>)k_step_f64<C, B, PULL, PULL ? IPPI_NPT : 1, NN, false>;       \
+        } else {                                                                                \
+            if (PULL && nt) return (StepKernel<T>)k_step_f32<C, B, PULL, IPPI_NPT, NN, PULL>;   \
+            return (StepKernel<T>)k_step_f32<C, B, PULL, PULL ? IPPI_NPT : 1, NN, false>;       \
+        }                                                                                       \
     }
     IPPI_PICK(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
     IPPI_PICK(IPPI_COLL_MRT, IPPI_BC_PERIODIC_BB)
@@ -1756,7 +1796,7 @@ StepArgs make_args(ippi_handle* h) {
     a.patch_f = h->d_pf;
     a.flag = h->flag;
     a.nbody = (int)h->bodies.size();
-    a.nt = 2 * 9 * h->N * h->esz > ((size_t)192 << 20);
+    a.nt = 2 * 9 * h->N * h->esz > ((size_t)512 << 20);
     a.nn_max = h->c.tau_nn_max > 0 ? h->c.tau_nn_max : 1e30;
     a.nn_maxtxf = (float)(a.nn_max - 0.5);
     a.mlp.frag = h->d_frag;
@@ -2008,8 +2048,8 @@ int run_steps(ippi_handle* h, int nsteps) {
             if (h->timing) h->ev_ib.push_back(new_event(h));
         }
         const bool nn = h->c.tau_mode == IPPI_TAU_NN;
-        StepKernel<T> k = pull ? (nn ? pick_step<T, true, true>(h->c.collision, h->c.bc)
-                                     : pick_step<T, true, false>(h->c.collision, h->c.bc))
+        StepKernel<T> k = pull ? (nn ? pick_step<T, true, true>(h->c.collision, h->c.bc, a.nt)
+                                     : pick_step<T, true, false>(h->c.collision, h->c.bc, a.nt))
                                : (nn ? pick_step<T, false, true>(h->c.collision, h->c.bc)
                                      : pick_step<T, false, false>(h->c.collision, h->c.bc));
         if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
@@ -2608,6 +2648,62 @@ int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, dou
     int r = sum_events(h, h->ev_step, step_ms, step_launches);
     if (r) return r;
     return sum_events(h, h->ev_ib, ib_ms, ib_launches);
+}
+
+int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch) {
+    if (nx < 4 || ny < 4 || precision < 0 || precision > 1 || mode < 0 || mode > 1 || reps < 1 || !us_per_launch)
+        return IPPI_ERR_ARG;
+    if (9 * (uint64_t)nx * (uint64_t)ny >= ((uint64_t)1 << 32)) return IPPI_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return IPPI_ERR_HIP;
+    const size_t N = (size_t)nx * ny, esz = precision == IPPI_FP64 ? 8 : 4;
+    const int nt = 2 * 9 * N * esz > ((size_t)512 << 20);
+    void* b[2] = {nullptr, nullptr};
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = IPPI_OK;
+    auto chk = [&](hipError_t e, int code) {
+        if (e != hipSuccess && rc == IPPI_OK) rc = code;
+        return e == hipSuccess;
+    };
+    if (chk(hipMalloc(&b[0], 9 * N * esz), IPPI_ERR_NOMEM) && chk(hipMalloc(&b[1], 9 * N * esz), IPPI_ERR_NOMEM) &&
+        chk(hipMemset(b[0], 0, 9 * N * esz), IPPI_ERR_HIP) && chk(hipMemset(b[1], 0, 9 * N * esz), IPPI_ERR_HIP) &&
+        chk(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), IPPI_ERR_HIP) &&
+        chk(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence), IPPI_ERR_HIP) &&
+        chk(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence), IPPI_ERR_HIP)) {
+        const dim3 g((nx + BX - 1) / BX, ny);
+        auto launch = [&](int i) {
+            const void* s0 = b[i & 1];
+            void* d0 = b[(i & 1) ^ 1];
+#define IPPI_PROBE(TT, M, NTV) \
+    hipLaunchKernelGGL((k_probe<TT, M, NTV>), g, dim3(BX), 0, st, (const TT*)s0, (TT*)d0, nx, ny)
+            if (esz == 8) {
+                if (mode && nt) IPPI_PROBE(double, 1, true);
+                else if (mode) IPPI_PROBE(double, 1, false);
+                else if (nt) IPPI_PROBE(double, 0, true);
+                else IPPI_PROBE(double, 0, false);
+            } else {
+                if (mode && nt) IPPI_PROBE(float, 1, true);
+                else if (mode) IPPI_PROBE(float, 1, false);
+                else if (nt) IPPI_PROBE(float, 0, true);
+                else IPPI_PROBE(float, 0, false);
+            }
+#undef IPPI_PROBE
+        };
+        for (int i = 0; i < 10; i++) launch(i);  // warm-up
+        chk(hipEventRecord(e0, st), IPPI_ERR_HIP);
+        for (int i = 0; i < reps; i++) launch(i);
+        chk(hipEventRecord(e1, st), IPPI_ERR_HIP);
+        chk(hipGetLastError(), IPPI_ERR_HIP);
+        float ms = 0.f;
+        if (chk(hipEventSynchronize(e1), IPPI_ERR_HIP) && chk(hipEventElapsedTime(&ms, e0, e1), IPPI_ERR_HIP))
+            *us_per_launch = 1e3 * (double)ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    for (void* q : b)
+        if (q) (void)hipFree(q);
+    return rc;
 }
 
 int ippi_get_adjoint_timing(ippi_handle* h, double* ms, int64_t* launches) {

@@ -337,8 +337,15 @@ IPPI_HD void geq(float drho, float rho, float ux, float uy, float* q) {
     q[6] = WF[6] * (drho + ad - bd);
 }
 
-// 2 - omega for tau = 1/2 + tx (correctly rounded division)
-IPPI_HD float dl_of(float tx) { return (4.0f * tx) / (1.0f + 2.0f * tx); }
+// 2 - omega = 4 tx / (1 + 2 tx) for tau = 1/2 + tx: the hardware reciprocal
+// and one Newton correction of the quotient (within an ulp of the correctly
+// rounded division, in 4 instructions instead of 10)
+IPPI_HD float dl_of(float tx) {
+    const float a = 4.0f * tx, b = 1.0f + 2.0f * tx;
+    const float r = rcp(b);
+    const float q = a * r;
+    return fmaf(fmaf(-b, q, a), r, q);
+}
 
 // Hybrid Smagorinsky tau (mesh.cpp:877-910) as tx = tau - 1/2:
 // tau = (sqrt(tau0^2 + X) + tau0) / 2 with X = 18 Cs^2 Q / rho, written
