@@ -395,12 +395,15 @@ __device__ __forceinline__ void ib_slots(const StepArgs& a, const int* sl, const
         if (sl[k] >= 0) slot[(size_t)k * a.nact + sl[k]] = v[k];
 }
 
-// Rows per thread of the fused step kernels: each thread walks NPT rows of
-// one column and issues the loads of row j+1 before the arithmetic of row j,
-// so every wave keeps two rows of populations in flight.
-#ifndef IPPI_NPT
-#define IPPI_NPT 1
-#endif
+// Rows per thread of the fused pull-step kernels: each thread walks NPT rows
+// of one column and issues the loads of row j+1 before the arithmetic of
+// row j, so every wave keeps two rows of populations in flight. Measured on
+// the headline lattice: FP32 is fastest with one row (two or four cost
+// 3-15%), FP64 without the network with four (5% faster than one).
+template <typename T, bool PULL, bool NN>
+constexpr int step_npt() {
+    return PULL && sizeof(T) == 8 && !NN ? 4 : 1;
+}
 
 // ------------------------------------------------------------- FP64 step
 // Raw pulled populations of node (x, y) (for IPPI_BC_INFLOW the NEE columns
@@ -1757,12 +1760,13 @@ template <typename T, bool PULL, bool NN>
 StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
 #define IPPI_PICK(C, B)                                                                         \
     if (coll == C && bc == B) {                                                                 \
+        constexpr int NPT_ = step_npt<T, PULL, NN>();                                         \
         if constexpr (sizeof(T) == 8) {                                                         \
-            if (PULL && nt) return (StepKernel<T>)k_step_f64<C, B, PULL, IPPI_NPT, NN, PULL>;   \
-            return (StepKernel<T>)k_step_f64<C, B, PULL, PULL ? IPPI_NPT : 1, NN, false>;       \
+            if (PULL && nt) return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, NN, PULL>;       \
+            return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, NN, false>;                      \
         } else {                                                                                \
-            if (PULL && nt) return (StepKernel<T>)k_step_f32<C, B, PULL, IPPI_NPT, NN, PULL>;   \
-            return (StepKernel<T>)k_step_f32<C, B, PULL, PULL ? IPPI_NPT : 1, NN, false>;       \
+            if (PULL && nt) return (StepKernel<T>)k_step_f32<C, B, PULL, NPT_, NN, PULL>;       \
+            return (StepKernel<T>)k_step_f32<C, B, PULL, NPT_, NN, false>;                      \
         }                                                                                       \
     }
     IPPI_PICK(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
@@ -2020,7 +2024,9 @@ int sum_events(ippi_handle* h, std::vector<hipEvent_t>& v, double* ms, int64_t* 
 template <typename T>
 int run_steps(ippi_handle* h, int nsteps) {
     const int nx = h->c.nx, ny = h->c.ny;
-    const dim3 grid_pull((nx + BX - 1) / BX, (ny + IPPI_NPT - 1) / IPPI_NPT), grid_first((nx + BX - 1) / BX, ny);
+    const bool nn_mode = h->c.tau_mode == IPPI_TAU_NN;
+    const int npt = nn_mode ? step_npt<T, true, true>() : step_npt<T, true, false>();
+    const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const bool ib = !h->bodies.empty();
     if (ib) {
         int r = upload_ib(h);
@@ -2239,7 +2245,8 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     for (int b = 0; b < nb; b++) aa.patch[b] = h->bodies[b].p;
     AdjIb ad{h->nact, h->ell, h->d_act, dan, lf, uact, h->d_pidx, h->d_w, dwx, dwy, nullptr,
              h->d_ecnt, h->d_emk, h->d_ecw, dtab, pl, gc};
-    const dim3 grid_pull((nx + BX - 1) / BX, (ny + IPPI_NPT - 1) / IPPI_NPT), grid_first((nx + BX - 1) / BX, ny);
+    constexpr int npt = step_npt<T, true, false>();
+    const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const dim3 grid_adj((nx + BX - 1) / BX, ny), grid_act((h->nact + 255) / 256);
     const StepKernel<T> kpull = pick_step<T, true, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB);
     const StepKernel<T> kfirst = pick_step<T, false, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB);
