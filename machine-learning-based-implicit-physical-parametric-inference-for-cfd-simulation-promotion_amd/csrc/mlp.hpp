@@ -74,6 +74,10 @@ __device__ __forceinline__ void split(float v, __bf16& h, __bf16& l) {
 // layer 1, the activation, layer 2 and the output sum before the next, so
 // only one block's tiles are live at a time.
 __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m, __bf16* lds) {
+    // contraction lets the output sum acc += (w3 t2) sigma(t2) use FMAs; the
+    // feature normalisation below keeps its subtraction (exact for features
+    // close to their mean) before the scaling
+#pragma clang fp contract(fast)
     const int l = threadIdx.x & 63;
     const int g = l >> 4;  // lane group
     const MlpFrag& fr = m.frag[l];
