@@ -37,6 +37,7 @@ namespace {
 struct Patch {
     int x0, y0, w, h, off;
     int roff;  // offset of the body's region (patch grown by one node) in the region numbering
+    int m0;    // first marker of the body
 };
 
 struct StepArgs {
@@ -61,6 +62,16 @@ struct StepArgs {
     const int* scat;        // [9][region node] active node pulling population k from it, or -1
     void* slot;             // [9][active] (the population type)
     int nreg, nact;
+    // patch forces gathered from the marker forces F (ib_gather = 1) instead
+    // of read from patch_f: pslot maps a patch node to its active node (-1:
+    // no marker reaches it), whose gather list (ecnt, emk, ecw, row width
+    // ell) names the markers in increasing order
+    int ib_gather, ell;
+    const int* pslot;
+    const int* ecnt;
+    const int* emk;
+    const double* ecw;
+    const double* F;
     Patch patch[IPPI_MAXB];
 };
 
@@ -354,8 +365,42 @@ __device__ __forceinline__ bool patch_force(const StepArgs& a, int x, int y, int
         if (bx1 < p.x0 || bx0 >= p.x0 + p.w || by1 < p.y0 || by0 >= p.y0 + p.h) continue;
         if (x >= p.x0 && x < p.x0 + p.w && y >= p.y0 && y < p.y0 + p.h) {
             const size_t i = (size_t)p.off + (size_t)(y - p.y0) * p.w + (x - p.x0);
-            fx = a.patch_f[2 * i];
-            fy = a.patch_f[2 * i + 1];
+            if (!a.ib_gather) {
+                fx = a.patch_f[2 * i];
+                fy = a.patch_f[2 * i + 1];
+                return true;
+            }
+            // SpreadForce as a gather: sum of F_m phi phi over the markers
+            // reaching the node, in increasing marker order (zero if none),
+            // the order and arithmetic of the IB kernel's own spread
+            fx = 0.0;
+            fy = 0.0;
+            const int ia = a.pslot[i];
+            if (ia >= 0) {
+                const size_t E = (size_t)a.ell;
+                const int cnt = a.ecnt[ia];
+                for (int e0 = 0; e0 < cnt; e0 += 2) {
+                    int mk[2];
+                    double w[2], Fx[2], Fy[2];
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {  // padding entries are valid (marker 0, weight 0)
+                        mk[j] = p.m0 + a.emk[E * ia + min(e0 + j, (int)E - 1)];
+                        w[j] = a.ecw[E * ia + min(e0 + j, (int)E - 1)];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {
+                        Fx[j] = a.F[2 * mk[j]];
+                        Fy[j] = a.F[2 * mk[j] + 1];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {
+                        if (e0 + j < cnt) {
+                            fx += (Fx[j] * w[j]);
+                            fy += (Fy[j] * w[j]);
+                        }
+                    }
+                }
+            }
             return true;
         }
     }
@@ -701,11 +746,12 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     }
 }
 
-// At least 4 waves per SIMD: the network variants are VALU-bound and need
-// the occupancy to overlap transcendental latency and the population loads
-// (caps them at 128 VGPRs; the others use far fewer).
+// At least 4 waves per SIMD for the network variants: they are VALU-bound
+// and need the occupancy to overlap transcendental latency and the
+// population loads (caps them at 128 VGPRs). The others are latency-bound
+// streams and keep the full 8 waves (64 VGPRs).
 template <int COLL, int BC, bool PULL, int NPT, bool NN, bool NT>
-__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? 4 : 8)))
 k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
     const int nx = a.nx, ny = a.ny;
@@ -1218,13 +1264,17 @@ __device__ __forceinline__ void slot_u(const T* f, double& ux, double& uy) {
     }
 }
 
+// gather: the step kernel gathers the patch forces from F itself, so only
+// the first block of each body runs (marker forces and record), without
+// the spread.
 template <typename T, bool PULL, int E, bool SLOT>
 __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const double* __restrict__ mac, IbDev d,
-                                             int nx, int ny, const T* __restrict__ slot, int nact_all) {
+                                             int nx, int ny, const T* __restrict__ slot, int nact_all, int gather) {
 #pragma clang fp contract(off)
     __shared__ double U[2 * IB_MAXACT];
     __shared__ double FB[2 * IB_MAXM];
     const IbBlk bk = d.blk[blockIdx.x];
+    if (gather && !bk.lead) return;  // block-uniform
     const int t = threadIdx.x;
     const unsigned N = (unsigned)nx * ny;
     // prefetch everything that does not depend on the populations
@@ -1320,7 +1370,7 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
     }
     __syncthreads();
     // C. spread (gather) for this block's slice of active nodes
-    if (bk.a0 + t < bk.a1) {
+    if (!gather && bk.a0 + t < bk.a1) {
         double fx = 0.0, fy = 0.0;
 #pragma unroll
         for (int e = 0; e < E; e++) {
@@ -1702,6 +1752,7 @@ struct ippi_handle {
     int ell = 8;
     double *d_w = nullptr, *d_ud = nullptr, *d_ui = nullptr, *d_F = nullptr, *d_ecw = nullptr, *d_pf = nullptr;
     int npn = 0, nact = 0, nblk_ib = 0;
+    int* d_pslot = nullptr;  // [patch node] active node (all bodies' numbering), or -1
     int* d_scat = nullptr;   // [9][region node] active node pulling population k from it, or -1
     void* d_slot = nullptr;  // [9][active] populations the next IB kernel pulls
     int nreg = 0;            // region nodes (patches grown by one node)
@@ -1825,6 +1876,7 @@ void free_ib(ippi_handle* h) {
     dfree(h->hist);
     dfree(h->d_scat);
     dfree(h->d_slot);
+    dfree(h->d_pslot);
     h->slots_ok = false;
 }
 
@@ -1835,6 +1887,7 @@ int upload_ib(ippi_handle* h) {
     int off = 0;
     for (auto& b : h->bodies) {
         b.p.off = off;
+        b.p.m0 = b.first;
         off += b.p.w * b.p.h;
     }
     h->npn = off;
@@ -1855,7 +1908,7 @@ int upload_ib(ippi_handle* h) {
     if (emax > 16) return fail(h, IPPI_ERR_ARG, "more than 16 markers reach one node (markers too dense)");
     const int E = emax <= 8 ? 8 : 16;
     h->ell = E;
-    std::vector<int> act, ecnt, emk, pidx(16 * (size_t)nm), slot(off, -1);
+    std::vector<int> act, ecnt, emk, pidx(16 * (size_t)nm), slot(off, -1), gslot(off, -1);
     std::vector<double> ecw;
     std::vector<IbBlk> blk;
     std::vector<int> act0s;
@@ -1866,6 +1919,7 @@ int upload_ib(ippi_handle* h) {
         for (int r = b.p.off; r < b.p.off + b.p.w * b.p.h; r++) {
             if (rows[r].empty()) continue;
             slot[r] = (int)act.size() - a0;
+            gslot[r] = (int)act.size();
             act.push_back(r);
             ecnt.push_back((int)rows[r].size());
             for (int e = 0; e < E; e++) {
@@ -1940,6 +1994,8 @@ int upload_ib(ippi_handle* h) {
         HIPC(h, hipMalloc(&h->d_scat, sizeof(int) * scat.size()));
         HIPC(h, hipMemcpy(h->d_scat, scat.data(), sizeof(int) * scat.size(), hipMemcpyHostToDevice));
         HIPC(h, hipMalloc(&h->d_slot, h->esz * 9 * act.size()));
+        HIPC(h, hipMalloc(&h->d_pslot, sizeof(int) * off));
+        HIPC(h, hipMemcpy(h->d_pslot, gslot.data(), sizeof(int) * off, hipMemcpyHostToDevice));
         h->slots_ok = false;
     }
     h->nrec = 0;
@@ -1950,17 +2006,17 @@ int upload_ib(ippi_handle* h) {
 // IB kernel on the handle's stream; slot: read the pulled populations from
 // the slots the previous step kernel wrote (pull steps only).
 template <typename T>
-void launch_ib_to(ippi_handle* h, const T* src, bool pull, const IbDev& d, bool slot = false) {
+void launch_ib_to(ippi_handle* h, const T* src, bool pull, const IbDev& d, bool slot = false, bool gather = false) {
     dim3 g(h->nblk_ib), b(IB_T);
-    const int nx = h->c.nx, ny = h->c.ny, na = h->nact;
+    const int nx = h->c.nx, ny = h->c.ny, na = h->nact, ga = gather ? 1 : 0;
     const T* sl = (const T*)h->d_slot;
 #define IPPI_IB_LAUNCH(E)                                                                                     \
     if (slot)                                                                                                 \
-        hipLaunchKernelGGL((k_ib<T, true, E, true>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na);        \
+        hipLaunchKernelGGL((k_ib<T, true, E, true>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na, ga);        \
     else if (pull)                                                                                            \
-        hipLaunchKernelGGL((k_ib<T, true, E, false>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na);       \
+        hipLaunchKernelGGL((k_ib<T, true, E, false>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na, ga);       \
     else                                                                                                      \
-        hipLaunchKernelGGL((k_ib<T, false, E, false>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na);
+        hipLaunchKernelGGL((k_ib<T, false, E, false>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na, ga);
     if (h->ell == 8) {
         IPPI_IB_LAUNCH(8)
     } else {
@@ -1975,7 +2031,7 @@ int launch_ib(ippi_handle* h, bool pull) {
     IbDev d{h->d_w, h->d_pidx, h->d_ud, h->d_ui, h->d_F, h->d_blk, h->d_act, h->d_ecnt, h->d_emk, h->d_ecw,
             h->d_pf, h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap)};
     const T* src = (const T*)h->buf[h->cur];
-    launch_ib_to<T>(h, src, pull, d, pull && h->slots_ok);
+    launch_ib_to<T>(h, src, pull, d, pull && h->slots_ok, true);
     HIPC(h, hipGetLastError());
     h->nrec++;
     return IPPI_OK;
@@ -2033,6 +2089,15 @@ int run_steps(ippi_handle* h, int nsteps) {
         if (r) return r;
     }
     StepArgs a = make_args(h);
+    if (ib) {  // patch forces gathered from the marker forces the IB kernel leaves
+        a.ib_gather = 1;
+        a.ell = h->ell;
+        a.pslot = h->d_pslot;
+        a.ecnt = h->d_ecnt;
+        a.emk = h->d_emk;
+        a.ecw = h->d_ecw;
+        a.F = h->d_F;
+    }
     if (ib && sizeof(T) == 4 && h->c.tau_mode != IPPI_TAU_NN) {  // FP32 step kernels without the network
                                                                  // copy what the next IB kernel pulls
         a.scat = h->d_scat;
@@ -2542,7 +2607,7 @@ int ippi_add_body(ippi_handle* h, double cx, double cy, double diameter, int M, 
     }
     if (x0 < 1 || y0 < 1 || x1 > h->c.nx - 2 || y1 > h->c.ny - 2)
         return fail(h, IPPI_ERR_ARG, "immersed-boundary support must lie in the interior");
-    b.p = Patch{x0, y0, x1 - x0 + 1, y1 - y0 + 1, 0, 0};
+    b.p = Patch{x0, y0, x1 - x0 + 1, y1 - y0 + 1, 0, 0, 0};
     for (const auto& o : h->bodies) {
         const Patch& q = o.p;
         if (!(b.p.x0 + b.p.w <= q.x0 || q.x0 + q.w <= b.p.x0 || b.p.y0 + b.p.h <= q.y0 || q.y0 + q.h <= b.p.y0))
