@@ -280,6 +280,37 @@ def leg_accuracy(ippi, marks=(1000, 2000, 5000, 10000, 20000)):
     return out
 
 
+def leg_tausearch(ippi, n=65536, n_cpu=48):
+    """FindAndWriteTau's candidate search (20,001 candidates, FP64) for n
+    nodes on the GPU, next to the oracle's sequential loop on a sample of
+    n_cpu nodes (one core), with the chosen candidates compared."""
+    sys.path.insert(0, ROOT)
+    from oracle import pyoracle
+    rng = np.random.default_rng(SEED)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    feq = w * (1 + rng.uniform(-0.02, 0.02, (n, 9)))
+    fnew = feq + w * rng.uniform(-0.05, 0.05, (n, 9))
+    tau = 0.55 + rng.uniform(-0.02, 0.02, (n, 9))
+    target = np.stack([1.0 + rng.uniform(-0.01, 0.01, n), rng.uniform(-0.05, 0.1, n),
+                       rng.uniform(-0.03, 0.03, n)], 1)
+    ippi.tau_search(fnew[:256], feq[:256], tau[:256], target[:256])  # warm-up
+    t0 = time.perf_counter()
+    bd, be = ippi.tau_search(fnew, feq, tau, target)
+    gpu_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bo, eo = pyoracle.tau_search(fnew[:n_cpu], feq[:n_cpu], tau[:n_cpu], target[:n_cpu])
+    cpu_s = time.perf_counter() - t0
+    cand = 20001
+    log(f"tau search: {n} nodes on the GPU in {gpu_s * 1e3:.1f} ms; {n_cpu} nodes on one CPU core in {cpu_s:.2f} s")
+    return {"workload": f"FindAndWriteTau candidate loop (m = -10000..10000, step 1e-5, FP64), {n} nodes, "
+                        "seeded populations and targets",
+            "gpu_ms_incl_transfers": round(gpu_s * 1e3, 2),
+            "gpu_candidates_per_s": round(n * cand / gpu_s, 1),
+            "cpu_candidates_per_s_one_core": round(n_cpu * cand / cpu_s, 1),
+            "speedup_vs_one_core": round((n * cand / gpu_s) / (n_cpu * cand / cpu_s), 1),
+            "cpu_sample_bitwise_equal": bool(np.array_equal(bd[:n_cpu], bo) and np.array_equal(be[:n_cpu], eo))}
+
+
 def leg_flatplate(ippi, steps=5000, marks=(1000, 5000)):
     """The reference's own run (IPPI-LBM/main.cpp): flat plate on
     2401 x 501, BGK + hybrid LES (Cs = 0.173), U = 0.1, Re = 5e5
@@ -354,8 +385,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison of the headline")
-    ap.add_argument("--legs", default="nn,adjoint,multibody,accuracy,flatplate",
-                    help="comma list of extra workloads (nn, adjoint, multibody, accuracy, flatplate) or 'none'")
+    ap.add_argument("--legs", default="nn,adjoint,multibody,accuracy,flatplate,tausearch",
+                    help="comma list of extra workloads (nn, adjoint, multibody, accuracy, flatplate, tausearch) "
+                         "or 'none'")
     ap.add_argument("--nn-steps", type=int, default=10000)
     ap.add_argument("--mb-steps", type=int, default=5000)
     ap.add_argument("--profile-steps", type=int, default=500,
@@ -423,6 +455,8 @@ def main():
         legs["accuracy_2048x1024"] = leg_accuracy(ippi)
     if "flatplate" in names:
         legs["flatplate_2401x501"] = leg_flatplate(ippi)
+    if "tausearch" in names:
+        legs["tausearch"] = leg_tausearch(ippi)
 
     cpu = None
     if not args.no_cpu:

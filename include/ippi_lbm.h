@@ -226,6 +226,19 @@ int ippi_adjoint_drag(ippi_handle* h, int nsteps, int avg_from, int ckpt, int fl
  * recorded while timing was enabled; clears them. */
 int ippi_get_adjoint_timing(ippi_handle* h, double* ms, int64_t* launches);
 
+/* Relaxation-time label search: the candidate loop of Mesh::FindAndWriteTau
+ * (IPPI-LBM/mesh.cpp:2667-2704) for n nodes at once, stateless, FP64.
+ * Node i: fnew, feq, tau [i][k] of the node population k streams in from
+ * (the reference's inei/jnei neighbour of the opposite direction), target
+ * [i] = (rho, u, v) of the high-fidelity data. For m = m_lo..m_hi, dtau =
+ * m * step, the sources are collided with tau + dtau and scored by
+ * |rho - rho*| + |sum e_x f - u*| + |sum e_y f - v*|; best_dtau[i] and
+ * best_err[i] get the first smallest score (1e10 and 1e10 when no score is
+ * below 1e10). The reference's own call is m_lo = -10000, m_hi = 10000,
+ * step = 1e-5. */
+int ippi_tau_search(int device, int n, const double* fnew, const double* feq, const double* tau,
+                    const double* target, int m_lo, int m_hi, double step, double* best_dtau, double* best_err);
+
 /* Bandwidth probe (no counterpart in the reference: measurement only). On
  * its own buffers and stream, times `reps` launches of a kernel with the
  * fused step's memory pattern and no arithmetic on an nx x ny lattice:

@@ -90,6 +90,8 @@ _SIGS = {
     "ippi_get_adjoint_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "ippi_pattern_probe": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D],
                            ctypes.c_int),
+    "ippi_tau_search": ([ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                         _D, _D], ctypes.c_int),
     "ippi_accumulate_mean": ([ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "ippi_get_mean": ([ctypes.c_void_p, _D, _D, _D], ctypes.c_int),
     "ippi_save": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
@@ -383,3 +385,19 @@ def pattern_probe(nx, ny, precision="fp32", mode="pull", reps=200, device=0):
     if rc != 0:
         raise IppiError(rc, "ippi_pattern_probe failed")
     return us.value
+
+
+def tau_search(fnew, feq, tau, target, m_lo=-10000, m_hi=10000, step=1e-5, device=0):
+    """The candidate loop of FindAndWriteTau (mesh.cpp:2667-2704) for n nodes
+    on the GPU: fnew, feq, tau [n, 9] of the node each population streams in
+    from, target [n, 3] = (rho, u, v). Returns (best dtau [n], score [n])."""
+    fnew, feq, tau, target = (np.ascontiguousarray(a, dtype=np.float64) for a in (fnew, feq, tau, target))
+    n = fnew.shape[0]
+    if fnew.shape != (n, 9) or feq.shape != (n, 9) or tau.shape != (n, 9) or target.shape != (n, 3):
+        raise ValueError("fnew, feq, tau must be [n, 9] and target [n, 3]")
+    bd, be = np.zeros(n), np.zeros(n)
+    rc = library().ippi_tau_search(int(device), n, _p(fnew), _p(feq), _p(tau), _p(target), int(m_lo),
+                                   int(m_hi), float(step), _p(bd), _p(be))
+    if rc != 0:
+        raise IppiError(rc, "ippi_tau_search failed")
+    return bd, be

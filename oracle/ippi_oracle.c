@@ -1008,3 +1008,37 @@ double or_drag_objective(int nx, int ny, uint64_t seed, double ux0, double uy0, 
     or_destroy(s);
     return J;
 }
+
+/* FindAndWriteTau's search loop, IPPI-LBM/mesh.cpp:2667-2704, for each of
+ * n nodes: candidate fcol_k = (1 - 1/(tau_k + dtau)) fnew_k + feq_k /
+ * (tau_k + dtau) in the reference's operation order, macros summed over
+ * k = 0..8, the score |rho - rho*| + |jx - u*| + |jy - v*| summed left to
+ * right, kept when strictly below the best so far. */
+void or_tau_search(int n, const double* fnew, const double* feq, const double* tau, const double* target, int m_lo,
+                   int m_hi, double step, double* best_dtau, double* best_err) {
+    for (int i = 0; i < n; i++) {
+        const double* fn = fnew + 9 * (size_t)i;
+        const double* fq = feq + 9 * (size_t)i;
+        const double* tk = tau + 9 * (size_t)i;
+        const double rt = target[3 * i], ut = target[3 * i + 1], vt = target[3 * i + 2];
+        double bestd = 1e10, beste = 1e10;
+        for (int m = m_lo; m <= m_hi; m++) {
+            const double dt = (double)m * step;
+            double fc[9];
+            for (int k = 0; k < 9; k++) fc[k] = (1.0 - 1.0 / (tk[k] + dt)) * fn[k] + fq[k] / (tk[k] + dt);
+            double r = 0, u = 0, v = 0;
+            for (int k = 0; k < 9; k++) {
+                r += fc[k];
+                u += fc[k] * EX[k];
+                v += fc[k] * EY[k];
+            }
+            const double err = fabs(r - rt) + fabs(u - ut) + fabs(v - vt);
+            if (err < beste) {
+                beste = err;
+                bestd = dt;
+            }
+        }
+        best_dtau[i] = bestd;
+        best_err[i] = beste;
+    }
+}

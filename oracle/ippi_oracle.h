@@ -95,6 +95,17 @@ int    or_adjoint_drag(int nx, int ny, uint64_t seed, double ux0, double uy0, do
 double or_drag_objective(int nx, int ny, uint64_t seed, double ux0, double uy0, double noise, const double* tau,
                          double cx, double cy, double diam, int M, int nsteps, int avg_from);
 
+/* ---- tau-label search (FindAndWriteTau, IPPI-LBM/mesh.cpp:2667-2704) ---- */
+/* n nodes; node i gives, for each population k, the post-streaming value
+ * fnew[i][k], the equilibrium feq[i][k] and the relaxation time tau[i][k]
+ * of the node k streams from, and target[i] = (rho, u, v) of the
+ * high-fidelity data. For m = m_lo..m_hi, dtau = m * step: collide those
+ * populations with tau + dtau, sum rho, sum e_x f, sum e_y f, and score
+ * |rho - rho*| + |jx - u*| + |jy - v*|. The first m with the smallest score
+ * wins (best_err = 1e10, best_dtau = 1e10 when no score is below 1e10). */
+void or_tau_search(int n, const double* fnew, const double* feq, const double* tau, const double* target, int m_lo,
+                   int m_hi, double step, double* best_dtau, double* best_err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,7 @@ def lib():
         L.or_adjoint_drag.restype = i
         L.or_drag_objective.argtypes = [i, i, ctypes.c_uint64, d, d, d, _D, d, d, d, i, i, i]
         L.or_drag_objective.restype = d
+        L.or_tau_search.argtypes = [i, _D, _D, _D, _D, i, i, d, _D, _D]
         _lib = L
     return _lib
 
@@ -277,3 +278,14 @@ def adjoint_drag(nx, ny, seed, u0, noise, tau, center, diam, M, nsteps, avg_from
     lib().or_adjoint_drag(nx, ny, seed, u0[0], u0[1], noise, _p(t), center[0], center[1], diam, M, nsteps,
                           avg_from, _p(J), _p(gt), _p(gc))
     return J[0], gt.reshape(ny, nx), gc
+
+
+def tau_search(fnew, feq, tau, target, m_lo=-10000, m_hi=10000, step=1e-5):
+    """FindAndWriteTau's candidate search (mesh.cpp:2667-2704) for n nodes:
+    fnew, feq, tau [n, 9] (of the node each population streams from),
+    target [n, 3] = (rho, u, v). Returns (best dtau [n], best score [n])."""
+    fnew, feq, tau, target = (np.ascontiguousarray(a, dtype=np.float64) for a in (fnew, feq, tau, target))
+    n = fnew.shape[0]
+    bd, be = np.zeros(n), np.zeros(n)
+    lib().or_tau_search(n, _p(fnew), _p(feq), _p(tau), _p(target), int(m_lo), int(m_hi), float(step), _p(bd), _p(be))
+    return bd, be
