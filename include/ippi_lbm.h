@@ -239,6 +239,18 @@ int ippi_get_adjoint_timing(ippi_handle* h, double* ms, int64_t* launches);
 int ippi_tau_search(int device, int n, const double* fnew, const double* feq, const double* tau,
                     const double* target, int m_lo, int m_hi, double step, double* best_dtau, double* best_err);
 
+/* Inputs of the label search for n nodes (xy[i] = (x, y), two or more
+ * nodes inside the domain), as Mesh::FindAndWriteTau gathers them
+ * (mesh.cpp:2657-2686): for population k of node i, the node it streams in
+ * from, (x, y) - e_k, gives fnew[i][k] (its pulled population), feq[i][k]
+ * (the equilibrium of its macros) and tau[i][k] (its base or hybrid
+ * Smagorinsky relaxation time, without the IB velocity correction), all as
+ * the last pull step's collision used them, in FP64. The reference's stored
+ * feq is one step older than its fnew; here both belong to the same
+ * collision. Not for network tau. IPPI_ERR_STATE until a pull step has run
+ * since the state was set. */
+int ippi_get_stencil(ippi_handle* h, int n, const int* xy, double* fnew, double* feq, double* tau);
+
 /* Bandwidth probe (no counterpart in the reference: measurement only). On
  * its own buffers and stream, times `reps` launches of a kernel with the
  * fused step's memory pattern and no arithmetic on an nx x ny lattice:

@@ -90,6 +90,7 @@ _SIGS = {
     "ippi_get_adjoint_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "ippi_pattern_probe": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D],
                            ctypes.c_int),
+    "ippi_get_stencil": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _D, _D, _D], ctypes.c_int),
     "ippi_tau_search": ([ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                          _D, _D], ctypes.c_int),
     "ippi_accumulate_mean": ([ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
@@ -277,6 +278,17 @@ class Solver:
         self._check(library().ippi_get_timing(self.h, ctypes.byref(sm), ctypes.byref(sc), ctypes.byref(im),
                                               ctypes.byref(ic)))
         return sm.value, sc.value, im.value, ic.value
+
+    def stencil(self, nodes):
+        """(fnew, feq, tau) [n, 9] of the nodes each population of the given
+        nodes ([n, 2] of (x, y)) streams in from, as the last pull step's
+        collision used them: the inputs of tau_search."""
+        xy = np.ascontiguousarray(np.asarray(nodes, dtype=np.int32).reshape(-1, 2))
+        n = xy.shape[0]
+        fn, fq, tk = np.zeros((n, 9)), np.zeros((n, 9)), np.zeros((n, 9))
+        self._check(library().ippi_get_stencil(self.h, n, xy.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(fn),
+                                               _p(fq), _p(tk)))
+        return fn, fq, tk
 
     def adjoint_drag(self, nsteps, avg_from, ckpt=100, body_w=None, fp64_adjoint=False):
         """Run nsteps steps and the discrete adjoint of the averaged body force.

@@ -782,6 +782,40 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
 }
 
 
+// Inputs of the last pull step's collision at the nine nodes population k
+// of node (x, y) streams in from, s_k = (x, y) - e_k, in FP64: fnew_k(s_k)
+// (pulled from the state that step read, prev), feq_k(s_k) from the macros
+// of s_k's pulled populations, and s_k's relaxation time (base, or the
+// hybrid Smagorinsky value; the IB velocity correction is not applied). One
+// thread per (node, k); the nodes lie two or more rows and columns inside
+// the domain, so every pull is the plain stencil.
+template <typename T>
+__global__ void k_stencil(const T* __restrict__ prev, const int* __restrict__ xy, int n, StepArgs a,
+                          double* __restrict__ fnew, double* __restrict__ feq, double* __restrict__ tau) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 9 * n) return;
+    const int node = i / 9, k = i - 9 * node;
+    const int nx = a.nx, sx = xy[2 * node] - EX[k], sy = xy[2 * node + 1] - EY[k];
+    const unsigned N = (unsigned)nx * a.ny, sn = (unsigned)sy * nx + sx;
+    double f[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        const T v = ld(prev + q * N, (unsigned)(sy - EY[q]) * nx + (sx - EX[q]));
+        f[q] = sizeof(T) == 8 ? (double)v : (double)v + WD[q];
+    }
+    double rho, ux, uy, fq[9];
+    ref::macro(f, rho, ux, uy);
+    ref::feq(rho, ux, uy, fq);
+    double tb = a.tau0;
+    if (a.tau_field)
+        tb = sizeof(T) == 8 ? (double)ld(static_cast<const double*>(a.tau_ptr), sn)
+                            : 0.5 + (double)ld(static_cast<const float*>(a.tau_ptr), sn);
+    fnew[i] = f[k];
+    feq[i] = fq[k];
+    tau[i] = a.les ? ref::les_tau_q(f, fq, rho, tb, a.cs) : tb;
+}
+
 // Bandwidth probe: the fused step's memory pattern with no arithmetic.
 // MODE 1 loads the nine populations exactly as the pull of an inflow step
 // does (shifted rows and columns, SGPR base + 32-bit offsets) and stores
@@ -1757,6 +1791,7 @@ struct ippi_handle {
     void* d_slot = nullptr;  // [9][active] populations the next IB kernel pulls
     int nreg = 0;            // region nodes (patches grown by one node)
     bool slots_ok = false;   // d_slot holds the stored state's populations
+    bool prev_pull = false;  // buf[cur ^ 1] holds the state the last (pull) step read
     std::vector<int> h_act;      // active node -> patch node (host copy)
     std::vector<int> body_act0;  // first active node of each body (+ total)
     double* hist = nullptr;
@@ -2130,6 +2165,7 @@ int run_steps(ippi_handle* h, int nsteps) {
         HIPC(h, hipGetLastError());
         if (h->timing && pull) h->ev_step.push_back(new_event(h));
         h->slots_ok = a.scat != nullptr;
+        h->prev_pull = pull;
         h->cur ^= 1;
         h->phase = 1;
         h->steps++;
@@ -2433,6 +2469,7 @@ static int init_common(ippi_handle* h, int mode, uint64_t seed, double rho0, dou
     h->cur = 0;
     h->phase = 0;
     h->slots_ok = false;
+    h->prev_pull = false;
     h->steps = 0;
     h->nrec = 0;
     return IPPI_OK;
@@ -2567,6 +2604,7 @@ int ippi_set_populations(ippi_handle* h, const double* f) {
     h->cur = 0;
     h->phase = 0;
     h->slots_ok = false;
+    h->prev_pull = false;
     h->steps = 0;
     h->nrec = 0;
     return IPPI_OK;
@@ -2802,6 +2840,7 @@ int ippi_adjoint_drag(ippi_handle* h, int nsteps, int avg_from, int ckpt, int fl
     else
         r = adjoint_run<float, float>(h, nsteps, avg_from, ckpt, body_w, J, g_tau, g_center);
     h->slots_ok = false;  // the recomputation ran step kernels that write no slots
+    h->prev_pull = false;
     return r;
 }
 
@@ -2849,6 +2888,44 @@ int ippi_get_mean(ippi_handle* h, double* rho, double* ux, double* uy) {
     HIPC(h, hipMemcpyAsync(uy, h->mean + 2 * h->N, B, hipMemcpyDeviceToHost, h->s));
     HIPC(h, hipStreamSynchronize(h->s));
     return check_flag(h);
+}
+
+int ippi_get_stencil(ippi_handle* h, int n, const int* xy, double* fnew, double* feq, double* tau) {
+    if (!h || n < 0 || n > INT_MAX / 9 || (n > 0 && (!xy || !fnew || !feq || !tau))) return IPPI_ERR_ARG;
+    if (h->c.tau_mode == IPPI_TAU_NN)
+        return fail(h, IPPI_ERR_ARG, "the stencil covers constant or per-node tau (with or without LES)");
+    if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
+    if (!h->prev_pull) return fail(h, IPPI_ERR_STATE, "no pull step since the state was set");
+    for (int i = 0; i < n; i++)
+        if (xy[2 * i] < 2 || xy[2 * i] > h->c.nx - 3 || xy[2 * i + 1] < 2 || xy[2 * i + 1] > h->c.ny - 3)
+            return fail(h, IPPI_ERR_ARG, "stencil nodes must lie two or more nodes inside the domain");
+    if (n == 0) return IPPI_OK;
+    StepArgs a = make_args(h);
+    int* d_xy = nullptr;
+    double* d_o = nullptr;
+    const size_t s9 = sizeof(double) * 9 * (size_t)n;
+    hipError_t e = hipMalloc(&d_xy, sizeof(int) * 2 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&d_o, 3 * s9);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_xy, xy, sizeof(int) * 2 * (size_t)n, hipMemcpyHostToDevice, h->s);
+    if (e == hipSuccess) {
+        const dim3 g((unsigned)((9 * (size_t)n + 255) / 256));
+        double* o = d_o;
+        if (h->esz == 8)
+            hipLaunchKernelGGL(k_stencil<double>, g, dim3(256), 0, h->s, (const double*)h->buf[h->cur ^ 1], d_xy, n,
+                               a, o, o + 9 * (size_t)n, o + 18 * (size_t)n);
+        else
+            hipLaunchKernelGGL(k_stencil<float>, g, dim3(256), 0, h->s, (const float*)h->buf[h->cur ^ 1], d_xy, n, a,
+                               o, o + 9 * (size_t)n, o + 18 * (size_t)n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(fnew, d_o, s9, hipMemcpyDeviceToHost, h->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(feq, d_o + 9 * (size_t)n, s9, hipMemcpyDeviceToHost, h->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tau, d_o + 18 * (size_t)n, s9, hipMemcpyDeviceToHost, h->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+    dfree(d_xy);
+    dfree(d_o);
+    if (e != hipSuccess) return fail(h, IPPI_ERR_HIP, std::string("ippi_get_stencil: ") + hipGetErrorString(e));
+    return IPPI_OK;
 }
 
 int ippi_get_tau(ippi_handle* h, double* tau) {
@@ -2996,6 +3073,7 @@ int ippi_load(ippi_handle* h, const void* buf, size_t size) {
     h->phase = hd.phase;
     h->steps = hd.steps;
     h->slots_ok = false;
+    h->prev_pull = false;
     return IPPI_OK;
 }
 

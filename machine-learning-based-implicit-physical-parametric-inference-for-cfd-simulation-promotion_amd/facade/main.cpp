@@ -6,6 +6,8 @@
 //       populations; appends cd/cl to particle/CD_CL.dat every SaveStep.
 //   ippi_main check
 //       every writer on a small case (see run_check).
+//   ippi_main xflow <prefix>
+//       high-fidelity data and relaxation-time labels (see run_xflow).
 //   ippi_main plate [NumStep] [SaveStep]
 //       the reference's own run (main.cpp:13-48): flat plate on 2401 x 501,
 //       BGK + hybrid Smagorinsky (Cs = 0.173), U = 0.1, Re = 5e5,
@@ -152,8 +154,49 @@ static int run_check() {
     return 0;
 }
 
+// High-fidelity data and relaxation-time labels on a small case (64 x 40,
+// FP64 BGK + LES, inflow, per-node tau): data files prefix + (t + 10014) +
+// ".csv". Writes the parsed points of step 10 and the inputs of
+// FindAndWriteTau (its result goes to cp/BestTau.dat) at
+// the reference's slab column i = 21 (node x = 20), row 20, step 15.
+static int run_xflow(const char* prefix) {
+    int RankID = 0, CoreNum = 1;
+    const int Nx = 64, Ny = 40;
+    Mesh mesh;
+    mesh.LatmeshSizeNx = Nx;
+    mesh.LatmeshSizeNy = Ny;
+    mesh.Collision = IPPI_COLL_BGK;
+    mesh.Precision = IPPI_FP64;
+    mesh.Boundary = IPPI_BC_INFLOW;
+    mesh.UseLES = 1;
+    mesh.TauField = 1;
+    mesh.Seed = 5;
+    mesh.NoiseAmplitude = 1e-3;
+    mesh.XflowPrefix = prefix;
+    mesh.InitialFile();
+    mesh.InitialLatMesh(Nx, Ny, 0.58, Point2D{0.05, 0.0}, 1.0, RankID, CoreNum, 0);
+    for (int t = 1; t <= 10; t++) mesh.OneLBMStep(RankID, CoreNum, 0.12, 1.0, t);
+    std::FILE* f = std::fopen("xflow_points.txt", "w");
+    for (const auto& p : mesh.ReadXflowdata(10))
+        std::fprintf(f, "%.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", p.vx, p.vy, p.ux, p.uy, p.TotalPressure,
+                     p.viscosity, p.VelocityModule);
+    std::fclose(f);
+    mesh.ChangeTauFromXflow(10);
+    for (int t = 11; t <= 15; t++) mesh.OneLBMStep(RankID, CoreNum, 0.12, 1.0, t);
+    const int xy[2] = {20, 20};
+    double fn[9], fq[9], tk[9];
+    if (ippi_get_stencil(mesh.Handle(), 1, xy, fn, fq, tk) != IPPI_OK) return 1;
+    f = std::fopen("xflow_stencil.txt", "w");
+    for (int k = 0; k < 9; k++) std::fprintf(f, "%.17g %.17g %.17g\n", fn[k], fq[k], tk[k]);
+    std::fclose(f);
+    mesh.FindAndWriteTau(21, 20, 15);
+    std::cout << "Xflow Completed" << std::endl;
+    return 0;
+}
+
 int main(int argc, char* argv[]) {
     if (argc > 1 && std::strcmp(argv[1], "check") == 0) return run_check();
+    if (argc > 2 && std::strcmp(argv[1], "xflow") == 0) return run_xflow(argv[2]);
     if (argc > 1 && std::strcmp(argv[1], "plate") == 0)
         return run_plate(argc > 2 ? std::atoi(argv[2]) : 200000, argc > 3 ? std::atoi(argv[3]) : 20000);
     return run_cylinder(argc > 1 ? std::atoi(argv[1]) : 2000, argc > 2 ? std::atoi(argv[2]) : 500);

@@ -4,7 +4,9 @@
 
 #include <sys/stat.h>
 
+#include <cmath>
 #include <cstdio>
+#include <map>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
@@ -36,6 +38,8 @@ void Mesh::InitialLatMesh(int Nx_P, int Ny, double tau, Point2D StartVel, double
     }
     nx_ = Nx_P;
     ny_ = Ny;
+    if (LatmeshSizeNx == 0) LatmeshSizeNx = Nx_P;  // main.cpp:64-65 sets them; one rank holds all columns
+    if (LatmeshSizeNy == 0) LatmeshSizeNy = Ny;
     startTau = tau;
     rho0_ = Rho;
     u0_ = StartVel;
@@ -55,7 +59,7 @@ void Mesh::EnsureHandle(REAL Cs) {
     c.precision = Precision;
     c.bc = Boundary;
     c.les = UseLES;
-    c.tau_mode = IPPI_TAU_CONST;
+    c.tau_mode = TauField ? IPPI_TAU_FIELD : IPPI_TAU_CONST;
     c.tau0 = startTau;
     c.cs = Cs;
     c.u_in_x = startU.x;
@@ -69,7 +73,13 @@ void Mesh::EnsureHandle(REAL Cs) {
         Check(ippi_add_body(h_, b.cx, b.cy, b.d, b.m, b.v.x, b.v.y, &id), "ippi_add_body");
     }
     pending_.clear();
+    if (TauField) {
+        tau_.assign((size_t)nx_ * ny_, startTau);
+        UploadTau();
+    }
 }
+
+void Mesh::UploadTau() { Check(ippi_set_tau_field(h_, tau_.data()), "ippi_set_tau_field"); }
 
 // InitialIBParticle, mesh.cpp:173-191 (markers on a circle of radius Radius)
 void Mesh::InitialIBParticle(int NumNodes, double Radius, double X, double Y, Point2D Vel) {
@@ -237,4 +247,157 @@ void Mesh::OutputCd(int RankID, int CoreNum, int time, double Radius) {
         std::ofstream f("particle/CD_CL.dat", std::fstream::app);
         f << time << " " << cd << " " << cl << std::endl;
     }
+}
+
+// ---------------------------------------------------------------------------
+// High-fidelity data (XFlow CSV) and relaxation-time labels.
+
+// ReadXflowdata, mesh.cpp:2414-2545: one CSV per step, header line skipped;
+// each line is read as the reference's stream extraction reads it (13
+// comma-terminated numbers: id, x, y, z, VelocityModule, StaticPressure, -,
+// TotalPressure, -, viscosity, u, v, w; a line that does not parse is
+// skipped). Coordinates are rounded to 1e-3 m, then converted to lattice
+// units (x + 0.4 m, dx = 1 mm), velocities by the lattice sound speed
+// (340.112 m/s / sqrt 3), density from the static pressure; points with the
+// same (x, y) are merged by halving after each addition; points outside the
+// columns this rank holds are dropped. Merged points are kept in the order
+// of their first occurrence.
+std::vector<PointData> Mesh::ReadXflowdata(int t) {
+    const std::string name = XflowPrefix + std::to_string(t + XflowStepOffset) + ".csv";
+    std::ifstream file(name);
+    std::vector<PointData> data;
+    if (!file.is_open()) {
+        std::cerr << "ReadXflowdata: cannot open " << name << std::endl;
+        return data;
+    }
+    std::string line;
+    std::getline(file, line);  // header
+    while (std::getline(file, line)) {
+        std::stringstream ss(line);
+        PointData p{};
+        char comma;
+        double unused;
+        if (!(ss >> unused >> comma >> p.vx >> comma >> p.vy >> comma >> p.vz >> comma >> p.VelocityModule >> comma >>
+              p.StaticPress >> comma >> unused >> comma >> p.TotalPressure >> comma >> unused >> comma >>
+              p.viscosity >> comma >> p.ux >> comma >> p.uy >> comma >> p.uz >> comma))
+            continue;
+        p.vx = std::round(p.vx * 1000.0) / 1000.0;  // RoundToThreeDecimalPlaces, mesh.cpp:41-46
+        p.vy = std::round(p.vy * 1000.0) / 1000.0;
+        data.push_back(p);
+    }
+    const double DX = 0.001, dv = 340.112;
+    for (auto& p : data) {
+        p.vx = p.vx + 0.4;
+        p.vx = p.vx / DX;
+        p.vy = p.vy / DX;
+        p.vz = p.vz / DX;
+        p.VelocityModule = p.VelocityModule / dv / std::sqrt(3.0);
+        p.ux = p.ux / dv / std::sqrt(3.0);
+        p.uy = p.uy / dv / std::sqrt(3.0);
+        p.uz = p.uz / dv / std::sqrt(3.0);
+        p.TotalPressure = (p.StaticPress + 101325) / 101325;
+    }
+    std::vector<PointData> merged;
+    std::map<std::pair<double, double>, size_t> seen;
+    for (const auto& p : data) {
+        auto it = seen.find({p.vx, p.vy});
+        if (it == seen.end()) {
+            seen[{p.vx, p.vy}] = merged.size();
+            merged.push_back(p);
+            continue;
+        }
+        PointData& c = merged[it->second];
+        c.VelocityModule = (c.VelocityModule + p.VelocityModule) / 2.0;
+        c.StaticPress = (c.StaticPress + p.StaticPress) / 2.0;
+        c.TotalPressure = (c.TotalPressure + p.TotalPressure) / 2.0;
+        c.ux = (c.ux + p.ux) / 2.0;
+        c.uy = (c.uy + p.uy) / 2.0;
+        c.uz = (c.uz + p.uz) / 2.0;
+        c.vz = (c.vz + p.vz) / 2.0;
+        c.viscosity = (c.viscosity + p.viscosity) / 2.0;
+    }
+    // columns of this rank (one rank: the whole domain and the two ghosts)
+    const double lo = -2.0 - 0.5, hi = (double)LatmeshSizeNx + 0.5;
+    std::vector<PointData> out;
+    for (const auto& p : merged)
+        if (!(p.vx < lo || p.vx > hi)) out.push_back(p);
+    return out;
+}
+
+std::vector<int> Mesh::MatchXflow(const std::vector<PointData>& pts) const {
+    const double DX = 0.001;
+    std::vector<int> m((size_t)nx_ * ny_, -1);
+    for (size_t q = 0; q < pts.size(); q++) {
+        const double X = std::round(pts[q].vx), Y = std::round(pts[q].vy);
+        if (!(std::abs(pts[q].vx - X) < DX && std::abs(pts[q].vy - Y) < DX)) continue;
+        if (X < 0 || X >= nx_ || Y < 0 || Y >= ny_) continue;
+        m[(size_t)Y * nx_ + (size_t)X] = (int)q;  // the last point in the search order wins
+    }
+    return m;
+}
+
+// ChangeTauFromXflow, mesh.cpp:2730-2766: tau = 0.5 + 3 nu from the data at
+// every node, in the reference's node order; a node without data keeps the
+// viscosity and misfits of the node before it (the first ones: startTau and
+// zero misfits). The mean squared velocity and density misfits over the
+// nodes go to cp/InletMSE.dat.
+void Mesh::ChangeTauFromXflow(int t) {
+    if (!h_ || !TauField) {
+        std::cerr << "ChangeTauFromXflow: needs TauField = 1 and an initialised mesh" << std::endl;
+        std::exit(2);
+    }
+    const std::vector<PointData> pts = ReadXflowdata(t);
+    const std::vector<int> m = MatchXflow(pts);
+    std::vector<double> rho, ux, uy;
+    Check(Fields(rho, ux, uy), "ippi_get_fields");
+    double nu = (startTau - 0.5) / 3.0, ev = 0.0, er = 0.0, sv = 0.0, sr = 0.0;
+    for (int x = 0; x < nx_; x++)
+        for (int y = 0; y < ny_; y++) {
+            const size_t n = (size_t)y * nx_ + x;
+            if (m[n] >= 0) {
+                const PointData& p = pts[(size_t)m[n]];
+                nu = p.viscosity;
+                ev = (ux[n] - p.ux) * (ux[n] - p.ux) + (uy[n] - p.uy) * (uy[n] - p.uy);
+                er = (rho[n] - p.TotalPressure) * (rho[n] - p.TotalPressure);
+            }
+            sv += ev;
+            sr += er;
+            tau_[n] = 0.5 + 3.0 * nu;
+        }
+    UploadTau();
+    std::ofstream out("cp/InletMSE.dat", std::ios::app);
+    out << t << " " << sv / ((double)nx_ * ny_) << " " << sr / ((double)nx_ * ny_) << std::endl;
+}
+
+// FindAndWriteTau, mesh.cpp:2647-2728: the relaxation-time shift of the nine
+// nodes around (i, j) that best reproduces the data of step t + 1 at (i, j):
+// the candidate search runs on the GPU (ippi_tau_search) over the inputs of
+// the last step's collision (ippi_get_stencil); tau += the best shift on the
+// nine nodes, and a line "t best tau_0 .. tau_8" goes to cp/BestTau.dat. i is
+// the reference's slab column (ghost column 0), so the node is x = i - 1.
+void Mesh::FindAndWriteTau(int i, int j, int t) {
+    if (!h_ || !TauField) {
+        std::cerr << "FindAndWriteTau: needs TauField = 1 and an initialised mesh" << std::endl;
+        std::exit(2);
+    }
+    const int x = i - 1, y = j;
+    const std::vector<PointData> pts = ReadXflowdata(t + 1);
+    const std::vector<int> m = MatchXflow(pts);
+    if (x < 0 || x >= nx_ || y < 0 || y >= ny_ || m[(size_t)y * nx_ + x] < 0) {
+        std::cerr << "FindAndWriteTau: no data at node (" << x << ", " << y << ")" << std::endl;
+        return;
+    }
+    const PointData& p = pts[(size_t)m[(size_t)y * nx_ + x]];
+    const int xy[2] = {x, y};
+    double fn[9], fq[9], tk[9], target[3] = {p.TotalPressure, p.ux, p.uy}, best = 0.0, err = 0.0;
+    Check(ippi_get_stencil(h_, 1, xy, fn, fq, tk), "ippi_get_stencil");
+    Check(ippi_tau_search(0, 1, fn, fq, tk, target, -10000, 10000, 1e-5, &best, &err), "ippi_tau_search");
+    static const int EXk[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1}, EYk[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    for (int k = 0; k < 9; k++) tau_[(size_t)(y + EYk[k]) * nx_ + (x + EXk[k])] += best;
+    UploadTau();
+    std::cerr << "Timestep:" << t << " " << "BestTau:" << best << std::endl;
+    std::ofstream out("cp/BestTau.dat", std::ios::app);
+    out << t << " " << best << " ";
+    for (int k = 0; k < 9; k++) out << tau_[(size_t)(y + EYk[k]) * nx_ + (x + EXk[k])] << " ";
+    out << std::endl;
 }

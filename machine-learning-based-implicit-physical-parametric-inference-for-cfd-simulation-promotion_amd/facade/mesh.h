@@ -15,6 +15,21 @@ struct Point2D {
     REAL x;
     REAL y;
 };
+// One point of the high-fidelity (XFlow) data (mesh.h:20-32): vx, vy, vz
+// are the point's coordinates (lattice units after ReadXflowdata), ux, uy,
+// uz its velocity, TotalPressure its density.
+struct PointData {
+    double VelocityModule;
+    double StaticPress;
+    double TotalPressure;
+    double viscosity;
+    double ux;
+    double uy;
+    double uz;
+    double vx;
+    double vy;
+    double vz;
+};
 
 class Mesh {
 public:
@@ -42,6 +57,12 @@ public:
     int PlateX = 400;                // IPPI_BC_FLATPLATE: 'm' / 'd' split of the bottom wall (mesh.cpp:1413)
     uint64_t Seed = 0;
     double NoiseAmplitude = 0.0;
+    // per-node relaxation times held by the facade (the reference's
+    // LatMesh[i][j].tau), needed by FindAndWriteTau and ChangeTauFromXflow
+    int TauField = 0;
+    // high-fidelity data files: XflowPrefix + (t + XflowStepOffset) + ".csv"
+    std::string XflowPrefix = "Xflowdata_pingban_every_1_step/pb_";
+    int XflowStepOffset = 10014;
 
     void InitialLatMesh(int Nx_P, int Ny, double tau, Point2D StartVel, double Rho, int RankID, int CoreNum,
                         int PartitionRemainderFlag);
@@ -58,6 +79,10 @@ public:
     void MergeAvgu(int t, int CoreNum);
     void WriteDatParticle(int t);
     void ErrorCatch(int i, int j);
+    // high-fidelity data and relaxation-time labels (mesh.cpp:2189-2766)
+    std::vector<PointData> ReadXflowdata(int t);
+    void ChangeTauFromXflow(int t);
+    void FindAndWriteTau(int i, int j, int t);
 
     // direct access for callers that read the fields back
     int Fields(std::vector<double>& rho, std::vector<double>& ux, std::vector<double>& uy);
@@ -81,4 +106,9 @@ private:
     bool have_mean_ = false;
     int Means(std::vector<double>& rho, std::vector<double>& u, std::vector<double>& v);
     int last_force_rec_ = 0;
+    std::vector<double> tau_;  // [y][x] when TauField
+    void UploadTau();
+    // index of the last data point within DX of each node (-1: none), the
+    // reference's search order (mesh.cpp:2679-2691)
+    std::vector<int> MatchXflow(const std::vector<PointData>& pts) const;
 };
