@@ -36,6 +36,7 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + MLP_NH * MLP_NIN, OFF_W2 = OFF_B1 + 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Per-lane fragments of the FP32/MFMA path (built on the host from the blob).
 // Constants are folded so that every hidden unit costs exp2 + add + rcp +
@@ -59,13 +60,32 @@ struct MlpDev {
 __device__ __forceinline__ float sig2(float t) {
     return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
-// softplus with log1p: the output is tau - 1/2, often ~1e-3, and
-// log(1 + e^z) would lose its digits to the rounding of 1 + e^z
-__device__ __forceinline__ float softplus_f(float z) { return z > 15.0f ? z : log1pf(__expf(z)); }
+// softplus: the output is tau - 1/2, often ~1e-3, and log(1 + e^z) alone
+// would lose its digits to the rounding of w = 1 + e^z. With u = e^z,
+// log1p(u) = log(w) - ((w - 1) - u) / w to first order in the rounding
+// error of w, where (w - 1) - u is that error exactly (Sterbenz). The term
+// is kept for w < 2 only, without the division: it is at most 2^-24 there,
+// and dividing it by w would change it by at most 2^-24 u / w, about half
+// an ulp of log1p(u). For w >= 2 the whole term is below 2^-24 / w, under
+// an ulp of log(w) >= 0.69. (Max relative error against log1p of the same
+// u: ~2e-7.)
+__device__ __forceinline__ float softplus_f(float z) {
+    if (z > 15.0f) return z;
+    const float u = __expf(z), w = 1.0f + u;
+    const float err = w < 2.0f ? (w - 1.0f) - u : 0.0f;
+    return __builtin_amdgcn_logf(w) * 0.69314718055994531f - err;
+}
 
-__device__ __forceinline__ void split(float v, __bf16& h, __bf16& l) {
-    h = (__bf16)v;
-    l = (__bf16)(v - (float)h);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// a, b -> hi = (bf16(a), bf16(b)), lo = (bf16(a - hi_a), bf16(b - hi_b)),
+// each pair packed in one dword (a in the low half): one conversion per
+// pair and per part, the hi values read back from the packed bits
+__device__ __forceinline__ void split2(float a, float b, unsigned& hi, unsigned& lo) {
+    hi = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+    const float ah = __uint_as_float(hi << 16), bh = __uint_as_float(hi & 0xffff0000u);
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a - ah, b - bh}, bf16x2));
 }
 
 // tau - 1/2 of this lane's node from its six raw features; every lane of the wave
@@ -82,17 +102,12 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
     const int g = l >> 4;  // lane group
     const MlpFrag& fr = m.frag[l];
     // normalised features, split hi / lo, to LDS as [node][k] (k 0..7)
-    bf16x8 vh, vl;
+    unsigned vh[4], vl[4];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float v = k < 6 ? (feat[k] - m.mu[k]) * m.isig[k] : 0.0f;
-        __bf16 h, lo;
-        split(v, h, lo);
-        vh[k] = h;
-        vl[k] = lo;
-    }
-    *reinterpret_cast<bf16x8*>(lds + 16 * l) = vh;
-    *reinterpret_cast<bf16x8*>(lds + 16 * l + 8) = vl;
+    for (int k = 0; k < 6; k += 2)
+        split2((feat[k] - m.mu[k]) * m.isig[k], (feat[k + 1] - m.mu[k + 1]) * m.isig[k + 1], vh[k / 2], vl[k / 2]);
+    *reinterpret_cast<u32x4*>(lds + 16 * l) = (u32x4){vh[0], vh[1], vh[2], 0u};
+    *reinterpret_cast<u32x4*>(lds + 16 * l + 8) = (u32x4){vl[0], vl[1], vl[2], 0u};
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     float part[4];
@@ -111,7 +126,7 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
             }
         }
         // layer 1 + activation -> layer-2 B operand (8 values per lane)
-        bf16x8 hh, hl;
+        unsigned ph[4], pl[4];
 #pragma unroll
         for (int mb = 0; mb < 2; mb++) {
             f32x4 c = {fr.b1[4 * mb], fr.b1[4 * mb + 1], fr.b1[4 * mb + 2], fr.b1[4 * mb + 3]};
@@ -119,14 +134,11 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1h[mb], bl, c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1l[mb], bh, c, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const float t = c[r];
-                __bf16 h, lo;
-                split(t * sig2(t), h, lo);
-                hh[4 * mb + r] = h;
-                hl[4 * mb + r] = lo;
-            }
+            for (int r = 0; r < 4; r += 2)
+                split2(c[r] * sig2(c[r]), c[r + 1] * sig2(c[r + 1]), ph[2 * mb + r / 2], pl[2 * mb + r / 2]);
         }
+        const bf16x8 hh = __builtin_bit_cast(bf16x8, (u32x4){ph[0], ph[1], ph[2], ph[3]});
+        const bf16x8 hl = __builtin_bit_cast(bf16x8, (u32x4){pl[0], pl[1], pl[2], pl[3]});
         // layer 2 + activation + output weights
         float acc = 0.0f;
 #pragma unroll
