@@ -378,6 +378,26 @@ def pattern_probe(ippi):
     print(json.dumps({"pattern_probe": out}), flush=True)
 
 
+def issue_probe(ippi):
+    """Cost of the instruction kinds the network variant issues, from
+    register-only loops (ippi_issue_probe): per instruction relative to an
+    FP32 FMA, and MFMA mixed with FMAs against the sum of the two alone
+    (a ratio of 1 means the two do not overlap)."""
+    us = {k: ippi.issue_probe(k) for k in ippi.ISSUE_KINDS}
+    per = {"fma": 64, "exp2": 64, "add_rcp": 64, "silu": 16, "mfma_bf16": 16, "mfma_f32": 16}
+    unit = us["fma"] / 64
+    out = {"us_per_launch": {k: round(v, 2) for k, v in us.items()},
+           "cost_in_fma": {"exp2": round(us["exp2"] / 64 / unit, 2),
+                           "rcp": round(us["add_rcp"] / 64 / unit - 1.0, 2),
+                           "silu_exp_add_rcp_mul": round(us["silu"] / 16 / unit, 2),
+                           "mfma_bf16_16x16x32": round(us["mfma_bf16"] / 16 / unit, 2),
+                           "mfma_f32_16x16x4": round(us["mfma_f32"] / 16 / unit, 2)},
+           "mixed_over_sum": {"mfma_bf16+fma": round(us["mfma_bf16+fma"] / (us["mfma_bf16"] + us["fma"]), 3),
+                              "mfma_f32+fma": round(us["mfma_f32+fma"] / (us["mfma_f32"] + us["fma"]), 3)}}
+    log(f"issue probe: {out}")
+    print(json.dumps({"issue_probe": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,6 +414,8 @@ def main():
                     help="steps of the event-timed pass used for the roofline figure")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (counter calibration)")
+    ap.add_argument("--issue-probe", action="store_true",
+                    help="print the instruction issue-cost probe and exit")
     ap.add_argument("--pattern-probe", action="store_true",
                     help="only time the fused step's memory pattern with no arithmetic (bandwidth ceiling)")
     args = ap.parse_args()
@@ -406,6 +428,9 @@ def main():
         return
     ippi = load_pkg()
     ippi.library()
+    if args.issue_probe:
+        issue_probe(ippi)
+        return
     if args.pattern_probe:
         pattern_probe(ippi)
         return

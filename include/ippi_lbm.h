@@ -259,6 +259,16 @@ int ippi_get_stencil(ippi_handle* h, int n, const int* xy, double* fnew, double*
  * mean time per launch (microseconds). */
 int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch);
 
+/* Issue-rate probe (no counterpart in the reference: measurement only). On
+ * its own buffer and stream, times `reps` launches of 4096 x 256 threads,
+ * each running 256 iterations of one instruction mix on registers: kind 0:
+ * 64 FP32 FMAs; 1: 64 exp2; 2: 64 (add, rcp); 3: 16 SiLU activations
+ * (exp2, add, rcp, mul); 4: 16 bf16 MFMAs 16x16x32; 5: kinds 4 and 0
+ * together; 6: 16 FP32 MFMAs 16x16x4; 7: kinds 6 and 0 together. The
+ * times give the cost of each kind against an FMA, and whether MFMA and
+ * VALU work overlap (kind 5 against kinds 4 + 0). */
+int ippi_issue_probe(int device, int kind, int reps, double* us_per_launch);
+
 /* Checkpoint: the full device state as a flat byte image. *size gives the
  * required size when buf is NULL. */
 int ippi_save(ippi_handle* h, void* buf, size_t* size);

@@ -88,6 +88,7 @@ _SIGS = {
     "ippi_adjoint_drag": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D],
                           ctypes.c_int),
     "ippi_get_adjoint_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "ippi_issue_probe": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, _D], ctypes.c_int),
     "ippi_pattern_probe": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D],
                            ctypes.c_int),
     "ippi_get_stencil": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _D, _D, _D], ctypes.c_int),
@@ -396,6 +397,19 @@ def pattern_probe(nx, ny, precision="fp32", mode="pull", reps=200, device=0):
                                       {"copy": 0, "pull": 1}[mode], int(reps), ctypes.byref(us))
     if rc != 0:
         raise IppiError(rc, "ippi_pattern_probe failed")
+    return us.value
+
+
+ISSUE_KINDS = ("fma", "exp2", "add_rcp", "silu", "mfma_bf16", "mfma_bf16+fma", "mfma_f32", "mfma_f32+fma")
+
+
+def issue_probe(kind, reps=10, device=0):
+    """Microseconds per launch of the issue-rate probe for one instruction
+    mix (a name of ISSUE_KINDS): 4096 x 256 threads, 256 iterations each."""
+    us = ctypes.c_double()
+    rc = library().ippi_issue_probe(int(device), ISSUE_KINDS.index(kind), int(reps), ctypes.byref(us))
+    if rc != 0:
+        raise IppiError(rc, "ippi_issue_probe failed")
     return us.value
 
 
