@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -72,6 +73,17 @@ struct StepArgs {
     const int* emk;
     const double* ecw;
     const double* F;
+    // IB inside the step kernel (k_step_f32_ib): row 0 of the grid computes
+    // the marker forces, the rows of the band [band0, band1) wait for them
+    const struct IbBlk* ib_lead;  // [body] the body's active-node range and markers
+    const int* ib_pidx;
+    const double* ib_w;
+    const double* ib_ud;
+    double* ib_ui;
+    double* ib_hist;        // [body][2] of this step's record
+    unsigned* ib_flag;      // [body] sequence number of the last step whose forces are stored
+    unsigned ib_seq;        // this step's sequence number
+    int band0, band1;
     Patch patch[IPPI_MAXB];
 };
 
@@ -358,6 +370,9 @@ __device__ __forceinline__ bool fp_state32(const float* __restrict__ s, int x, i
 
 // IB force of node (x, y) from the per-body patches (the block-level test
 // keeps the per-node test off all blocks that touch no patch).
+// COH: the marker forces were stored by another block of the same kernel
+// (k_step_f32_ib) and are read with agent-coherent loads.
+template <bool COH = false>
 __device__ __forceinline__ bool patch_force(const StepArgs& a, int x, int y, int bx0, int bx1, int by0, int by1,
                                             double& fx, double& fy) {
     for (int b = 0; b < a.nbody; ++b) {
@@ -389,8 +404,14 @@ __device__ __forceinline__ bool patch_force(const StepArgs& a, int x, int y, int
                     }
 #pragma unroll
                     for (int j = 0; j < 2; j++) {
-                        Fx[j] = a.F[2 * mk[j]];
-                        Fy[j] = a.F[2 * mk[j] + 1];
+                        if (COH) {
+                            double* Fp = const_cast<double*>(a.F) + 2 * mk[j];
+                            Fx[j] = __hip_atomic_load(Fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            Fy[j] = __hip_atomic_load(Fp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        } else {
+                            Fx[j] = a.F[2 * mk[j]];
+                            Fy[j] = a.F[2 * mk[j] + 1];
+                        }
                     }
 #pragma unroll
                     for (int j = 0; j < 2; j++) {
@@ -600,7 +621,7 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
 // never touched, so FP32 rounding cannot drift mass or momentum. RR is
 // evaluated on populations with the increment projected onto exact mass and
 // momentum. The LES stress comes from the second moments directly.
-template <int COLL, int BC, bool PULL, bool NN, bool NT>
+template <int COLL, int BC, bool PULL, bool NN, bool NT, bool COH = false>
 __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const float* __restrict__ src,
                                          const double* __restrict__ mac, float* __restrict__ dst, const StepArgs& a,
                                          bool active, __bf16* lds) {
@@ -647,7 +668,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     if (a.nbody > 0) {
         const int bx0 = blockIdx.x * BX;
         double fxd = 0.0, fyd = 0.0;
-        forced = patch_force(a, x, y, bx0, bx0 + BX - 1, y, y, fxd, fyd);
+        forced = patch_force<COH>(a, x, y, bx0, bx0 + BX - 1, y, y, fxd, fyd);
         if (forced) {
             fx = (float)fxd;
             fy = (float)fyd;
@@ -1441,6 +1462,159 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
     }
 }
 
+// ------------------------------------------------- IB inside the step kernel
+// The IB kernel's work (lead blocks, gather mode, slots) moved into the FP32
+// step kernel, so that it overlaps the lattice update instead of preceding
+// it. Grid row 0 holds one block per body, dispatched first; it computes the
+// body's marker forces from the slots the previous step kernel wrote, stores
+// them write-through at agent scope and then sets the body's flag to this
+// step's sequence number. The lattice rows follow in rotated order so that
+// the band of rows whose nodes read marker forces or write slots (the
+// bodies' regions) comes last; only those blocks wait for the flags, with a
+// bounded wait (a timeout sets the divergence flag to 2 and the step is
+// reported as failed, never left hanging). The arithmetic, the summation
+// orders and the record's tree are those of k_ib, so the results are bit for
+// bit those of the separate kernel.
+constexpr int INK_MAXACT = 2048;     // active nodes per body (LDS: 16 KB of float velocities)
+constexpr int INK_SPIN = 1 << 22;    // wait limit, in s_sleep(2) rounds (~0.2 s)
+
+__device__ void ib_in_step(const StepArgs& a, int body, float* U) {
+#pragma clang fp contract(off)
+    const IbBlk bk = a.ib_lead[body];
+    const int t = threadIdx.x;
+    const float* slot = static_cast<const float*>(a.slot);
+    // A. velocity of every active node of the body from its slots (k_ib, SLOT)
+    for (int c0 = 0; c0 < bk.nact; c0 += 2 * BX) {
+        float ux[2], uy[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int i = bk.act0 + min(c0 + j * BX + t, bk.nact - 1);
+            float f[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) f[k] = slot[(size_t)k * a.nact + i];
+            Mo o;
+            moments32(f, o);
+            ux[j] = o.ux;
+            uy[j] = o.uy;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int i = c0 + j * BX + t;
+            if (i < bk.nact) {
+                U[2 * i] = ux[j];
+                U[2 * i + 1] = uy[j];
+            }
+        }
+    }
+    __syncthreads();
+    // B. interpolation and direct forcing, markers t, t + BX, t + 2 BX, t + 3 BX
+    double F[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int ml = t + r * BX;
+        F[r][0] = 0.0;
+        F[r][1] = 0.0;
+        if (ml < bk.M) {
+            const int m = bk.m0 + ml;
+            double uix = 0, uiy = 0;
+#pragma unroll
+            for (int s4 = 0; s4 < 16; s4 += 4) {
+                int pi[4];
+                double pw[4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    pi[s] = a.ib_pidx[16 * m + s4 + s];
+                    pw[s] = a.ib_w[16 * m + s4 + s];
+                }
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    uix += ((double)U[2 * pi[s]] * pw[s]);
+                    uiy += ((double)U[2 * pi[s] + 1] * pw[s]);
+                }
+            }
+            const double Fx = 2 * (a.ib_ud[2 * m] - uix), Fy = 2 * (a.ib_ud[2 * m + 1] - uiy);
+            a.ib_ui[2 * m] = uix;
+            a.ib_ui[2 * m + 1] = uiy;
+            double* Fp = const_cast<double*>(a.F) + 2 * m;
+            __hip_atomic_store(Fp, Fx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(Fp + 1, Fy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            F[r][0] = Fx;
+            F[r][1] = Fy;
+        }
+    }
+    // every force of the block is stored before the flag
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(a.ib_flag + body, a.ib_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // record: the 512-entry tree of k_ib (entry e = F[e] + F[e + 512]), whose
+    // entries t and t + 256 this thread holds
+    double v[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        double e0 = 0.0, e1 = 0.0;
+        if (t < bk.M) e0 += F[0][c];
+        if (t + 2 * BX < bk.M) e0 += F[2][c];
+        if (t + BX < bk.M) e1 += F[1][c];
+        if (t + 3 * BX < bk.M) e1 += F[3][c];
+        v[0][c] = e0;
+        v[1][c] = e1;
+    }
+    double* R = reinterpret_cast<double*>(U);  // 2 x 256 doubles, after the barrier above
+    R[2 * t] = v[0][0] + v[1][0];  // level 256
+    R[2 * t + 1] = v[0][1] + v[1][1];
+    __syncthreads();
+    if (t < 128) {
+        R[2 * t] += R[2 * (t + 128)];
+        R[2 * t + 1] += R[2 * (t + 128) + 1];
+    }
+    __syncthreads();
+    if (t < 64) {
+        double sx = R[2 * t] + R[2 * (t + 64)], sy = R[2 * t + 1] + R[2 * (t + 64) + 1];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ox = __shfl_down(sx, off, 64), oy = __shfl_down(sy, off, 64);
+            if (t < off) {
+                sx += ox;
+                sy += oy;
+            }
+        }
+        if (t == 0) {
+            a.ib_hist[2 * bk.body] = -sx;
+            a.ib_hist[2 * bk.body + 1] = -sy;
+        }
+    }
+}
+
+template <int COLL, int BC, bool NT>
+__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(8)))
+k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const double* __restrict__ mac, StepArgs a) {
+    __shared__ float U[2 * INK_MAXACT];
+    const int nx = a.nx, ny = a.ny;
+    if (blockIdx.y == 0) {
+        if ((int)blockIdx.x < a.nbody) ib_in_step(a, blockIdx.x, U);
+        return;  // block-uniform
+    }
+    const int y = ((int)blockIdx.y - 1 + a.band1) % ny;
+    if (y >= a.band0 && y < a.band1) {  // block-uniform: wait for every body's forces of this step
+        for (int b = 0; b < a.nbody; b++) {
+            int spins = 0;
+            while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_seq) {
+                if (++spins > INK_SPIN) {
+                    *a.flag = 2;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
+    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
+    const unsigned N = (unsigned)nx * ny;
+    float cur[9];
+    load_raw<float, BC, true>(src, x, y, nx, ny, N, cur);
+    node_f32<COLL, BC, true, false, NT, true>(cur, x, y, N, src, mac, dst, a, active, nullptr);
+}
+
 // Deterministic FP64 sum (fixed block partition, fixed tree).
 __global__ void k_sum_partial(const double* __restrict__ v, size_t n, double* __restrict__ part) {
     __shared__ double sm[256];
@@ -1792,6 +1966,13 @@ struct ippi_handle {
     int nreg = 0;            // region nodes (patches grown by one node)
     bool slots_ok = false;   // d_slot holds the stored state's populations
     bool prev_pull = false;  // buf[cur ^ 1] holds the state the last (pull) step read
+    // IB inside the FP32 step kernel (k_step_f32_ib)
+    IbBlk* d_lead = nullptr;      // [body] lead block entry
+    unsigned* d_ibflag = nullptr;  // [body] sequence number of the last stored forces
+    unsigned ib_seq = 0;
+    int band0 = 0, band1 = 0;     // rows that read marker forces or write slots
+    bool ink_ok = false;          // bodies fit the in-step IB
+    bool ink_enabled = true;      // environment IPPI_IB_IN_STEP=0 turns it off
     std::vector<int> h_act;      // active node -> patch node (host copy)
     std::vector<int> body_act0;  // first active node of each body (+ total)
     double* hist = nullptr;
@@ -1833,6 +2014,7 @@ int check_flag(ippi_handle* h) {
     int v = 0;
     HIPC(h, hipMemcpyAsync(&v, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->s));
     HIPC(h, hipStreamSynchronize(h->s));
+    if (v == 2) return fail(h, IPPI_ERR_HIP, "in-step IB: the marker forces of a step were not stored in time");
     if (v) return fail(h, IPPI_ERR_DIVERGED, "density left [0, 10] or became NaN (ErrorCatch)");
     return IPPI_OK;
 }
@@ -1865,6 +2047,26 @@ StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
     IPPI_PICK(IPPI_COLL_MRT, IPPI_BC_FLATPLATE)
     IPPI_PICK(IPPI_COLL_RR, IPPI_BC_FLATPLATE)
 #undef IPPI_PICK
+    return nullptr;
+}
+
+template <int COLL, int BC>
+StepKernel<float> pick_step_ib_bc(bool nt) {
+    if (nt) return k_step_f32_ib<COLL, BC, true>;
+    return k_step_f32_ib<COLL, BC, false>;
+}
+
+// FP32 pull step with the IB inside (periodic and inflow boundaries)
+StepKernel<float> pick_step_ib(int coll, int bc, bool nt) {
+#define IPPI_PICK_IB(C, B) \
+    if (coll == C && bc == B) return pick_step_ib_bc<C, B>(nt);
+    IPPI_PICK_IB(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
+    IPPI_PICK_IB(IPPI_COLL_MRT, IPPI_BC_PERIODIC_BB)
+    IPPI_PICK_IB(IPPI_COLL_RR, IPPI_BC_PERIODIC_BB)
+    IPPI_PICK_IB(IPPI_COLL_BGK, IPPI_BC_INFLOW)
+    IPPI_PICK_IB(IPPI_COLL_MRT, IPPI_BC_INFLOW)
+    IPPI_PICK_IB(IPPI_COLL_RR, IPPI_BC_INFLOW)
+#undef IPPI_PICK_IB
     return nullptr;
 }
 
@@ -1912,6 +2114,9 @@ void free_ib(ippi_handle* h) {
     dfree(h->d_scat);
     dfree(h->d_slot);
     dfree(h->d_pslot);
+    dfree(h->d_lead);
+    dfree(h->d_ibflag);
+    h->ink_ok = false;
     h->slots_ok = false;
 }
 
@@ -2033,6 +2238,33 @@ int upload_ib(ippi_handle* h) {
         HIPC(h, hipMemcpy(h->d_pslot, gslot.data(), sizeof(int) * off, hipMemcpyHostToDevice));
         h->slots_ok = false;
     }
+    // in-step IB: one lead entry per body, the band of rows that read marker
+    // forces or write slots (the regions), and whether the bodies fit
+    {
+        std::vector<IbBlk> lead;
+        for (const auto& e : blk)
+            if (e.lead) lead.push_back(e);
+        int b0 = h->c.ny, b1 = 0;
+        bool fit = h->esz == 4 && h->c.bc != IPPI_BC_FLATPLATE && nb <= (h->c.nx + BX - 1) / BX &&
+                   (int)lead.size() == nb;
+        for (int bi = 0; bi < nb; bi++) {
+            const Patch& p = h->bodies[bi].p;
+            b0 = std::min(b0, p.y0 - 1);
+            b1 = std::max(b1, p.y0 + p.h + 1);
+            fit = fit && act0s[bi + 1] - act0s[bi] <= INK_MAXACT;
+        }
+        h->band0 = std::max(b0, 0);
+        h->band1 = std::min(b1, h->c.ny);
+        // worth it only while the band is a minority of the rows
+        h->ink_ok = fit && h->band1 > h->band0 && 2 * (h->band1 - h->band0) <= h->c.ny;
+        if (h->ink_ok) {
+            HIPC(h, hipMalloc(&h->d_lead, sizeof(IbBlk) * nb));
+            HIPC(h, hipMemcpy(h->d_lead, lead.data(), sizeof(IbBlk) * nb, hipMemcpyHostToDevice));
+            HIPC(h, hipMalloc(&h->d_ibflag, sizeof(unsigned) * nb));
+            HIPC(h, hipMemset(h->d_ibflag, 0, sizeof(unsigned) * nb));
+            h->ib_seq = 0;
+        }
+    }
     h->nrec = 0;
     h->ib_dirty = false;
     return IPPI_OK;
@@ -2147,13 +2379,42 @@ int run_steps(ippi_handle* h, int nsteps) {
                                (unsigned)h->N);
             HIPC(h, hipGetLastError());
         }
+        const bool nn = h->c.tau_mode == IPPI_TAU_NN;
+        // IB inside the step kernel: FP32 pull steps without the network whose
+        // slots the previous step kernel wrote
+        if (sizeof(T) == 4 && ib && pull && !nn && h->ink_ok && h->ink_enabled && h->slots_ok && a.scat) {
+            StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, a.nt);
+            if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
+            const int nb = (int)h->bodies.size();
+            a.ib_lead = h->d_lead;
+            a.ib_pidx = h->d_pidx;
+            a.ib_w = h->d_w;
+            a.ib_ud = h->d_ud;
+            a.ib_ui = h->d_ui;
+            a.ib_hist = h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap);
+            a.ib_flag = h->d_ibflag;
+            a.ib_seq = ++h->ib_seq;
+            if (a.ib_seq == 0) a.ib_seq = ++h->ib_seq;  // 0 is the flags' initial value
+            a.band0 = h->band0;
+            a.band1 = h->band1;
+            if (h->timing) h->ev_step.push_back(new_event(h));
+            hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, ny + 1), dim3(BX), 0, h->s, (const float*)h->buf[h->cur],
+                               (float*)h->buf[h->cur ^ 1], h->mac, a);
+            HIPC(h, hipGetLastError());
+            if (h->timing) h->ev_step.push_back(new_event(h));
+            h->nrec++;
+            h->slots_ok = true;
+            h->prev_pull = true;
+            h->cur ^= 1;
+            h->steps++;
+            continue;
+        }
         if (ib) {
             if (h->timing) h->ev_ib.push_back(new_event(h));
             int r = launch_ib<T>(h, pull);
             if (r) return r;
             if (h->timing) h->ev_ib.push_back(new_event(h));
         }
-        const bool nn = h->c.tau_mode == IPPI_TAU_NN;
         StepKernel<T> k = pull ? (nn ? pick_step<T, true, true>(h->c.collision, h->c.bc, a.nt)
                                      : pick_step<T, true, false>(h->c.collision, h->c.bc, a.nt))
                                : (nn ? pick_step<T, false, true>(h->c.collision, h->c.bc)
@@ -2421,6 +2682,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (9 * (uint64_t)cfg->nx * (uint64_t)cfg->ny >= ((uint64_t)1 << 32)) return IPPI_ERR_ARG;
     ippi_handle* h = new ippi_handle();
     h->c = *cfg;
+    if (const char* e = std::getenv("IPPI_IB_IN_STEP")) h->ink_enabled = std::strcmp(e, "0") != 0;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;
     auto bail = [&](int code) {

@@ -1,0 +1,78 @@
+"""The immersed boundary inside the FP32 step kernel (k_step_f32_ib).
+
+With bodies whose regions span at most half the rows, FP32 pull steps
+compute the marker forces (InterpolateVel / CalParticleForce,
+IPPI-LBM/mesh.cpp:514-605) in a block of the step kernel itself instead of
+a separate IB kernel before it; the lattice rows that read the forces run
+last and wait for them. The arithmetic and summation orders are the
+separate kernel's, so fields, marker state and force records must equal a
+run with the in-step path switched off (IPPI_IB_IN_STEP=0) bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+from tests import _pkg
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (nx, ny, collision, bc, les, bodies)
+    "mrt_les_inflow_1body": (256, 80, "mrt", "inflow", True, [(70.0, 40.2, 20.0, 63)]),
+    "bgk_periodic_1body": (256, 80, "bgk", "periodic_bb", False, [(90.4, 39.6, 24.0, 75)]),
+    "rr_les_inflow_2bodies": (512, 96, "rr", "inflow", True, [(60.3, 46.7, 16.0, 51), (250.6, 49.1, 20.0, 63)]),
+    "mrt_les_inflow_wide": (640, 320, "mrt", "inflow", True, [(160.2, 160.7, 80.0, 251)]),
+}
+
+
+def run(name, in_step, nsteps=120):
+    nx, ny, coll, bc, les, bodies = CASES[name]
+    ippi = _pkg.load()
+    old = os.environ.get("IPPI_IB_IN_STEP")
+    os.environ["IPPI_IB_IN_STEP"] = "1" if in_step else "0"
+    try:
+        s = ippi.Solver(nx, ny, coll, "fp32", bc, les=les, tau0=0.56, cs=0.12, u_in=(0.06, 0.0))
+    finally:
+        if old is None:
+            del os.environ["IPPI_IB_IN_STEP"]
+        else:
+            os.environ["IPPI_IB_IN_STEP"] = old
+    with s:
+        s.init_fields(21, 1.0, (0.06, 0.0), 1e-3)
+        for (cx, cy, d, m) in bodies:
+            s.add_body(cx, cy, d, m)
+        s.step(3)
+        s.set_timing(True)
+        s.step(nsteps - 3)
+        _, nstep, _, nib = s.timing()
+        s.set_timing(False)
+        return s.fields(), s.markers(), s.body_forces(), nstep, nib
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_in_step_ib_equals_separate_kernel(name):
+    print(f"[ib-in-step] {name}", flush=True)
+    a = run(name, True)
+    b = run(name, False)
+    n = 120 - 3
+    assert a[3] == n and a[4] == 0, "the in-step path did not run"
+    assert b[3] == n and b[4] == n
+    for x, y in zip(a[0], b[0]):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
+    assert np.all(np.abs(a[2][-1, :, 0]) > 0)
+
+
+def test_band_wider_than_half_keeps_separate_kernel():
+    """Bodies spread over most rows: the in-step path is not taken."""
+    ippi = _pkg.load()
+    with ippi.Solver(512, 80, "mrt", "fp32", "inflow", les=True, tau0=0.56, cs=0.12, u_in=(0.06, 0.0)) as s:
+        s.init_fields(21, 1.0, (0.06, 0.0), 1e-3)
+        for (cx, cy, d, m) in [(50.3, 20.7, 16.0, 51), (150.6, 58.1, 24.0, 75)]:
+            s.add_body(cx, cy, d, m)
+        s.step(3)
+        s.set_timing(True)
+        s.step(10)
+        _, nstep, _, nib = s.timing()
+        assert nstep == 10 and nib == 10
