@@ -76,6 +76,7 @@ struct StepArgs {
     // IB inside the step kernel (k_step_f32_ib): row 0 of the grid computes
     // the marker forces, the rows of the band [band0, band1) wait for them
     const struct IbBlk* ib_lead;  // [body] the body's active-node range and markers
+    const int* ib_act;            // active node -> patch node
     const int* ib_pidx;
     const double* ib_w;
     const double* ib_ud;
@@ -84,6 +85,7 @@ struct StepArgs {
     unsigned* ib_flag;      // [body] sequence number of the last step whose forces are stored
     unsigned ib_seq;        // this step's sequence number
     int band0, band1;
+    int rot;                // lattice row of grid row 1 (the rows follow cyclically)
     Patch patch[IPPI_MAXB];
 };
 
@@ -1463,39 +1465,48 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
 }
 
 // ------------------------------------------------- IB inside the step kernel
-// The IB kernel's work (lead blocks, gather mode, slots) moved into the FP32
-// step kernel, so that it overlaps the lattice update instead of preceding
-// it. Grid row 0 holds one block per body, dispatched first; it computes the
-// body's marker forces from the slots the previous step kernel wrote, stores
-// them write-through at agent scope and then sets the body's flag to this
-// step's sequence number. The lattice rows follow in rotated order so that
-// the band of rows whose nodes read marker forces or write slots (the
-// bodies' regions) comes last; only those blocks wait for the flags, with a
-// bounded wait (a timeout sets the divergence flag to 2 and the step is
-// reported as failed, never left hanging). The arithmetic, the summation
-// orders and the record's tree are those of k_ib, so the results are bit for
-// bit those of the separate kernel.
-constexpr int INK_MAXACT = 2048;     // active nodes per body (LDS: 16 KB of float velocities)
+// The IB kernel's work (lead blocks, gather mode) moved into the FP32 step
+// kernel, so that it overlaps the lattice update instead of preceding it.
+// Grid row 0 holds one block per body, dispatched first; it pulls the
+// post-streaming populations of the body's active nodes from the kernel's
+// input (read-only here), computes the marker forces, stores them
+// write-through at agent scope and then sets the body's flag to this step's
+// sequence number. The lattice rows follow in rotated order, which places
+// the band of rows whose nodes read marker forces (the bodies' patches) at a
+// set point of the order; only those blocks wait for the flags, with a bounded
+// wait (a timeout sets the divergence flag to 2 and the step is reported as
+// failed, never left hanging). The arithmetic, the summation orders and the
+// record's tree are those of k_ib, so the results are bit for bit those of
+// the separate kernel.
+constexpr int INK_MAXACT = 2048;     // active nodes per body, lattice-only kernel (16 KB of float velocities)
+constexpr int INK_MAXACT_NN = 3072;  // with the network (fewer resident blocks, more LDS each)
 constexpr int INK_SPIN = 1 << 22;    // wait limit, in s_sleep(2) rounds (~0.2 s)
+// Position of the band in the row order, as a fraction of the rows. Timed on
+// the 2048 x 1024 cylinder (fused step, one body) from 0.15 to 0.9 of the
+// rows: 0.15 is fastest (26.3-26.8 us per step, 27.5 us at 0.9) - the
+// blocks that wait do so while the rest of the lattice still has work. With
+// the network (4096 x 2048) the step is flat from 0.3 on and 7% slower with
+// the band first.
+constexpr double INK_BAND_AT = 0.15, INK_BAND_AT_NN = 0.3;
 
-__device__ void ib_in_step(const StepArgs& a, int body, float* U) {
+__device__ void ib_in_step(const StepArgs& a, int body, const float* __restrict__ src, float* U) {
 #pragma clang fp contract(off)
     const IbBlk bk = a.ib_lead[body];
     const int t = threadIdx.x;
-    const float* slot = static_cast<const float*>(a.slot);
-    // A. velocity of every active node of the body from its slots (k_ib, SLOT)
+    const unsigned N = (unsigned)a.nx * a.ny;
+    // A. velocity of every active node of the body (k_ib, pulling)
     for (int c0 = 0; c0 < bk.nact; c0 += 2 * BX) {
+        int nd[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) nd[j] = a.ib_act[bk.act0 + min(c0 + j * BX + t, bk.nact - 1)];
         float ux[2], uy[2];
 #pragma unroll
         for (int j = 0; j < 2; j++) {
-            const int i = bk.act0 + min(c0 + j * BX + t, bk.nact - 1);
-            float f[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++) f[k] = slot[(size_t)k * a.nact + i];
-            Mo o;
-            moments32(f, o);
-            ux[j] = o.ux;
-            uy[j] = o.uy;
+            const int r = nd[j] - bk.pad;
+            double vx, vy;
+            support_u<float, true>(src, nullptr, bk.x0 + r % bk.w, bk.y0 + r / bk.w, a.nx, N, vx, vy);
+            ux[j] = (float)vx;  // exact: the moments are FP32
+            uy[j] = (float)vy;
         }
 #pragma unroll
         for (int j = 0; j < 2; j++) {
@@ -1585,16 +1596,18 @@ __device__ void ib_in_step(const StepArgs& a, int body, float* U) {
     }
 }
 
-template <int COLL, int BC, bool NT>
-__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(8)))
+template <int COLL, int BC, bool NN, bool NT>
+__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? 4 : 8)))
 k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const double* __restrict__ mac, StepArgs a) {
-    __shared__ float U[2 * INK_MAXACT];
+    __shared__ float U[2 * (NN ? INK_MAXACT_NN : INK_MAXACT)];
+    __shared__ __attribute__((aligned(16))) __bf16 lds_mlp[NN ? BX * 16 : 1];
     const int nx = a.nx, ny = a.ny;
     if (blockIdx.y == 0) {
-        if ((int)blockIdx.x < a.nbody) ib_in_step(a, blockIdx.x, U);
+        if ((int)blockIdx.x < a.nbody) ib_in_step(a, blockIdx.x, src, U);
         return;  // block-uniform
     }
-    const int y = ((int)blockIdx.y - 1 + a.band1) % ny;
+    int y = (int)blockIdx.y - 1 + a.rot;  // rot < ny
+    if (y >= ny) y -= ny;
     if (y >= a.band0 && y < a.band1) {  // block-uniform: wait for every body's forces of this step
         for (int b = 0; b < a.nbody; b++) {
             int spins = 0;
@@ -1612,7 +1625,8 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     const unsigned N = (unsigned)nx * ny;
     float cur[9];
     load_raw<float, BC, true>(src, x, y, nx, ny, N, cur);
-    node_f32<COLL, BC, true, false, NT, true>(cur, x, y, N, src, mac, dst, a, active, nullptr);
+    __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
+    node_f32<COLL, BC, true, NN, NT, true>(cur, x, y, N, src, mac, dst, a, active, lds);
 }
 
 // Deterministic FP64 sum (fixed block partition, fixed tree).
@@ -1970,7 +1984,7 @@ struct ippi_handle {
     IbBlk* d_lead = nullptr;      // [body] lead block entry
     unsigned* d_ibflag = nullptr;  // [body] sequence number of the last stored forces
     unsigned ib_seq = 0;
-    int band0 = 0, band1 = 0;     // rows that read marker forces or write slots
+    int band0 = 0, band1 = 0;     // rows that read marker forces (the patches)
     bool ink_ok = false;          // bodies fit the in-step IB
     bool ink_enabled = true;      // environment IPPI_IB_IN_STEP=0 turns it off
     std::vector<int> h_act;      // active node -> patch node (host copy)
@@ -2051,15 +2065,15 @@ StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
 }
 
 template <int COLL, int BC>
-StepKernel<float> pick_step_ib_bc(bool nt) {
-    if (nt) return k_step_f32_ib<COLL, BC, true>;
-    return k_step_f32_ib<COLL, BC, false>;
+StepKernel<float> pick_step_ib_bc(bool nn, bool nt) {
+    if (nn) return nt ? k_step_f32_ib<COLL, BC, true, true> : k_step_f32_ib<COLL, BC, true, false>;
+    return nt ? k_step_f32_ib<COLL, BC, false, true> : k_step_f32_ib<COLL, BC, false, false>;
 }
 
 // FP32 pull step with the IB inside (periodic and inflow boundaries)
-StepKernel<float> pick_step_ib(int coll, int bc, bool nt) {
+StepKernel<float> pick_step_ib(int coll, int bc, bool nn, bool nt) {
 #define IPPI_PICK_IB(C, B) \
-    if (coll == C && bc == B) return pick_step_ib_bc<C, B>(nt);
+    if (coll == C && bc == B) return pick_step_ib_bc<C, B>(nn, nt);
     IPPI_PICK_IB(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
     IPPI_PICK_IB(IPPI_COLL_MRT, IPPI_BC_PERIODIC_BB)
     IPPI_PICK_IB(IPPI_COLL_RR, IPPI_BC_PERIODIC_BB)
@@ -2239,7 +2253,7 @@ int upload_ib(ippi_handle* h) {
         h->slots_ok = false;
     }
     // in-step IB: one lead entry per body, the band of rows that read marker
-    // forces or write slots (the regions), and whether the bodies fit
+    // forces (the patches), and whether the bodies fit
     {
         std::vector<IbBlk> lead;
         for (const auto& e : blk)
@@ -2249,9 +2263,9 @@ int upload_ib(ippi_handle* h) {
                    (int)lead.size() == nb;
         for (int bi = 0; bi < nb; bi++) {
             const Patch& p = h->bodies[bi].p;
-            b0 = std::min(b0, p.y0 - 1);
-            b1 = std::max(b1, p.y0 + p.h + 1);
-            fit = fit && act0s[bi + 1] - act0s[bi] <= INK_MAXACT;
+            b0 = std::min(b0, p.y0);
+            b1 = std::max(b1, p.y0 + p.h);
+            fit = fit && act0s[bi + 1] - act0s[bi] <= (h->c.tau_mode == IPPI_TAU_NN ? INK_MAXACT_NN : INK_MAXACT);
         }
         h->band0 = std::max(b0, 0);
         h->band1 = std::min(b1, h->c.ny);
@@ -2380,30 +2394,34 @@ int run_steps(ippi_handle* h, int nsteps) {
             HIPC(h, hipGetLastError());
         }
         const bool nn = h->c.tau_mode == IPPI_TAU_NN;
-        // IB inside the step kernel: FP32 pull steps without the network whose
-        // slots the previous step kernel wrote
-        if (sizeof(T) == 4 && ib && pull && !nn && h->ink_ok && h->ink_enabled && h->slots_ok && a.scat) {
-            StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, a.nt);
+        // IB inside the step kernel (FP32 pull steps)
+        if (sizeof(T) == 4 && ib && pull && h->ink_ok && h->ink_enabled) {
+            StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nn, a.nt);
             if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
             const int nb = (int)h->bodies.size();
-            a.ib_lead = h->d_lead;
-            a.ib_pidx = h->d_pidx;
-            a.ib_w = h->d_w;
-            a.ib_ud = h->d_ud;
-            a.ib_ui = h->d_ui;
-            a.ib_hist = h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap);
-            a.ib_flag = h->d_ibflag;
-            a.ib_seq = ++h->ib_seq;
-            if (a.ib_seq == 0) a.ib_seq = ++h->ib_seq;  // 0 is the flags' initial value
-            a.band0 = h->band0;
-            a.band1 = h->band1;
+            StepArgs ai = a;
+            ai.scat = nullptr;  // the in-step IB pulls: no slots
+            ai.ib_lead = h->d_lead;
+            ai.ib_act = h->d_act;
+            ai.ib_pidx = h->d_pidx;
+            ai.ib_w = h->d_w;
+            ai.ib_ud = h->d_ud;
+            ai.ib_ui = h->d_ui;
+            ai.ib_hist = h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap);
+            ai.ib_flag = h->d_ibflag;
+            ai.ib_seq = ++h->ib_seq;
+            if (ai.ib_seq == 0) ai.ib_seq = ++h->ib_seq;  // 0 is the flags' initial value
+            ai.band0 = h->band0;
+            ai.band1 = h->band1;
+            const int at = (int)((nn ? INK_BAND_AT_NN : INK_BAND_AT) * ny);
+            ai.rot = ((h->band0 - at) % ny + ny) % ny;
             if (h->timing) h->ev_step.push_back(new_event(h));
             hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, ny + 1), dim3(BX), 0, h->s, (const float*)h->buf[h->cur],
-                               (float*)h->buf[h->cur ^ 1], h->mac, a);
+                               (float*)h->buf[h->cur ^ 1], h->mac, ai);
             HIPC(h, hipGetLastError());
             if (h->timing) h->ev_step.push_back(new_event(h));
             h->nrec++;
-            h->slots_ok = true;
+            h->slots_ok = false;
             h->prev_pull = true;
             h->cur ^= 1;
             h->steps++;

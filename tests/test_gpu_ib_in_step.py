@@ -1,6 +1,6 @@
 """The immersed boundary inside the FP32 step kernel (k_step_f32_ib).
 
-With bodies whose regions span at most half the rows, FP32 pull steps
+With bodies whose patches span at most half the rows, FP32 pull steps
 compute the marker forces (InterpolateVel / CalParticleForce,
 IPPI-LBM/mesh.cpp:514-605) in a block of the step kernel itself instead of
 a separate IB kernel before it; the lattice rows that read the forces run
@@ -22,6 +22,8 @@ CASES = {
     "bgk_periodic_1body": (256, 80, "bgk", "periodic_bb", False, [(90.4, 39.6, 24.0, 75)]),
     "rr_les_inflow_2bodies": (512, 96, "rr", "inflow", True, [(60.3, 46.7, 16.0, 51), (250.6, 49.1, 20.0, 63)]),
     "mrt_les_inflow_wide": (640, 320, "mrt", "inflow", True, [(160.2, 160.7, 80.0, 251)]),
+    "mrt_les_inflow_network": (512, 256, "mrt", "inflow", True, [(128.3, 127.6, 60.0, 189)]),
+    "rr_network_2bodies": (512, 160, "rr", "inflow", False, [(100.2, 80.4, 30.0, 95), (300.7, 79.1, 36.0, 113)]),
 }
 
 
@@ -30,8 +32,10 @@ def run(name, in_step, nsteps=120):
     ippi = _pkg.load()
     old = os.environ.get("IPPI_IB_IN_STEP")
     os.environ["IPPI_IB_IN_STEP"] = "1" if in_step else "0"
+    nn = "network" in name
     try:
-        s = ippi.Solver(nx, ny, coll, "fp32", bc, les=les, tau0=0.56, cs=0.12, u_in=(0.06, 0.0))
+        s = ippi.Solver(nx, ny, coll, "fp32", bc, les=les, tau0=0.56, cs=0.12, u_in=(0.06, 0.0),
+                        tau_mode="nn" if nn else "const")
     finally:
         if old is None:
             del os.environ["IPPI_IB_IN_STEP"]
@@ -39,6 +43,9 @@ def run(name, in_step, nsteps=120):
             os.environ["IPPI_IB_IN_STEP"] = old
     with s:
         s.init_fields(21, 1.0, (0.06, 0.0), 1e-3)
+        if nn:
+            import ippi_amd.workloads as wl
+            s.set_mlp(wl.xavier_blob(5))
         for (cx, cy, d, m) in bodies:
             s.add_body(cx, cy, d, m)
         s.step(3)
