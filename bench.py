@@ -58,7 +58,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "machine-learning-based-implicit-physical-parametric-inference-for-cfd-simulation-promotion_amd")
 METRIC = "MLUPS + % HBM roofline, FP32/NN-tau vs plain FP64 at ≤1e-5 rel L2 velocity"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_step_f32_mrt_les_ib_2048x1024.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_step_f32_ib_mrt_les_ib_2048x1024.md")
 
 NX, NY = 2048, 1024
 U0 = 0.08
@@ -464,7 +464,8 @@ def main():
     probe_us = ippi.pattern_probe(NX, NY, "fp32", "pull", reps=300)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
-                "kernel": "k_step_f32<MRT,INFLOW,pull>", "avg_launch_us": round(k_us, 3),
+                "kernel": "k_step_f32_ib<MRT,INFLOW> (fused step, IB inside)" if ib_us == 0.0
+                          else "k_step_f32<MRT,INFLOW,pull>", "avg_launch_us": round(k_us, 3),
                 "algorithmic_bytes_per_launch": algo_bytes, "ib_kernel_avg_us": round(ib_us, 3),
                 "pattern_probe_us": round(probe_us, 3), "frac_of_pattern_probe": round(probe_us / k_us, 4)}
 

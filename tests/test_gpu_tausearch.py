@@ -57,3 +57,26 @@ def test_edges():
     assert bd.shape == (0,)
     with pytest.raises(ippi.IppiError):
         ippi.tau_search(fnew, feq, tau, target, m_lo=5, m_hi=4)
+
+
+def test_both_division_paths():
+    """Nodes whose candidate denominators tau + dtau or numerators feq leave
+    2^-200 .. 2^200 (here: tau + dtau crossing zero, a denormal feq, a huge
+    feq, a zero feq) divide plainly; the others share one refined
+    reciprocal per population. Both must give the oracle's bits, wide
+    sampling of tau and feq included."""
+    print("[tausearch] division paths", flush=True)
+    ippi = _pkg.load()
+    rng = np.random.default_rng(17)
+    n = 48
+    fnew, feq, tau = make_case(rng, n, tau0=0.55, spread=0.02)
+    tau[0, 3] = 0.04          # 0.04 + dtau crosses zero in [-0.1, 0.1]
+    feq[1, 5] = 5e-310        # denormal numerator
+    feq[2, 2] = 1e70          # huge numerator
+    feq[3, 7] = 0.0           # zero numerator (the shared path)
+    tau[4:12] = rng.uniform(0.15, 3.0, (8, 9))       # wide relaxation times
+    feq[12:20] *= 10.0 ** rng.uniform(-30, 30, (8, 9))  # wide numerators
+    target = np.stack([1.0 + rng.uniform(-0.01, 0.01, n), rng.uniform(-0.05, 0.1, n), rng.uniform(-0.03, 0.03, n)], 1)
+    bd_g, be_g = ippi.tau_search(fnew, feq, tau, target)
+    bd_o, be_o = o.tau_search(fnew, feq, tau, target)
+    assert np.array_equal(bd_g, bd_o) and np.array_equal(be_g, be_o)
