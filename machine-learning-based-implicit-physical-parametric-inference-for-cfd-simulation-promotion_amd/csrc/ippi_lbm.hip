@@ -514,7 +514,7 @@ __device__ __forceinline__ double nn_tau_f64(const double* f, const double* q, d
     return mlp_tau_f64(in, blob);
 }
 
-template <int COLL, int BC, bool PULL, bool NN, bool NT>
+template <int COLL, int BC, bool PULL, bool NN, bool NT, bool COH = false>
 __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, const double* __restrict__ src,
                                          const double* __restrict__ mac, double* __restrict__ dst, const StepArgs& a,
                                          bool active) {
@@ -548,7 +548,7 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
     bool forced = false;
     if (a.nbody > 0) {
         const int bx0 = blockIdx.x * BX;
-        forced = patch_force(a, x, y, bx0, bx0 + BX - 1, y, y, fx, fy);
+        forced = patch_force<COH>(a, x, y, bx0, bx0 + BX - 1, y, y, fx, fy);
         if (forced) {  // UpdateVelocity, mesh.cpp:1254-1255
             ux = ux + 0.5 * 1 * fx / rho;
             uy = uy + 0.5 * 1 * fy / rho;
@@ -1480,6 +1480,7 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
 // the separate kernel.
 constexpr int INK_MAXACT = 2048;     // active nodes per body, lattice-only kernel (16 KB of float velocities)
 constexpr int INK_MAXACT_NN = 3072;  // with the network (fewer resident blocks, more LDS each)
+constexpr int INK_MAXACT64 = 1280;   // FP64 (20 KB of double velocities)
 constexpr int INK_SPIN = 1 << 22;    // wait limit, in s_sleep(2) rounds (~0.2 s)
 // Position of the band in the row order, as a fraction of the rows. Timed on
 // the 2048 x 1024 cylinder (fused step, one body) from 0.15 to 0.9 of the
@@ -1489,27 +1490,31 @@ constexpr int INK_SPIN = 1 << 22;    // wait limit, in s_sleep(2) rounds (~0.2 s
 // the band first.
 constexpr double INK_BAND_AT = 0.15, INK_BAND_AT_NN = 0.3;
 
-__device__ void ib_in_step(const StepArgs& a, int body, const float* __restrict__ src, float* U) {
+template <typename T>
+__device__ void ib_in_step(const StepArgs& a, int body, const T* __restrict__ src, T* U) {
 #pragma clang fp contract(off)
     const IbBlk bk = a.ib_lead[body];
     const int t = threadIdx.x;
     const unsigned N = (unsigned)a.nx * a.ny;
-    // A. velocity of every active node of the body (k_ib, pulling)
-    for (int c0 = 0; c0 < bk.nact; c0 += 2 * BX) {
-        int nd[2];
+    // A. velocity of every active node of the body (k_ib, pulling), NA nodes
+    // in flight per lane (one in FP64, where the lattice path's registers
+    // already set the kernel's budget)
+    constexpr int NA = sizeof(T) == 8 ? 1 : 2;
+    for (int c0 = 0; c0 < bk.nact; c0 += NA * BX) {
+        int nd[NA];
 #pragma unroll
-        for (int j = 0; j < 2; j++) nd[j] = a.ib_act[bk.act0 + min(c0 + j * BX + t, bk.nact - 1)];
-        float ux[2], uy[2];
+        for (int j = 0; j < NA; j++) nd[j] = a.ib_act[bk.act0 + min(c0 + j * BX + t, bk.nact - 1)];
+        T ux[NA], uy[NA];
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
+        for (int j = 0; j < NA; j++) {
             const int r = nd[j] - bk.pad;
             double vx, vy;
-            support_u<float, true>(src, nullptr, bk.x0 + r % bk.w, bk.y0 + r / bk.w, a.nx, N, vx, vy);
-            ux[j] = (float)vx;  // exact: the moments are FP32
-            uy[j] = (float)vy;
+            support_u<T, true>(src, nullptr, bk.x0 + r % bk.w, bk.y0 + r / bk.w, a.nx, N, vx, vy);
+            ux[j] = (T)vx;  // exact: the FP32 path's moments are FP32
+            uy[j] = (T)vy;
         }
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
+        for (int j = 0; j < NA; j++) {
             const int i = c0 + j * BX + t;
             if (i < bk.nact) {
                 U[2 * i] = ux[j];
@@ -1627,6 +1632,53 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     load_raw<float, BC, true>(src, x, y, nx, ny, N, cur);
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
     node_f32<COLL, BC, true, NN, NT, true>(cur, x, y, N, src, mac, dst, a, active, lds);
+}
+
+// FP64 variant: four rows per thread as k_step_f64, the rotation and the
+// band in units of those row groups.
+template <int COLL, int BC, bool NT>
+__global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ src, double* __restrict__ dst, const double* __restrict__ mac, StepArgs a) {
+    constexpr int NPT = step_npt<double, true, false>();
+    __shared__ double U[2 * INK_MAXACT64];
+    const int nx = a.nx, ny = a.ny;
+    if (blockIdx.y == 0) {
+        if ((int)blockIdx.x < a.nbody) ib_in_step(a, blockIdx.x, src, U);
+        return;  // block-uniform
+    }
+    const int G = (ny + NPT - 1) / NPT;
+    int gi = (int)blockIdx.y - 1 + a.rot;  // rot < G
+    if (gi >= G) gi -= G;
+    const int y0 = gi * NPT;
+    if (y0 < a.band1 && y0 + NPT > a.band0) {  // block-uniform: wait for every body's forces of this step
+        for (int b = 0; b < a.nbody; b++) {
+            int spins = 0;
+            while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_seq) {
+                if (++spins > INK_SPIN) {
+                    *a.flag = 2;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
+    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
+    const unsigned N = (unsigned)nx * ny;
+    double cur[9];
+    load_raw<double, BC, true>(src, x, y0, nx, ny, N, cur);
+#pragma unroll
+    for (int j = 0; j < NPT; j++) {
+        const int y = y0 + j;
+        if (y >= ny) break;
+        const bool more = j + 1 < NPT && y + 1 < ny;
+        double nxt[9];
+        if (j + 1 < NPT) load_raw<double, BC, true>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
+        node_f64<COLL, BC, true, false, NT, true>(cur, x, y, N, src, mac, dst, a, active);
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) cur[k] = nxt[k];
+        }
+    }
 }
 
 // Deterministic FP64 sum (fixed block partition, fixed tree).
@@ -2070,6 +2122,25 @@ StepKernel<float> pick_step_ib_bc(bool nn, bool nt) {
     return nt ? k_step_f32_ib<COLL, BC, false, true> : k_step_f32_ib<COLL, BC, false, false>;
 }
 
+template <int COLL, int BC>
+StepKernel<double> pick_step_ib64_bc(bool nt) {
+    return nt ? k_step_f64_ib<COLL, BC, true> : k_step_f64_ib<COLL, BC, false>;
+}
+
+// FP64 pull step with the IB inside (no network; periodic and inflow boundaries)
+StepKernel<double> pick_step_ib64(int coll, int bc, bool nt) {
+#define IPPI_PICK_IB(C, B) \
+    if (coll == C && bc == B) return pick_step_ib64_bc<C, B>(nt);
+    IPPI_PICK_IB(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
+    IPPI_PICK_IB(IPPI_COLL_MRT, IPPI_BC_PERIODIC_BB)
+    IPPI_PICK_IB(IPPI_COLL_RR, IPPI_BC_PERIODIC_BB)
+    IPPI_PICK_IB(IPPI_COLL_BGK, IPPI_BC_INFLOW)
+    IPPI_PICK_IB(IPPI_COLL_MRT, IPPI_BC_INFLOW)
+    IPPI_PICK_IB(IPPI_COLL_RR, IPPI_BC_INFLOW)
+#undef IPPI_PICK_IB
+    return nullptr;
+}
+
 // FP32 pull step with the IB inside (periodic and inflow boundaries)
 StepKernel<float> pick_step_ib(int coll, int bc, bool nn, bool nt) {
 #define IPPI_PICK_IB(C, B) \
@@ -2259,13 +2330,15 @@ int upload_ib(ippi_handle* h) {
         for (const auto& e : blk)
             if (e.lead) lead.push_back(e);
         int b0 = h->c.ny, b1 = 0;
-        bool fit = h->esz == 4 && h->c.bc != IPPI_BC_FLATPLATE && nb <= (h->c.nx + BX - 1) / BX &&
-                   (int)lead.size() == nb;
+        const bool nnm = h->c.tau_mode == IPPI_TAU_NN;
+        bool fit = h->c.bc != IPPI_BC_FLATPLATE && nb <= (h->c.nx + BX - 1) / BX && (int)lead.size() == nb &&
+                   (h->esz == 4 || !nnm);
+        const int cap = h->esz == 8 ? INK_MAXACT64 : (nnm ? INK_MAXACT_NN : INK_MAXACT);
         for (int bi = 0; bi < nb; bi++) {
             const Patch& p = h->bodies[bi].p;
             b0 = std::min(b0, p.y0);
             b1 = std::max(b1, p.y0 + p.h);
-            fit = fit && act0s[bi + 1] - act0s[bi] <= (h->c.tau_mode == IPPI_TAU_NN ? INK_MAXACT_NN : INK_MAXACT);
+            fit = fit && act0s[bi + 1] - act0s[bi] <= cap;
         }
         h->band0 = std::max(b0, 0);
         h->band1 = std::min(b1, h->c.ny);
@@ -2394,10 +2467,8 @@ int run_steps(ippi_handle* h, int nsteps) {
             HIPC(h, hipGetLastError());
         }
         const bool nn = h->c.tau_mode == IPPI_TAU_NN;
-        // IB inside the step kernel (FP32 pull steps)
-        if (sizeof(T) == 4 && ib && pull && h->ink_ok && h->ink_enabled) {
-            StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nn, a.nt);
-            if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
+        // IB inside the step kernel (pull steps; FP64 without the network)
+        if (ib && pull && h->ink_ok && h->ink_enabled && (sizeof(T) == 4 || !nn)) {
             const int nb = (int)h->bodies.size();
             StepArgs ai = a;
             ai.scat = nullptr;  // the in-step IB pulls: no slots
@@ -2414,10 +2485,22 @@ int run_steps(ippi_handle* h, int nsteps) {
             ai.band0 = h->band0;
             ai.band1 = h->band1;
             const int at = (int)((nn ? INK_BAND_AT_NN : INK_BAND_AT) * ny);
-            ai.rot = ((h->band0 - at) % ny + ny) % ny;
             if (h->timing) h->ev_step.push_back(new_event(h));
-            hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, ny + 1), dim3(BX), 0, h->s, (const float*)h->buf[h->cur],
-                               (float*)h->buf[h->cur ^ 1], h->mac, ai);
+            if (sizeof(T) == 4) {
+                StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nn, a.nt);
+                if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
+                ai.rot = ((h->band0 - at) % ny + ny) % ny;
+                hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, ny + 1), dim3(BX), 0, h->s,
+                                   (const float*)h->buf[h->cur], (float*)h->buf[h->cur ^ 1], h->mac, ai);
+            } else {
+                StepKernel<double> k = pick_step_ib64(h->c.collision, h->c.bc, a.nt);
+                if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
+                constexpr int NPT = step_npt<double, true, false>();
+                const int G = (ny + NPT - 1) / NPT;
+                ai.rot = ((h->band0 / NPT - at / NPT) % G + G) % G;
+                hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, G + 1), dim3(BX), 0, h->s,
+                                   (const double*)h->buf[h->cur], (double*)h->buf[h->cur ^ 1], h->mac, ai);
+            }
             HIPC(h, hipGetLastError());
             if (h->timing) h->ev_step.push_back(new_event(h));
             h->nrec++;

@@ -1,10 +1,12 @@
-"""The immersed boundary inside the FP32 step kernel (k_step_f32_ib).
+"""The immersed boundary inside the step kernel (k_step_f32_ib, and
+k_step_f64_ib without the network).
 
-With bodies whose patches span at most half the rows, FP32 pull steps
+With bodies whose patches span at most half the rows, pull steps
 compute the marker forces (InterpolateVel / CalParticleForce,
 IPPI-LBM/mesh.cpp:514-605) in a block of the step kernel itself instead of
 a separate IB kernel before it; the lattice rows that read the forces run
-last and wait for them. The arithmetic and summation orders are the
+late in a rotated row order and wait for them. The arithmetic and summation
+orders are the
 separate kernel's, so fields, marker state and force records must equal a
 run with the in-step path switched off (IPPI_IB_IN_STEP=0) bit for bit."""
 import os
@@ -24,6 +26,9 @@ CASES = {
     "mrt_les_inflow_wide": (640, 320, "mrt", "inflow", True, [(160.2, 160.7, 80.0, 251)]),
     "mrt_les_inflow_network": (512, 256, "mrt", "inflow", True, [(128.3, 127.6, 60.0, 189)]),
     "rr_network_2bodies": (512, 160, "rr", "inflow", False, [(100.2, 80.4, 30.0, 95), (300.7, 79.1, 36.0, 113)]),
+    "fp64_mrt_les_inflow_1body": (256, 80, "mrt", "inflow", True, [(70.0, 40.2, 20.0, 63)]),
+    "fp64_bgk_periodic_2bodies": (512, 96, "bgk", "periodic_bb", False, [(60.3, 46.7, 16.0, 51), (250.6, 49.1, 20.0, 63)]),
+    "fp64_mrt_les_inflow_wide": (640, 322, "mrt", "inflow", True, [(160.2, 160.7, 80.0, 251)]),
 }
 
 
@@ -34,7 +39,8 @@ def run(name, in_step, nsteps=120):
     os.environ["IPPI_IB_IN_STEP"] = "1" if in_step else "0"
     nn = "network" in name
     try:
-        s = ippi.Solver(nx, ny, coll, "fp32", bc, les=les, tau0=0.56, cs=0.12, u_in=(0.06, 0.0),
+        s = ippi.Solver(nx, ny, coll, "fp64" if name.startswith("fp64") else "fp32", bc, les=les, tau0=0.56,
+                        cs=0.12, u_in=(0.06, 0.0),
                         tau_mode="nn" if nn else "const")
     finally:
         if old is None:
