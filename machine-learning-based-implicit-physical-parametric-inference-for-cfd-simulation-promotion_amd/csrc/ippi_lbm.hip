@@ -84,6 +84,7 @@ struct StepArgs {
     double* ib_hist;        // [body][2] of this step's record
     unsigned* ib_flag;      // [body] sequence number of the last step whose forces are stored
     unsigned ib_seq;        // this step's sequence number
+    unsigned ib_wait;       // the value the band waits for (ib_seq; ib_seq + 1 only to test the time-out)
     int band0, band1;
     int rot;                // lattice row of grid row 1 (the rows follow cyclically)
     Patch patch[IPPI_MAXB];
@@ -1481,7 +1482,10 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
 constexpr int INK_MAXACT = 2048;     // active nodes per body, lattice-only kernel (16 KB of float velocities)
 constexpr int INK_MAXACT_NN = 3072;  // with the network (fewer resident blocks, more LDS each)
 constexpr int INK_MAXACT64 = 1280;   // FP64 (20 KB of double velocities)
-constexpr int INK_SPIN = 1 << 22;    // wait limit, in s_sleep(2) rounds (~0.2 s)
+// wait limit in polls of the flag; each poll is a load from memory (~0.5-1
+// us) and a short sleep, so the limit is some 30-70 ms: hundreds of times
+// the longest step kernel, so only a lost flag reaches it
+constexpr int INK_SPIN = 1 << 16;
 // Position of the band in the row order, as a fraction of the rows. Timed on
 // the 2048 x 1024 cylinder (fused step, one body) from 0.15 to 0.9 of the
 // rows: 0.15 is fastest (26.3-26.8 us per step, 27.5 us at 0.9) - the
@@ -1616,7 +1620,7 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     if (y >= a.band0 && y < a.band1) {  // block-uniform: wait for every body's forces of this step
         for (int b = 0; b < a.nbody; b++) {
             int spins = 0;
-            while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_seq) {
+            while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_wait) {
                 if (++spins > INK_SPIN) {
                     *a.flag = 2;
                     break;
@@ -1652,7 +1656,7 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
     if (y0 < a.band1 && y0 + NPT > a.band0) {  // block-uniform: wait for every body's forces of this step
         for (int b = 0; b < a.nbody; b++) {
             int spins = 0;
-            while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_seq) {
+            while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_wait) {
                 if (++spins > INK_SPIN) {
                     *a.flag = 2;
                     break;
@@ -2039,6 +2043,8 @@ struct ippi_handle {
     int band0 = 0, band1 = 0;     // rows that read marker forces (the patches)
     bool ink_ok = false;          // bodies fit the in-step IB
     bool ink_enabled = true;      // environment IPPI_IB_IN_STEP=0 turns it off
+    bool ink_wait_skew = false;   // environment IPPI_IB_WAIT_SKEW=1: the band waits for a value never
+                                  // stored, so its bounded wait times out (tests of that path only)
     std::vector<int> h_act;      // active node -> patch node (host copy)
     std::vector<int> body_act0;  // first active node of each body (+ total)
     double* hist = nullptr;
@@ -2482,6 +2488,7 @@ int run_steps(ippi_handle* h, int nsteps) {
             ai.ib_flag = h->d_ibflag;
             ai.ib_seq = ++h->ib_seq;
             if (ai.ib_seq == 0) ai.ib_seq = ++h->ib_seq;  // 0 is the flags' initial value
+            ai.ib_wait = ai.ib_seq + (h->ink_wait_skew ? 1u : 0u);
             ai.band0 = h->band0;
             ai.band1 = h->band1;
             const int at = (int)((nn ? INK_BAND_AT_NN : INK_BAND_AT) * ny);
@@ -2784,6 +2791,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     ippi_handle* h = new ippi_handle();
     h->c = *cfg;
     if (const char* e = std::getenv("IPPI_IB_IN_STEP")) h->ink_enabled = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("IPPI_IB_WAIT_SKEW")) h->ink_wait_skew = std::strcmp(e, "1") == 0;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;
     auto bail = [&](int code) {

@@ -89,3 +89,24 @@ def test_band_wider_than_half_keeps_separate_kernel():
         s.step(10)
         _, nstep, _, nib = s.timing()
         assert nstep == 10 and nib == 10
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_wait_time_out_is_reported(precision):
+    """The band's wait for the marker forces is bounded: with the host
+    asking it to wait for a sequence number the IB block never stores
+    (IPPI_IB_WAIT_SKEW=1), every waiting wave gives up after its limit, the
+    kernel ends, and the next read-back reports the failure."""
+    print(f"[ib-in-step] time-out {precision}", flush=True)
+    ippi = _pkg.load()
+    os.environ["IPPI_IB_WAIT_SKEW"] = "1"
+    try:
+        s = ippi.Solver(256, 80, "mrt", precision, "inflow", les=True, tau0=0.56, cs=0.12, u_in=(0.06, 0.0))
+    finally:
+        del os.environ["IPPI_IB_WAIT_SKEW"]
+    with s:
+        s.init_fields(21, 1.0, (0.06, 0.0), 1e-3)
+        s.add_body(70.0, 40.2, 20.0, 63)
+        s.step(2)  # the first collision, then one pull step with the IB inside
+        with pytest.raises(ippi.IppiError, match="not stored in time"):
+            s.fields()
