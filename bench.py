@@ -363,13 +363,14 @@ def copy_probe(mb):
 
 def pattern_probe(ippi):
     """Bandwidth ceiling of the fused step's memory pattern: the same nine
-    pulled loads and nine stores with no arithmetic, next to a plain
-    nine-array copy, at the headline and network-leg sizes."""
+    pulled loads and nine stores with no arithmetic, in the plain and the
+    XCD-grouped block order, next to a plain nine-array copy, at the
+    headline and network-leg sizes."""
     out = {}
     for (nx, ny) in ((NX, NY), (4096, 2048)):
         for prec in ("fp32", "fp64"):
             bpn = 72 if prec == "fp32" else 144
-            for mode in ("pull", "copy"):
+            for mode in ("pull", "pull_xcd", "copy"):
                 us = ippi.pattern_probe(nx, ny, prec, mode, reps=300)
                 gbs = bpn * nx * ny / (us * 1e-6) / 1e9
                 out[f"{nx}x{ny}_{prec}_{mode}"] = {"us": round(us, 3), "gbs": round(gbs, 1),
@@ -460,14 +461,17 @@ def main():
     s32.close()
     algo_bytes = 72 * NX * NY
     achieved = algo_bytes / (k_us * 1e-6) / 1e9
-    # the same memory pattern with no arithmetic: the ceiling the kernel's traffic can reach here
-    probe_us = ippi.pattern_probe(NX, NY, "fp32", "pull", reps=300)
+    # the same memory pattern with no arithmetic, in the kernel's block order (XCD-grouped rows): the
+    # ceiling the kernel's traffic can reach here; the plain order beside it
+    probe_us = ippi.pattern_probe(NX, NY, "fp32", "pull_xcd", reps=300)
+    probe_plain_us = ippi.pattern_probe(NX, NY, "fp32", "pull", reps=300)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
                 "kernel": "k_step_f32_ib<MRT,INFLOW> (fused step, IB inside)" if ib_us == 0.0
                           else "k_step_f32<MRT,INFLOW,pull>", "avg_launch_us": round(k_us, 3),
                 "algorithmic_bytes_per_launch": algo_bytes, "ib_kernel_avg_us": round(ib_us, 3),
-                "pattern_probe_us": round(probe_us, 3), "frac_of_pattern_probe": round(probe_us / k_us, 4)}
+                "pattern_probe_us": round(probe_us, 3), "frac_of_pattern_probe": round(probe_us / k_us, 4),
+                "pattern_probe_plain_order_us": round(probe_plain_us, 3)}
 
     legs = {}
     names = [x for x in args.legs.split(",") if x and x != "none"]

@@ -87,6 +87,7 @@ struct StepArgs {
     unsigned ib_wait;       // the value the band waits for (ib_seq; ib_seq + 1 only to test the time-out)
     int band0, band1;
     int rot;                // lattice row of grid row 1 (the rows follow cyclically)
+    int xcd;                // XCD-grouped block order (xcd_block)
     Patch patch[IPPI_MAXB];
 };
 
@@ -433,15 +434,33 @@ __device__ __forceinline__ bool patch_force(const StepArgs& a, int x, int y, int
 
 constexpr int BX = 256;  // nodes per block along x (one row segment)
 
+// XCD-grouped block order. Consecutive workgroups go to the eight XCDs in
+// turn, so workgroup L runs on XCD L % 8. Giving the i-th workgroup of an
+// XCD column i % gx of row 8 (i / gx) + L % 8 puts all blocks of a row on
+// one XCD: the cache lines that neighbouring blocks' x-shifted loads share
+// are then fetched into one L2, not two. Needs a multiple of 8 rows of
+// blocks. Used while the stores are temporal (both population buffers
+// within 512 MB). Measured per step, grouped against plain order: the
+// 2048 x 1024 cylinder 25.9 against 26.4 us (FP32) and 63.3 against 65.5
+// us (FP64), 1024 x 512 FP32 9.8 against 10.3 us; with non-temporal stores
+// (4096 x 2048) grouping costs 6% (FP32 110.3 against 103.7 us) and 3%
+// with the network, so those keep the plain order.
+__device__ __forceinline__ void xcd_block(int L, int gx, int& bx, int& by) {
+    const int k = L >> 3;
+    bx = k % gx;
+    by = 8 * (k / gx) + (L & 7);
+}
+
+
 // Slots of node (x, y) for the next IB kernel: sl[k] is the slot of the
 // active node that pulls population k from here (-1: none). Loaded at the
 // start of the node's work, so the loads overlap the population loads. A
 // node lies in at most one body's region for each k that has a slot.
-__device__ __forceinline__ void ib_slot_ids(const StepArgs& a, int x, int y, int* sl) {
+__device__ __forceinline__ void ib_slot_ids(const StepArgs& a, int x, int y, int bx0, int* sl) {
 #pragma unroll
     for (int k = 0; k < 9; k++) sl[k] = -1;
     if (!a.scat) return;
-    const int bx0 = blockIdx.x * BX, bx1 = bx0 + BX - 1;
+    const int bx1 = bx0 + BX - 1;
     for (int b = 0; b < a.nbody; ++b) {
         const Patch p = a.patch[b];
         if (y < p.y0 - 1 || y > p.y0 + p.h || bx1 < p.x0 - 1 || bx0 > p.x0 + p.w) continue;
@@ -518,7 +537,7 @@ __device__ __forceinline__ double nn_tau_f64(const double* f, const double* q, d
 template <int COLL, int BC, bool PULL, bool NN, bool NT, bool COH = false>
 __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, const double* __restrict__ src,
                                          const double* __restrict__ mac, double* __restrict__ dst, const StepArgs& a,
-                                         bool active) {
+                                         bool active, int bx0) {
 #pragma clang fp contract(off)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
@@ -548,7 +567,6 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
     double fx = 0.0, fy = 0.0;
     bool forced = false;
     if (a.nbody > 0) {
-        const int bx0 = blockIdx.x * BX;
         forced = patch_force<COH>(a, x, y, bx0, bx0 + BX - 1, y, y, fx, fy);
         if (forced) {  // UpdateVelocity, mesh.cpp:1254-1255
             ux = ux + 0.5 * 1 * fx / rho;
@@ -596,9 +614,11 @@ template <int COLL, int BC, bool PULL, int NPT, bool NN, bool NT>
 __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src, double* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
     const int nx = a.nx, ny = a.ny;
-    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
-    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
-    const int y0 = blockIdx.y * NPT;
+    int bxi = blockIdx.x, byi = blockIdx.y;
+    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, bxi, byi);
+    const bool active = (int)(bxi * BX + threadIdx.x) < nx;
+    const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
+    const int y0 = byi * NPT;
     const unsigned N = (unsigned)nx * ny;
     constexpr bool RAW = !(PULL && BC == IPPI_BC_FLATPLATE);  // the flat-plate rules load for themselves
     double cur[9];
@@ -610,7 +630,7 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
         const bool more = j + 1 < NPT && y + 1 < ny;
         double nxt[9];
         if (RAW && j + 1 < NPT) load_raw<double, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f64<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active);
+        node_f64<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active, bxi * BX);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -627,12 +647,12 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
 template <int COLL, int BC, bool PULL, bool NN, bool NT, bool COH = false>
 __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const float* __restrict__ src,
                                          const double* __restrict__ mac, float* __restrict__ dst, const StepArgs& a,
-                                         bool active, __bf16* lds) {
+                                         bool active, __bf16* lds, int bx0) {
 #pragma clang fp contract(fast)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
     int sl[9];  // not in the network variants: VALU-bound, they need every register
-    if (!NN) ib_slot_ids(a, x, y, sl);
+    if (!NN) ib_slot_ids(a, x, y, bx0, sl);
     Mo o;
     bool fresh = true;  // false: flat-plate corner, stored populations copied through
     float keep[BC == IPPI_BC_FLATPLATE ? 9 : 1];
@@ -669,7 +689,6 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     float fx = 0.f, fy = 0.f;
     bool forced = false;
     if (a.nbody > 0) {
-        const int bx0 = blockIdx.x * BX;
         double fxd = 0.0, fyd = 0.0;
         forced = patch_force<COH>(a, x, y, bx0, bx0 + BX - 1, y, y, fxd, fyd);
         if (forced) {
@@ -781,9 +800,11 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
     const int nx = a.nx, ny = a.ny;
     // all lanes stay resident (the MLP's MFMAs are wave-wide); lanes past
     // the last column recompute column nx-1 and do not store
-    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
-    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
-    const int y0 = blockIdx.y * NPT;
+    int bxi = blockIdx.x, byi = blockIdx.y;
+    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, bxi, byi);
+    const bool active = (int)(bxi * BX + threadIdx.x) < nx;
+    const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
+    const int y0 = byi * NPT;
     __shared__ __bf16 lds_mlp[NN ? BX * 16 : 1];
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
     const unsigned N = (unsigned)nx * ny;
@@ -797,7 +818,7 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
         const bool more = j + 1 < NPT && y + 1 < ny;
         float nxt[9];
         if (RAW && j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f32<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active, lds);
+        node_f32<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -845,13 +866,16 @@ __global__ void k_stencil(const T* __restrict__ prev, const int* __restrict__ xy
 // does (shifted rows and columns, SGPR base + 32-bit offsets) and stores
 // them with the step's store rule; MODE 0 copies them in place. The time
 // per launch is the ceiling the fused kernel's memory traffic can reach.
+// MODE 2 is MODE 1 with the XCD-grouped block order (ny % 8 == 0).
 template <typename T, int MODE, bool NT>
 __global__ void __launch_bounds__(BX) k_probe(const T* __restrict__ src, T* __restrict__ dst, int nx, int ny) {
-    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
-    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1), y = blockIdx.y;
+    int bx = blockIdx.x, y = blockIdx.y;
+    if (MODE == 2) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, bx, y);
+    const bool active = (int)(bx * BX + threadIdx.x) < nx;
+    const int x = min((int)(bx * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
     T f[9];
-    if (MODE == 1) {
+    if (MODE >= 1) {
         load_raw<T, IPPI_BC_INFLOW, true>(src, x, y, nx, ny, N, f);
     } else {
 #pragma unroll
@@ -1615,7 +1639,9 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
         if ((int)blockIdx.x < a.nbody) ib_in_step(a, blockIdx.x, src, U);
         return;  // block-uniform
     }
-    int y = (int)blockIdx.y - 1 + a.rot;  // rot < ny
+    int bxi = blockIdx.x, g = (int)blockIdx.y - 1;
+    if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, bxi, g);
+    int y = g + a.rot;  // rot < ny
     if (y >= ny) y -= ny;
     if (y >= a.band0 && y < a.band1) {  // block-uniform: wait for every body's forces of this step
         for (int b = 0; b < a.nbody; b++) {
@@ -1629,13 +1655,13 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
             }
         }
     }
-    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
-    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
+    const bool active = (int)(bxi * BX + threadIdx.x) < nx;
+    const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny;
     float cur[9];
     load_raw<float, BC, true>(src, x, y, nx, ny, N, cur);
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
-    node_f32<COLL, BC, true, NN, NT, true>(cur, x, y, N, src, mac, dst, a, active, lds);
+    node_f32<COLL, BC, true, NN, NT, true>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX);
 }
 
 // FP64 variant: four rows per thread as k_step_f64, the rotation and the
@@ -1650,7 +1676,9 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
         return;  // block-uniform
     }
     const int G = (ny + NPT - 1) / NPT;
-    int gi = (int)blockIdx.y - 1 + a.rot;  // rot < G
+    int bxi = blockIdx.x, g = (int)blockIdx.y - 1;
+    if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, bxi, g);
+    int gi = g + a.rot;  // rot < G
     if (gi >= G) gi -= G;
     const int y0 = gi * NPT;
     if (y0 < a.band1 && y0 + NPT > a.band0) {  // block-uniform: wait for every body's forces of this step
@@ -1665,8 +1693,8 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
             }
         }
     }
-    const bool active = (int)(blockIdx.x * BX + threadIdx.x) < nx;
-    const int x = min((int)(blockIdx.x * BX + threadIdx.x), nx - 1);
+    const bool active = (int)(bxi * BX + threadIdx.x) < nx;
+    const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny;
     double cur[9];
     load_raw<double, BC, true>(src, x, y0, nx, ny, N, cur);
@@ -1677,7 +1705,7 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
         const bool more = j + 1 < NPT && y + 1 < ny;
         double nxt[9];
         if (j + 1 < NPT) load_raw<double, BC, true>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f64<COLL, BC, true, false, NT, true>(cur, x, y, N, src, mac, dst, a, active);
+        node_f64<COLL, BC, true, false, NT, true>(cur, x, y, N, src, mac, dst, a, active, bxi * BX);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -2497,6 +2525,7 @@ int run_steps(ippi_handle* h, int nsteps) {
                 StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nn, a.nt);
                 if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
                 ai.rot = ((h->band0 - at) % ny + ny) % ny;
+                ai.xcd = ny % 8 == 0 && !a.nt;
                 hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, ny + 1), dim3(BX), 0, h->s,
                                    (const float*)h->buf[h->cur], (float*)h->buf[h->cur ^ 1], h->mac, ai);
             } else {
@@ -2505,6 +2534,7 @@ int run_steps(ippi_handle* h, int nsteps) {
                 constexpr int NPT = step_npt<double, true, false>();
                 const int G = (ny + NPT - 1) / NPT;
                 ai.rot = ((h->band0 / NPT - at / NPT) % G + G) % G;
+                ai.xcd = G % 8 == 0 && !a.nt;
                 hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, G + 1), dim3(BX), 0, h->s,
                                    (const double*)h->buf[h->cur], (double*)h->buf[h->cur ^ 1], h->mac, ai);
             }
@@ -2528,6 +2558,8 @@ int run_steps(ippi_handle* h, int nsteps) {
                                : (nn ? pick_step<T, false, true>(h->c.collision, h->c.bc)
                                      : pick_step<T, false, false>(h->c.collision, h->c.bc));
         if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
+        // XCD-grouped block order while the stores stay temporal
+        a.xcd = (pull ? grid_pull.y : grid_first.y) % 8 == 0 && !(pull && a.nt);
         if (h->timing && pull) h->ev_step.push_back(new_event(h));
         hipLaunchKernelGGL(k, pull ? grid_pull : grid_first, dim3(BX), 0, h->s, (const T*)h->buf[h->cur],
                            (T*)h->buf[h->cur ^ 1], h->mac, a);
@@ -3132,7 +3164,8 @@ int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, dou
 }
 
 int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch) {
-    if (nx < 4 || ny < 4 || precision < 0 || precision > 1 || mode < 0 || mode > 1 || reps < 1 || !us_per_launch)
+    if (nx < 4 || ny < 4 || precision < 0 || precision > 1 || mode < 0 || mode > 2 || (mode == 2 && ny % 8) ||
+        reps < 1 || !us_per_launch)
         return IPPI_ERR_ARG;
     if (9 * (uint64_t)nx * (uint64_t)ny >= ((uint64_t)1 << 32)) return IPPI_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return IPPI_ERR_HIP;
@@ -3157,17 +3190,19 @@ int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int 
             void* d0 = b[(i & 1) ^ 1];
 #define IPPI_PROBE(TT, M, NTV) \
     hipLaunchKernelGGL((k_probe<TT, M, NTV>), g, dim3(BX), 0, st, (const TT*)s0, (TT*)d0, nx, ny)
+#define IPPI_PROBE_T(TT)                               \
+    if (mode == 2 && nt) IPPI_PROBE(TT, 2, true);      \
+    else if (mode == 2) IPPI_PROBE(TT, 2, false);      \
+    else if (mode == 1 && nt) IPPI_PROBE(TT, 1, true); \
+    else if (mode == 1) IPPI_PROBE(TT, 1, false);      \
+    else if (nt) IPPI_PROBE(TT, 0, true);              \
+    else IPPI_PROBE(TT, 0, false);
             if (esz == 8) {
-                if (mode && nt) IPPI_PROBE(double, 1, true);
-                else if (mode) IPPI_PROBE(double, 1, false);
-                else if (nt) IPPI_PROBE(double, 0, true);
-                else IPPI_PROBE(double, 0, false);
+                IPPI_PROBE_T(double)
             } else {
-                if (mode && nt) IPPI_PROBE(float, 1, true);
-                else if (mode) IPPI_PROBE(float, 1, false);
-                else if (nt) IPPI_PROBE(float, 0, true);
-                else IPPI_PROBE(float, 0, false);
+                IPPI_PROBE_T(float)
             }
+#undef IPPI_PROBE_T
 #undef IPPI_PROBE
         };
         for (int i = 0; i < 10; i++) launch(i);  // warm-up
