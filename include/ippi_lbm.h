@@ -176,11 +176,14 @@ int ippi_trainer_loss_grad(ippi_trainer* t, double* loss, double* grad);
 /* iters Adam steps on minibatches of `batch` samples drawn with splitmix64. */
 int ippi_trainer_adam(ippi_trainer* t, int iters, int batch, double lr, uint64_t seed, double* loss);
 
-/* Kernel timing (no counterpart in the reference: measurement only). While
- * enabled, every fused stream-collide launch and every immersed-boundary
- * kernel launch is bracketed by HIP events on the handle's stream;
- * ippi_get_timing synchronises, returns the summed durations (ms) and
- * launch counts since the last call, and clears them. */
+/* Kernel timing (no counterpart in the reference: measurement only).
+ * enable = 1: every fused stream-collide launch and every immersed-boundary
+ * kernel launch is bracketed by HIP events on the handle's stream.
+ * enable = 2: one event pair brackets all launches of each ippi_step call
+ * (back to back, as untimed); step_ms is then the calls' time, the launch
+ * counts those of the step and IB kernels inside, and ib_ms 0.
+ * enable = 0: off. ippi_get_timing synchronises, returns the summed
+ * durations (ms) and launch counts since the last call, and clears them. */
 int ippi_set_timing(ippi_handle* h, int enable);
 int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, double* ib_ms,
                     int64_t* ib_launches);

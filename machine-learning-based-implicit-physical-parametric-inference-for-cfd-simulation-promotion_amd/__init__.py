@@ -270,7 +270,10 @@ class Solver:
         return m.value
 
     def set_timing(self, enable):
-        self._check(library().ippi_set_timing(self.h, int(bool(enable))))
+        """False/0: off; True/1: HIP events around every kernel launch;
+        "calls"/2: one event pair around the launches of each step() call."""
+        mode = 2 if enable == "calls" else int(enable)
+        self._check(library().ippi_set_timing(self.h, mode))
 
     def timing(self):
         """(fused-kernel ms, launches, IB-kernels ms, IB launches) since the last call."""

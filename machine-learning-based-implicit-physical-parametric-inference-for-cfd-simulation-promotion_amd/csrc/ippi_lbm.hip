@@ -2084,7 +2084,8 @@ struct ippi_handle {
     double* d_blob = nullptr;
     float mlp_mu[6] = {0}, mlp_isig[6] = {0}, mlp_b3 = 0.f;
     // optional per-launch timing (HIP events on this handle's stream)
-    int timing = 0;
+    int timing = 0;                      // 1: events around each launch; 2: around each ippi_step call
+    int64_t batch_steps = 0, batch_ib = 0;  // kernels launched inside the timed calls (mode 2)
     std::vector<hipEvent_t> ev_step, ev_ib, ev_adj;  // start/end pairs
     std::string err;
 };
@@ -2465,8 +2466,21 @@ int sum_events(ippi_handle* h, std::vector<hipEvent_t>& v, double* ms, int64_t* 
     return IPPI_OK;
 }
 
+// Timing mode 2: one event pair around the launches of one ippi_step call
+// (also when the call returns early with an error).
+struct CallTimer {
+    ippi_handle* h;
+    explicit CallTimer(ippi_handle* hh) : h(hh) {
+        if (h->timing == 2) h->ev_step.push_back(new_event(h));
+    }
+    ~CallTimer() {
+        if (h->timing == 2) h->ev_step.push_back(new_event(h));
+    }
+};
+
 template <typename T>
 int run_steps(ippi_handle* h, int nsteps) {
+    CallTimer call_timer(h);
     const int nx = h->c.nx, ny = h->c.ny;
     const bool nn_mode = h->c.tau_mode == IPPI_TAU_NN;
     const int npt = nn_mode ? step_npt<T, true, true>() : step_npt<T, true, false>();
@@ -2520,7 +2534,7 @@ int run_steps(ippi_handle* h, int nsteps) {
             ai.band0 = h->band0;
             ai.band1 = h->band1;
             const int at = (int)((nn ? INK_BAND_AT_NN : INK_BAND_AT) * ny);
-            if (h->timing) h->ev_step.push_back(new_event(h));
+            if (h->timing == 1) h->ev_step.push_back(new_event(h));
             if (sizeof(T) == 4) {
                 StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nn, a.nt);
                 if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
@@ -2539,7 +2553,8 @@ int run_steps(ippi_handle* h, int nsteps) {
                                    (const double*)h->buf[h->cur], (double*)h->buf[h->cur ^ 1], h->mac, ai);
             }
             HIPC(h, hipGetLastError());
-            if (h->timing) h->ev_step.push_back(new_event(h));
+            if (h->timing == 1) h->ev_step.push_back(new_event(h));
+            h->batch_steps++;
             h->nrec++;
             h->slots_ok = false;
             h->prev_pull = true;
@@ -2548,10 +2563,11 @@ int run_steps(ippi_handle* h, int nsteps) {
             continue;
         }
         if (ib) {
-            if (h->timing) h->ev_ib.push_back(new_event(h));
+            if (h->timing == 1) h->ev_ib.push_back(new_event(h));
             int r = launch_ib<T>(h, pull);
             if (r) return r;
-            if (h->timing) h->ev_ib.push_back(new_event(h));
+            if (h->timing == 1) h->ev_ib.push_back(new_event(h));
+            h->batch_ib++;
         }
         StepKernel<T> k = pull ? (nn ? pick_step<T, true, true>(h->c.collision, h->c.bc, a.nt)
                                      : pick_step<T, true, false>(h->c.collision, h->c.bc, a.nt))
@@ -2560,11 +2576,12 @@ int run_steps(ippi_handle* h, int nsteps) {
         if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
         // XCD-grouped block order while the stores stay temporal
         a.xcd = (pull ? grid_pull.y : grid_first.y) % 8 == 0 && !(pull && a.nt);
-        if (h->timing && pull) h->ev_step.push_back(new_event(h));
+        if (h->timing == 1 && pull) h->ev_step.push_back(new_event(h));
         hipLaunchKernelGGL(k, pull ? grid_pull : grid_first, dim3(BX), 0, h->s, (const T*)h->buf[h->cur],
                            (T*)h->buf[h->cur ^ 1], h->mac, a);
         HIPC(h, hipGetLastError());
-        if (h->timing && pull) h->ev_step.push_back(new_event(h));
+        if (h->timing == 1 && pull) h->ev_step.push_back(new_event(h));
+        h->batch_steps++;
         h->slots_ok = a.scat != nullptr;
         h->prev_pull = pull;
         h->cur ^= 1;
@@ -3151,7 +3168,10 @@ int ippi_set_timing(ippi_handle* h, int enable) {
     clear_events(h->ev_step);
     clear_events(h->ev_ib);
     clear_events(h->ev_adj);
-    h->timing = enable ? 1 : 0;
+    if (enable < 0 || enable > 2) return IPPI_ERR_ARG;
+    h->timing = enable;
+    h->batch_steps = 0;
+    h->batch_ib = 0;
     return IPPI_OK;
 }
 
@@ -3160,7 +3180,15 @@ int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, dou
     HIPC(h, hipStreamSynchronize(h->s));
     int r = sum_events(h, h->ev_step, step_ms, step_launches);
     if (r) return r;
-    return sum_events(h, h->ev_ib, ib_ms, ib_launches);
+    r = sum_events(h, h->ev_ib, ib_ms, ib_launches);
+    if (r) return r;
+    if (h->timing == 2) {  // the calls' time over the step kernels they launched
+        if (step_launches) *step_launches = h->batch_steps;
+        if (ib_launches) *ib_launches = h->batch_ib;
+        h->batch_steps = 0;
+        h->batch_ib = 0;
+    }
+    return IPPI_OK;
 }
 
 int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch) {
