@@ -183,13 +183,15 @@ def leg_nn(ippi, steps, warmup, profile_steps):
         s.add_body(nx / 4, ny / 2, D, workloads.markers_for(D))
         s.set_mlp(blob)
         log(f"nn 4096x2048 {prec}: {steps} steps")
-        dt = timed_run(s, steps, warmup)
+        dt, k_timed = timed_run_events(s, steps, warmup)
         res[prec] = (nx * ny * steps / dt / 1e6, s.fields(), s.tau())
         if prec == "fp32":
-            k_us, ib_us = kernel_avg_us(s, profile_steps)
+            k_each, ib_us = kernel_avg_us(s, profile_steps)
+            k_us = k_timed if k_timed is not None else k_each
             gbs = 72 * nx * ny / (k_us * 1e-6) / 1e9
-            out.update({"fused_kernel_avg_us": round(k_us, 2), "fused_kernel_gbs": round(gbs, 1),
-                        "fused_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "ib_kernel_avg_us": round(ib_us, 2)})
+            out.update({"fused_kernel_avg_us": round(k_us, 2), "fused_kernel_avg_us_timed_individually": round(k_each, 2),
+                        "fused_kernel_gbs": round(gbs, 1), "fused_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "ib_kernel_avg_us": round(ib_us, 2)})
         s.close()
     t32, t64 = res["fp32"][2], res["fp64"][2]
     out.update({"mlups_fp32": round(res["fp32"][0], 1), "mlups_fp64": round(res["fp64"][0], 1),
