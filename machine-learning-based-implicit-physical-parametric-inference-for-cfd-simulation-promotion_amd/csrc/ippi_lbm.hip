@@ -1773,6 +1773,7 @@ struct AdjArgs {
     const double* patch_l;  // velocity adjoint from the interpolation [patch node][2]
     int nbody;
     int store;              // write the adjoint populations
+    int xcd;                // XCD-grouped block order (xcd_block)
     Patch patch[IPPI_MAXB];
 };
 
@@ -1978,7 +1979,9 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
                                                  const S* __restrict__ src, const double* __restrict__ mac,
                                                  A* __restrict__ gacc, AdjArgs a) {
     const int nx = a.nx, ny = a.ny;
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y;
+    int bxi = blockIdx.x, y = blockIdx.y;
+    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, bxi, y);
+    const int x = bxi * BX + threadIdx.x;
     if (x >= nx) return;
     const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
     A l[9];
@@ -1989,7 +1992,7 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
     for (int k = 0; k < 9; k++) l[k] -= c0;
     adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, f, drho, rho, ux, uy);
     A fx = A(0), fy = A(0), lix = A(0), liy = A(0);
-    const int bx0 = blockIdx.x * BX;
+    const int bx0 = bxi * BX;
     const int r = a.nbody > 0 ? adj_patch(a, x, y, bx0, bx0 + BX - 1) : -1;
     const bool forced = r >= 0;
     if (forced) {
@@ -2761,6 +2764,8 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     aa.tau_ptr = h->tau;
     aa.patch_l = pl;
     aa.nbody = nb;
+    // XCD-grouped rows while the two adjoint buffers stay within 512 MB (temporal stores)
+    aa.xcd = ny % 8 == 0 && 2 * 9 * h->N * sizeof(A) <= ((size_t)512 << 20);
     for (int b = 0; b < nb; b++) aa.patch[b] = h->bodies[b].p;
     AdjIb ad{h->nact, h->ell, h->d_act, dan, lf, uact, h->d_pidx, h->d_w, dwx, dwy, nullptr,
              h->d_ecnt, h->d_emk, h->d_ecw, dtab, pl, gc};
