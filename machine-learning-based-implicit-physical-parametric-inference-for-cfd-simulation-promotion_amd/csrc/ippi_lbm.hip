@@ -1895,13 +1895,28 @@ __global__ void __launch_bounds__(256) k_adj_act(const A* __restrict__ lin, cons
 // objective. One lane per (marker, support node) pair, in chunks of IB_T
 // pairs (32 markers) held in flight together; the sum over a marker's 16
 // supports is a shuffle tree inside its 16-lane group, the centre gradient
-// a fixed-order block tree, so the result is deterministic.
+// a shuffle tree per wave and the wave sums in wave order, so the result is
+// deterministic. The gather lists of each lane's first support node are
+// loaded at the start (all E entries: padding entries have weight 0 and
+// add nothing), so the gather at the end reads only LDS.
 constexpr int ADJ_CH = 4;  // chunks in flight per lane (IB_T * ADJ_CH / 16 = 128 markers)
 __global__ void __launch_bounds__(IB_T) k_adj_mk(AdjIb d, double cj) {
     __shared__ double LU[2 * IB_MAXM];
     __shared__ double red[2 * IB_T];
     const AdjBody b = d.body[blockIdx.x];
     const int t = threadIdx.x;
+    // gather list of this lane's first support node (E = 8 lists only)
+    int gmk[8];
+    double gw[8];
+    const bool pre = d.E == 8 && t < b.nact;
+    if (pre) {
+        const int i = b.a0 + t;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            gmk[e] = d.emk[8 * (size_t)i + e];
+            gw[e] = d.ecw[8 * (size_t)i + e];
+        }
+    }
     double gx = 0.0, gy = 0.0;
     for (int c0 = 0; c0 < 16 * b.M; c0 += ADJ_CH * IB_T) {
         double w[ADJ_CH], wxs[ADJ_CH], wys[ADJ_CH], lfx[ADJ_CH], lfy[ADJ_CH], ux[ADJ_CH], uy[ADJ_CH];
@@ -1943,22 +1958,38 @@ __global__ void __launch_bounds__(IB_T) k_adj_mk(AdjIb d, double cj) {
             gy += wys[j] * cc;
         }
     }
-    red[2 * t] = gx;
-    red[2 * t + 1] = gy;
-    __syncthreads();
-    for (int off = IB_T / 2; off > 0; off >>= 1) {
-        if (t < off) {
-            red[2 * t] += red[2 * (t + off)];
-            red[2 * t + 1] += red[2 * (t + off) + 1];
-        }
-        __syncthreads();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        gx += __shfl_down(gx, off, 64);
+        gy += __shfl_down(gy, off, 64);
     }
+    if ((t & 63) == 0) {
+        red[2 * (t >> 6)] = gx;
+        red[2 * (t >> 6) + 1] = gy;
+    }
+    __syncthreads();  // also orders the LU stores before the gather
     if (t == 0) {
-        d.gc[2 * blockIdx.x] += red[0];
-        d.gc[2 * blockIdx.x + 1] += red[1];
+        double sx = 0.0, sy = 0.0;
+        for (int w = 0; w < IB_T / 64; w++) {
+            sx += red[2 * w];
+            sy += red[2 * w + 1];
+        }
+        d.gc[2 * blockIdx.x] += sx;
+        d.gc[2 * blockIdx.x + 1] += sy;
     }
     // velocity adjoint at the support nodes: the spread's gather lists
-    for (int il = t; il < b.nact; il += IB_T) {
+    if (pre) {
+        double sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            sx += gw[e] * LU[2 * gmk[e]];
+            sy += gw[e] * LU[2 * gmk[e] + 1];
+        }
+        const int r = d.act[b.a0 + t];
+        d.patch_l[2 * r] = sx;
+        d.patch_l[2 * r + 1] = sy;
+    }
+    for (int il = pre ? t + IB_T : t; il < b.nact; il += IB_T) {
         const int i = b.a0 + il;
         const int cnt = d.ecnt[i];
         double sx = 0.0, sy = 0.0;
