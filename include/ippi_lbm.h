@@ -258,7 +258,7 @@ int ippi_get_stencil(ippi_handle* h, int n, const int* xy, double* fnew, double*
  * its own buffers and stream, times `reps` launches of a kernel with the
  * fused step's memory pattern and no arithmetic on an nx x ny lattice:
  * mode 1 = the nine pulled loads of an inflow step and nine stores, mode 2
- * = mode 1 with the rows grouped by XCD (ny a multiple of 8), mode 0 = a
+ * = mode 1 with the rows grouped by XCD, mode 0 = a
  * nine-array copy; precision IPPI_FP64 or IPPI_FP32_SHIFTED. Writes the
  * mean time per launch (microseconds). */
 int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch);

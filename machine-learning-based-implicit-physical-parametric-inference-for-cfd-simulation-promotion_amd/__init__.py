@@ -394,8 +394,8 @@ def drag_coefficient(force_x, u_ref, radius):
 def pattern_probe(nx, ny, precision="fp32", mode="pull", reps=200, device=0):
     """Microseconds per launch of a kernel with the fused step's memory
     pattern and no arithmetic (mode "pull": the nine shifted loads of an
-    inflow step; "pull_xcd": the same with the rows grouped by XCD, ny a
-    multiple of 8; "copy": a nine-array copy) on an nx x ny lattice."""
+    inflow step; "pull_xcd": the same with the rows grouped by XCD;
+    "copy": a nine-array copy) on an nx x ny lattice."""
     us = ctypes.c_double()
     rc = library().ippi_pattern_probe(int(device), int(nx), int(ny), PRECISION[precision],
                                       {"copy": 0, "pull": 1, "pull_xcd": 2}[mode], int(reps), ctypes.byref(us))

@@ -438,14 +438,21 @@ constexpr int BX = 256;  // nodes per block along x (one row segment)
 // turn, so workgroup L runs on XCD L % 8. Giving the i-th workgroup of an
 // XCD column i % gx of row 8 (i / gx) + L % 8 puts all blocks of a row on
 // one XCD: the cache lines that neighbouring blocks' x-shifted loads share
-// are then fetched into one L2, not two. Needs a multiple of 8 rows of
-// blocks. Used while the stores are temporal (both population buffers
+// are then fetched into one L2, not two. With G rows of blocks, the rows
+// past the last whole group of 8 follow in plain order. Used while the
+// stores are temporal (both population buffers
 // within 512 MB). Measured per step, grouped against plain order: the
 // 2048 x 1024 cylinder 25.9 against 26.4 us (FP32) and 63.3 against 65.5
 // us (FP64), 1024 x 512 FP32 9.8 against 10.3 us; with non-temporal stores
 // (4096 x 2048) grouping costs 6% (FP32 110.3 against 103.7 us) and 3%
 // with the network, so those keep the plain order.
-__device__ __forceinline__ void xcd_block(int L, int gx, int& bx, int& by) {
+__device__ __forceinline__ void xcd_block(int L, int gx, int G, int& bx, int& by) {
+    const int full = gx * (G & ~7);  // blocks in whole groups of 8 rows
+    if (L >= full) {
+        bx = (L - full) % gx;
+        by = (G & ~7) + (L - full) / gx;
+        return;
+    }
     const int k = L >> 3;
     bx = k % gx;
     by = 8 * (k / gx) + (L & 7);
@@ -615,7 +622,7 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
                                                  const double* __restrict__ mac, StepArgs a) {
     const int nx = a.nx, ny = a.ny;
     int bxi = blockIdx.x, byi = blockIdx.y;
-    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, bxi, byi);
+    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, bxi, byi);
     const bool active = (int)(bxi * BX + threadIdx.x) < nx;
     const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const int y0 = byi * NPT;
@@ -801,7 +808,7 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
     // all lanes stay resident (the MLP's MFMAs are wave-wide); lanes past
     // the last column recompute column nx-1 and do not store
     int bxi = blockIdx.x, byi = blockIdx.y;
-    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, bxi, byi);
+    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, bxi, byi);
     const bool active = (int)(bxi * BX + threadIdx.x) < nx;
     const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const int y0 = byi * NPT;
@@ -866,11 +873,11 @@ __global__ void k_stencil(const T* __restrict__ prev, const int* __restrict__ xy
 // does (shifted rows and columns, SGPR base + 32-bit offsets) and stores
 // them with the step's store rule; MODE 0 copies them in place. The time
 // per launch is the ceiling the fused kernel's memory traffic can reach.
-// MODE 2 is MODE 1 with the XCD-grouped block order (ny % 8 == 0).
+// MODE 2 is MODE 1 with the XCD-grouped block order.
 template <typename T, int MODE, bool NT>
 __global__ void __launch_bounds__(BX) k_probe(const T* __restrict__ src, T* __restrict__ dst, int nx, int ny) {
     int bx = blockIdx.x, y = blockIdx.y;
-    if (MODE == 2) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, bx, y);
+    if (MODE == 2) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, bx, y);
     const bool active = (int)(bx * BX + threadIdx.x) < nx;
     const int x = min((int)(bx * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
@@ -1640,7 +1647,7 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
         return;  // block-uniform
     }
     int bxi = blockIdx.x, g = (int)blockIdx.y - 1;
-    if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, bxi, g);
+    if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, ny, bxi, g);
     int y = g + a.rot;  // rot < ny
     if (y >= ny) y -= ny;
     if (y >= a.band0 && y < a.band1) {  // block-uniform: wait for every body's forces of this step
@@ -1677,7 +1684,7 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
     }
     const int G = (ny + NPT - 1) / NPT;
     int bxi = blockIdx.x, g = (int)blockIdx.y - 1;
-    if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, bxi, g);
+    if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, G, bxi, g);
     int gi = g + a.rot;  // rot < G
     if (gi >= G) gi -= G;
     const int y0 = gi * NPT;
@@ -2011,7 +2018,7 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
                                                  A* __restrict__ gacc, AdjArgs a) {
     const int nx = a.nx, ny = a.ny;
     int bxi = blockIdx.x, y = blockIdx.y;
-    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, bxi, y);
+    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, bxi, y);
     const int x = bxi * BX + threadIdx.x;
     if (x >= nx) return;
     const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
@@ -2573,7 +2580,7 @@ int run_steps(ippi_handle* h, int nsteps) {
                 StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nn, a.nt);
                 if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
                 ai.rot = ((h->band0 - at) % ny + ny) % ny;
-                ai.xcd = ny % 8 == 0 && !a.nt;
+                ai.xcd = !a.nt;
                 hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, ny + 1), dim3(BX), 0, h->s,
                                    (const float*)h->buf[h->cur], (float*)h->buf[h->cur ^ 1], h->mac, ai);
             } else {
@@ -2582,7 +2589,7 @@ int run_steps(ippi_handle* h, int nsteps) {
                 constexpr int NPT = step_npt<double, true, false>();
                 const int G = (ny + NPT - 1) / NPT;
                 ai.rot = ((h->band0 / NPT - at / NPT) % G + G) % G;
-                ai.xcd = G % 8 == 0 && !a.nt;
+                ai.xcd = !a.nt;
                 hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, G + 1), dim3(BX), 0, h->s,
                                    (const double*)h->buf[h->cur], (double*)h->buf[h->cur ^ 1], h->mac, ai);
             }
@@ -2609,7 +2616,7 @@ int run_steps(ippi_handle* h, int nsteps) {
                                      : pick_step<T, false, false>(h->c.collision, h->c.bc));
         if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
         // XCD-grouped block order while the stores stay temporal
-        a.xcd = (pull ? grid_pull.y : grid_first.y) % 8 == 0 && !(pull && a.nt);
+        a.xcd = !(pull && a.nt);
         if (h->timing == 1 && pull) h->ev_step.push_back(new_event(h));
         hipLaunchKernelGGL(k, pull ? grid_pull : grid_first, dim3(BX), 0, h->s, (const T*)h->buf[h->cur],
                            (T*)h->buf[h->cur ^ 1], h->mac, a);
@@ -2796,7 +2803,7 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     aa.patch_l = pl;
     aa.nbody = nb;
     // XCD-grouped rows while the two adjoint buffers stay within 512 MB (temporal stores)
-    aa.xcd = ny % 8 == 0 && 2 * 9 * h->N * sizeof(A) <= ((size_t)512 << 20);
+    aa.xcd = 2 * 9 * h->N * sizeof(A) <= ((size_t)512 << 20);
     for (int b = 0; b < nb; b++) aa.patch[b] = h->bodies[b].p;
     AdjIb ad{h->nact, h->ell, h->d_act, dan, lf, uact, h->d_pidx, h->d_w, dwx, dwy, nullptr,
              h->d_ecnt, h->d_emk, h->d_ecw, dtab, pl, gc};
@@ -3228,7 +3235,7 @@ int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, dou
 }
 
 int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch) {
-    if (nx < 4 || ny < 4 || precision < 0 || precision > 1 || mode < 0 || mode > 2 || (mode == 2 && ny % 8) ||
+    if (nx < 4 || ny < 4 || precision < 0 || precision > 1 || mode < 0 || mode > 2 ||
         reps < 1 || !us_per_launch)
         return IPPI_ERR_ARG;
     if (9 * (uint64_t)nx * (uint64_t)ny >= ((uint64_t)1 << 32)) return IPPI_ERR_ARG;
