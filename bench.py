@@ -196,17 +196,32 @@ def leg_nn(ippi, steps, warmup, profile_steps):
     # error growth: FP32 against FP64 after 1,000 steps from the same start, beside FP64 against FP64
     # from a start perturbed by 1e-9 (the wake amplifies any perturbation; at `steps` only the first
     # comparison is made, above)
-    early = {}
-    for name, prec, noise in (("fp32", "fp32", 1e-3), ("fp64", "fp64", 1e-3), ("fp64_perturbed", "fp64", 1e-3 + 1e-9)):
-        s = ippi.Solver(nx, ny, "mrt", prec, "inflow", les=True, tau_mode="nn", tau0=tau0, cs=CS, u_in=(U, 0.0))
+    early, etau = {}, {}
+    for name, prec, mode, noise in (("fp32", "fp32", "nn", 1e-3), ("fp64", "fp64", "nn", 1e-3),
+                                    ("fp64_perturbed", "fp64", "nn", 1e-3 + 1e-9),
+                                    ("fp32_bf16_plain", "fp32", "nn_bf16", 1e-3)):
+        s = ippi.Solver(nx, ny, "mrt", prec, "inflow", les=True, tau_mode=mode, tau0=tau0, cs=CS, u_in=(U, 0.0))
         s.init_fields(SEED, 1.0, (U, 0.0), noise)
         s.add_body(nx / 4, ny / 2, D, workloads.markers_for(D))
         s.set_mlp(blob)
         s.step(1000)
         early[name] = s.fields()
+        etau[name] = s.tau()
+        if name == "fp32_bf16_plain":  # speed of the plain-bf16 network (IPPI_TAU_NN_BF16)
+            dtb, kb = timed_run_events(s, min(steps, 2000), warmup)
+            gbs_b = 72 * nx * ny / (kb * 1e-6) / 1e9 if kb else None
+            out["bf16_plain"] = {"mlups_fp32": round(nx * ny * min(steps, 2000) / dtb / 1e6, 1),
+                                 "fused_kernel_avg_us": round(kb, 2) if kb else None,
+                                 "fused_kernel_roofline_frac": round(gbs_b / HBM_PEAK_GBS, 4) if gbs_b else None}
         s.close()
     out["rel_l2_u_vs_fp64_at_1000"] = vel_err(early["fp32"], early["fp64"])
     out["rel_l2_u_fp64_perturbed_vs_fp64_at_1000"] = vel_err(early["fp64_perturbed"], early["fp64"])
+    out["bf16_plain"].update({
+        "rel_l2_u_vs_fp64_at_1000": vel_err(early["fp32_bf16_plain"], early["fp64"]),
+        "rel_l2_tau_vs_fp64_at_1000": float(np.linalg.norm(etau["fp32_bf16_plain"] - etau["fp64"]) /
+                                            np.linalg.norm(etau["fp64"])),
+        "rel_l2_tau_compensated_vs_fp64_at_1000": float(np.linalg.norm(etau["fp32"] - etau["fp64"]) /
+                                                        np.linalg.norm(etau["fp64"]))})
     t32, t64 = res["fp32"][2], res["fp64"][2]
     out.update({"mlups_fp32": round(res["fp32"][0], 1), "mlups_fp64": round(res["fp64"][0], 1),
                 "speedup_vs_fp64": round(res["fp32"][0] / res["fp64"][0], 3),

@@ -68,7 +68,12 @@ enum ippi_bc { IPPI_BC_PERIODIC_BB = 0, IPPI_BC_INFLOW = 1, IPPI_BC_FLATPLATE = 
  *                        ('d': rho from y = 1, u = 0) beyond; the corners
  *                        (0, 0) and (nx-1, 0), which no rule reaches, keep
  *                        their state. */
-enum ippi_tau_mode { IPPI_TAU_CONST = 0, IPPI_TAU_FIELD = 1, IPPI_TAU_NN = 2 };
+/* IPPI_TAU_NN_BF16: the network with single bf16 products (no hi/lo
+ * compensation) and FP32 accumulation in the FP32 path, a third of the
+ * MFMAs and about 8 mantissa bits per operand; no counterpart in the
+ * reference (a speed/accuracy option of the fused network). In FP64 it is
+ * IPPI_TAU_NN. */
+enum ippi_tau_mode { IPPI_TAU_CONST = 0, IPPI_TAU_FIELD = 1, IPPI_TAU_NN = 2, IPPI_TAU_NN_BF16 = 3 };
 
 typedef struct ippi_config {
     int nx, ny;          /* whole domain (no ghost columns) */

@@ -20,7 +20,7 @@ IPPI_OK, IPPI_ERR_ARG, IPPI_ERR_HIP, IPPI_ERR_DIVERGED, IPPI_ERR_STATE, IPPI_ERR
 COLLISION = {"bgk": 0, "mrt": 1, "rr": 2}
 PRECISION = {"fp64": 0, "fp32": 1}
 BC = {"periodic_bb": 0, "inflow": 1, "flatplate": 2}
-TAU_MODE = {"const": 0, "field": 1, "nn": 2}
+TAU_MODE = {"const": 0, "field": 1, "nn": 2, "nn_bf16": 3}
 MLP_BLOB = 6 * 32 + 32 + 32 * 32 + 32 + 32 + 1 + 12
 IPPI_ADJ_FP64 = 1
 

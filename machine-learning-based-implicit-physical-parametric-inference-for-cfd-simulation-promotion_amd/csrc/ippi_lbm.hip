@@ -651,7 +651,9 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
 // never touched, so FP32 rounding cannot drift mass or momentum. RR is
 // evaluated on populations with the increment projected onto exact mass and
 // momentum. The LES stress comes from the second moments directly.
-template <int COLL, int BC, bool PULL, bool NN, bool NT, bool COH = false>
+// NN: 0 no network, 1 network with compensated (hi/lo) bf16 products,
+// 2 network with plain bf16 products (IPPI_TAU_NN_BF16)
+template <int COLL, int BC, bool PULL, int NN, bool NT, bool COH = false>
 __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const float* __restrict__ src,
                                          const double* __restrict__ mac, float* __restrict__ dst, const StepArgs& a,
                                          bool active, __bf16* lds, int bx0) {
@@ -725,7 +727,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         feat[3] = 1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy));
         feat[4] = nxx - nyy;
         feat[5] = nxy;
-        const float t = mlp_tau_wave(feat, a.mlp, lds);  // tau - 1/2
+        const float t = mlp_tau_wave<NN == 1>(feat, a.mlp, lds);  // tau - 1/2
         nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
         if (nn_ok) tx = t;
     }
@@ -800,7 +802,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
 // and need the occupancy to overlap transcendental latency and the
 // population loads (caps them at 128 VGPRs). The others are latency-bound
 // streams and keep the full 8 waves (64 VGPRs).
-template <int COLL, int BC, bool PULL, int NPT, bool NN, bool NT>
+template <int COLL, int BC, bool PULL, int NPT, int NN, bool NT>
 __global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? 4 : 8)))
 k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
@@ -897,7 +899,7 @@ __global__ void __launch_bounds__(BX) k_probe(const T* __restrict__ src, T* __re
 // Relaxation time the next collision uses at each node, from the current
 // post-streaming state (IB velocity correction not applied): base value,
 // LES, or the network with its guard.
-template <int BC, bool NN>
+template <int BC, int NN>
 __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, double* __restrict__ out, StepArgs a) {
 #pragma clang fp contract(fast)
     const int nx = a.nx, ny = a.ny;
@@ -934,7 +936,7 @@ __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, dou
     if (NN) {
         float feat[6] = {o.rho, o.ux, o.uy,
                          1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy)), nxx - nyy, nxy};
-        const float t = mlp_tau_wave(feat, a.mlp, lds_mlp + (threadIdx.x & ~63) * 16);
+        const float t = mlp_tau_wave<NN == 1>(feat, a.mlp, lds_mlp + (threadIdx.x & ~63) * 16);
         nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
         if (nn_ok) tx = t;
     }
@@ -1636,7 +1638,22 @@ __device__ void ib_in_step(const StepArgs& a, int body, const T* __restrict__ sr
     }
 }
 
-template <int COLL, int BC, bool NN, bool NT>
+// Wait (every wave of a band block) until every body's forces of this step
+// are stored: agent-coherent polls of the flags, bounded by INK_SPIN.
+__device__ __forceinline__ void wait_ib_forces(const StepArgs& a) {
+    for (int b = 0; b < a.nbody; b++) {
+        int spins = 0;
+        while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_wait) {
+            if (++spins > INK_SPIN) {
+                *a.flag = 2;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+}
+
+template <int COLL, int BC, int NN, bool NT>
 __global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? 4 : 8)))
 k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const double* __restrict__ mac, StepArgs a) {
     __shared__ float U[2 * (NN ? INK_MAXACT_NN : INK_MAXACT)];
@@ -1650,18 +1667,7 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, ny, bxi, g);
     int y = g + a.rot;  // rot < ny
     if (y >= ny) y -= ny;
-    if (y >= a.band0 && y < a.band1) {  // block-uniform: wait for every body's forces of this step
-        for (int b = 0; b < a.nbody; b++) {
-            int spins = 0;
-            while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_wait) {
-                if (++spins > INK_SPIN) {
-                    *a.flag = 2;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-    }
+    if (y >= a.band0 && y < a.band1) wait_ib_forces(a);  // block-uniform
     const bool active = (int)(bxi * BX + threadIdx.x) < nx;
     const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny;
@@ -1688,18 +1694,7 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
     int gi = g + a.rot;  // rot < G
     if (gi >= G) gi -= G;
     const int y0 = gi * NPT;
-    if (y0 < a.band1 && y0 + NPT > a.band0) {  // block-uniform: wait for every body's forces of this step
-        for (int b = 0; b < a.nbody; b++) {
-            int spins = 0;
-            while (__hip_atomic_load(a.ib_flag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.ib_wait) {
-                if (++spins > INK_SPIN) {
-                    *a.flag = 2;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-    }
+    if (y0 < a.band1 && y0 + NPT > a.band0) wait_ib_forces(a);  // block-uniform
     const bool active = (int)(bxi * BX + threadIdx.x) < nx;
     const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny;
@@ -2152,6 +2147,14 @@ void dfree(P*& p) {
     p = nullptr;
 }
 
+// Network mode of the step kernels: 0 none, 1 compensated bf16 products,
+// 2 plain bf16 products (FP32 only; FP64 evaluates the network in FP64).
+int nn_kind(const ippi_handle* h) {
+    if (h->c.tau_mode == IPPI_TAU_NN) return 1;
+    if (h->c.tau_mode == IPPI_TAU_NN_BF16) return h->esz == 4 ? 2 : 1;
+    return 0;
+}
+
 int check_flag(ippi_handle* h) {
     int v = 0;
     HIPC(h, hipMemcpyAsync(&v, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->s));
@@ -2166,14 +2169,15 @@ using StepKernel = void (*)(const T*, T*, const double*, StepArgs);
 
 // nt: non-temporal stores (pull steps of lattices whose two population
 // buffers exceed 512 MB)
-template <typename T, bool PULL, bool NN>
+// NN: network mode (0 none, 1 compensated, 2 plain bf16; FP64 has one network)
+template <typename T, bool PULL, int NN>
 StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
 #define IPPI_PICK(C, B)                                                                         \
     if (coll == C && bc == B) {                                                                 \
-        constexpr int NPT_ = step_npt<T, PULL, NN>();                                         \
+        constexpr int NPT_ = step_npt<T, PULL, (NN != 0)>();                                  \
         if constexpr (sizeof(T) == 8) {                                                         \
-            if (PULL && nt) return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, NN, PULL>;       \
-            return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, NN, false>;                      \
+            if (PULL && nt) return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, (NN != 0), PULL>; \
+            return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, (NN != 0), false>;               \
         } else {                                                                                \
             if (PULL && nt) return (StepKernel<T>)k_step_f32<C, B, PULL, NPT_, NN, PULL>;       \
             return (StepKernel<T>)k_step_f32<C, B, PULL, NPT_, NN, false>;                      \
@@ -2193,9 +2197,10 @@ StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
 }
 
 template <int COLL, int BC>
-StepKernel<float> pick_step_ib_bc(bool nn, bool nt) {
-    if (nn) return nt ? k_step_f32_ib<COLL, BC, true, true> : k_step_f32_ib<COLL, BC, true, false>;
-    return nt ? k_step_f32_ib<COLL, BC, false, true> : k_step_f32_ib<COLL, BC, false, false>;
+StepKernel<float> pick_step_ib_bc(int nn, bool nt) {
+    if (nn == 2) return nt ? k_step_f32_ib<COLL, BC, 2, true> : k_step_f32_ib<COLL, BC, 2, false>;
+    if (nn == 1) return nt ? k_step_f32_ib<COLL, BC, 1, true> : k_step_f32_ib<COLL, BC, 1, false>;
+    return nt ? k_step_f32_ib<COLL, BC, 0, true> : k_step_f32_ib<COLL, BC, 0, false>;
 }
 
 template <int COLL, int BC>
@@ -2218,7 +2223,7 @@ StepKernel<double> pick_step_ib64(int coll, int bc, bool nt) {
 }
 
 // FP32 pull step with the IB inside (periodic and inflow boundaries)
-StepKernel<float> pick_step_ib(int coll, int bc, bool nn, bool nt) {
+StepKernel<float> pick_step_ib(int coll, int bc, int nn, bool nt) {
 #define IPPI_PICK_IB(C, B) \
     if (coll == C && bc == B) return pick_step_ib_bc<C, B>(nn, nt);
     IPPI_PICK_IB(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
@@ -2406,7 +2411,7 @@ int upload_ib(ippi_handle* h) {
         for (const auto& e : blk)
             if (e.lead) lead.push_back(e);
         int b0 = h->c.ny, b1 = 0;
-        const bool nnm = h->c.tau_mode == IPPI_TAU_NN;
+        const bool nnm = nn_kind(h) != 0;
         bool fit = h->c.bc != IPPI_BC_FLATPLATE && nb <= (h->c.nx + BX - 1) / BX && (int)lead.size() == nb &&
                    (h->esz == 4 || !nnm);
         const int cap = h->esz == 8 ? INK_MAXACT64 : (nnm ? INK_MAXACT_NN : INK_MAXACT);
@@ -2523,7 +2528,7 @@ template <typename T>
 int run_steps(ippi_handle* h, int nsteps) {
     CallTimer call_timer(h);
     const int nx = h->c.nx, ny = h->c.ny;
-    const bool nn_mode = h->c.tau_mode == IPPI_TAU_NN;
+    const bool nn_mode = nn_kind(h) != 0;
     const int npt = nn_mode ? step_npt<T, true, true>() : step_npt<T, true, false>();
     const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const bool ib = !h->bodies.empty();
@@ -2541,7 +2546,7 @@ int run_steps(ippi_handle* h, int nsteps) {
         a.ecw = h->d_ecw;
         a.F = h->d_F;
     }
-    if (ib && sizeof(T) == 4 && h->c.tau_mode != IPPI_TAU_NN) {  // FP32 step kernels without the network
+    if (ib && sizeof(T) == 4 && !nn_mode) {  // FP32 step kernels without the network
                                                                  // copy what the next IB kernel pulls
         a.scat = h->d_scat;
         a.slot = h->d_slot;
@@ -2555,7 +2560,8 @@ int run_steps(ippi_handle* h, int nsteps) {
                                (unsigned)h->N);
             HIPC(h, hipGetLastError());
         }
-        const bool nn = h->c.tau_mode == IPPI_TAU_NN;
+        const int nk = nn_kind(h);
+        const bool nn = nk != 0;
         // IB inside the step kernel (pull steps; FP64 without the network)
         if (ib && pull && h->ink_ok && h->ink_enabled && (sizeof(T) == 4 || !nn)) {
             const int nb = (int)h->bodies.size();
@@ -2577,7 +2583,7 @@ int run_steps(ippi_handle* h, int nsteps) {
             const int at = (int)((nn ? INK_BAND_AT_NN : INK_BAND_AT) * ny);
             if (h->timing == 1) h->ev_step.push_back(new_event(h));
             if (sizeof(T) == 4) {
-                StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nn, a.nt);
+                StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nk, a.nt);
                 if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
                 ai.rot = ((h->band0 - at) % ny + ny) % ny;
                 ai.xcd = !a.nt;
@@ -2610,10 +2616,12 @@ int run_steps(ippi_handle* h, int nsteps) {
             if (h->timing == 1) h->ev_ib.push_back(new_event(h));
             h->batch_ib++;
         }
-        StepKernel<T> k = pull ? (nn ? pick_step<T, true, true>(h->c.collision, h->c.bc, a.nt)
-                                     : pick_step<T, true, false>(h->c.collision, h->c.bc, a.nt))
-                               : (nn ? pick_step<T, false, true>(h->c.collision, h->c.bc)
-                                     : pick_step<T, false, false>(h->c.collision, h->c.bc));
+        StepKernel<T> k = pull ? (nk == 2   ? pick_step<T, true, 2>(h->c.collision, h->c.bc, a.nt)
+                                  : nk == 1 ? pick_step<T, true, 1>(h->c.collision, h->c.bc, a.nt)
+                                            : pick_step<T, true, 0>(h->c.collision, h->c.bc, a.nt))
+                               : (nk == 2   ? pick_step<T, false, 2>(h->c.collision, h->c.bc)
+                                  : nk == 1 ? pick_step<T, false, 1>(h->c.collision, h->c.bc)
+                                            : pick_step<T, false, 0>(h->c.collision, h->c.bc));
         if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
         // XCD-grouped block order while the stores stay temporal
         a.xcd = !(pull && a.nt);
@@ -2876,7 +2884,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (!cfg || !out) return IPPI_ERR_ARG;
     *out = nullptr;
     if (cfg->nx < 4 || cfg->ny < 4 || cfg->collision < 0 || cfg->collision > 2 || cfg->precision < 0 ||
-        cfg->precision > 1 || cfg->bc < 0 || cfg->bc > 2 || cfg->tau_mode < 0 || cfg->tau_mode > 2 ||
+        cfg->precision > 1 || cfg->bc < 0 || cfg->bc > 2 || cfg->tau_mode < 0 || cfg->tau_mode > 3 ||
         !(cfg->tau0 > 0.5) || cfg->tau_nn_max < 0)
         return IPPI_ERR_ARG;
     if (9 * (uint64_t)cfg->nx * (uint64_t)cfg->ny >= ((uint64_t)1 << 32)) return IPPI_ERR_ARG;
@@ -3201,7 +3209,7 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
 int ippi_step(ippi_handle* h, int nsteps) {
     if (!h || nsteps < 0) return IPPI_ERR_ARG;
     if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
-    if (h->c.tau_mode == IPPI_TAU_NN && !h->has_mlp) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
+    if (nn_kind(h) && !h->has_mlp) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
     return h->esz == 8 ? run_steps<double>(h, nsteps) : run_steps<float>(h, nsteps);
 }
 
@@ -3303,8 +3311,7 @@ int ippi_adjoint_drag(ippi_handle* h, int nsteps, int avg_from, int ckpt, int fl
                       double* J, double* g_tau, double* g_center) {
     if (!h || nsteps < 1 || avg_from < 1 || avg_from > nsteps || ckpt < 1 || (flags & ~IPPI_ADJ_FP64))
         return IPPI_ERR_ARG;
-    if (h->c.collision != IPPI_COLL_BGK || h->c.bc != IPPI_BC_PERIODIC_BB || h->c.les ||
-        h->c.tau_mode == IPPI_TAU_NN)
+    if (h->c.collision != IPPI_COLL_BGK || h->c.bc != IPPI_BC_PERIODIC_BB || h->c.les || nn_kind(h))
         return fail(h, IPPI_ERR_ARG, "the adjoint covers BGK, periodic x / bounce-back walls, no LES, "
                                      "constant or per-node tau");
     if (h->bodies.empty()) return fail(h, IPPI_ERR_STATE, "the drag objective needs an immersed body");
@@ -3369,7 +3376,7 @@ int ippi_get_mean(ippi_handle* h, double* rho, double* ux, double* uy) {
 
 int ippi_get_stencil(ippi_handle* h, int n, const int* xy, double* fnew, double* feq, double* tau) {
     if (!h || n < 0 || n > INT_MAX / 9 || (n > 0 && (!xy || !fnew || !feq || !tau))) return IPPI_ERR_ARG;
-    if (h->c.tau_mode == IPPI_TAU_NN)
+    if (nn_kind(h))
         return fail(h, IPPI_ERR_ARG, "the stencil covers constant or per-node tau (with or without LES)");
     if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
     if (!h->prev_pull) return fail(h, IPPI_ERR_STATE, "no pull step since the state was set");
@@ -3409,12 +3416,13 @@ int ippi_get_tau(ippi_handle* h, double* tau) {
     if (!h || !tau) return IPPI_ERR_ARG;
     if (h->phase == 0) return fail(h, IPPI_ERR_STATE, "no step taken yet");
     if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
-    if (h->c.tau_mode == IPPI_TAU_NN && !h->has_mlp) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
+    if (nn_kind(h) && !h->has_mlp) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
     StepArgs a = make_args(h);
     double* d_out = nullptr;
     HIPC(h, hipMalloc(&d_out, sizeof(double) * h->N));
     dim3 grid((h->c.nx + BX - 1) / BX, h->c.ny);
-    const bool nn = h->c.tau_mode == IPPI_TAU_NN;
+    const int nk = nn_kind(h);
+    const bool nn = nk != 0;
     const int bc = h->c.bc;
 #define IPPI_TAU_LAUNCH(K, T, B)                                                                          \
     if (bc == B) {                                                                                        \
@@ -3423,10 +3431,20 @@ int ippi_get_tau(ippi_handle* h, double* tau) {
         else                                                                                              \
             hipLaunchKernelGGL((K<B, false>), grid, dim3(BX), 0, h->s, (const T*)h->buf[h->cur], d_out, a); \
     }
+#define IPPI_TAU32_LAUNCH(B)                                                                               \
+    if (bc == B) {                                                                                         \
+        if (nk == 2)                                                                                       \
+            hipLaunchKernelGGL((k_tau32<B, 2>), grid, dim3(BX), 0, h->s, (const float*)h->buf[h->cur], d_out, a); \
+        else if (nk == 1)                                                                                  \
+            hipLaunchKernelGGL((k_tau32<B, 1>), grid, dim3(BX), 0, h->s, (const float*)h->buf[h->cur], d_out, a); \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_tau32<B, 0>), grid, dim3(BX), 0, h->s, (const float*)h->buf[h->cur], d_out, a); \
+    }
     if (h->esz == 4) {
-        IPPI_TAU_LAUNCH(k_tau32, float, IPPI_BC_PERIODIC_BB)
-        IPPI_TAU_LAUNCH(k_tau32, float, IPPI_BC_INFLOW)
-        IPPI_TAU_LAUNCH(k_tau32, float, IPPI_BC_FLATPLATE)
+        IPPI_TAU32_LAUNCH(IPPI_BC_PERIODIC_BB)
+        IPPI_TAU32_LAUNCH(IPPI_BC_INFLOW)
+        IPPI_TAU32_LAUNCH(IPPI_BC_FLATPLATE)
+#undef IPPI_TAU32_LAUNCH
     } else {
         IPPI_TAU_LAUNCH(k_tau64, double, IPPI_BC_PERIODIC_BB)
         IPPI_TAU_LAUNCH(k_tau64, double, IPPI_BC_INFLOW)

@@ -20,6 +20,10 @@
 // operand without a transpose: the host permutes W2's columns to the
 // order in which the lanes hold the layer-1 outputs.
 //
+// SPLIT = false (IPPI_TAU_NN_BF16): single bf16 products (the hi parts
+// only), FP32 accumulation: a third of the MFMAs and no lo parts, about 8
+// mantissa bits per operand.
+//
 // FP64 path: the same network on VALU in FP64 with the oracle's loop order.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -93,6 +97,7 @@ __device__ __forceinline__ void split2(float a, float b, unsigned& hi, unsigned&
 // wave. The 64 nodes are handled as four blocks of 16; each block runs
 // layer 1, the activation, layer 2 and the output sum before the next, so
 // only one block's tiles are live at a time.
+template <bool SPLIT>
 __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m, __bf16* lds) {
     // contraction lets the output sum acc += (w3 t2) sigma(t2) use FMAs; the
     // feature normalisation below keeps its subtraction (exact for features
@@ -104,10 +109,15 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
     // normalised features, split hi / lo, to LDS as [node][k] (k 0..7)
     unsigned vh[4], vl[4];
 #pragma unroll
-    for (int k = 0; k < 6; k += 2)
-        split2((feat[k] - m.mu[k]) * m.isig[k], (feat[k + 1] - m.mu[k + 1]) * m.isig[k + 1], vh[k / 2], vl[k / 2]);
+    for (int k = 0; k < 6; k += 2) {
+        const float a = (feat[k] - m.mu[k]) * m.isig[k], b = (feat[k + 1] - m.mu[k + 1]) * m.isig[k + 1];
+        if (SPLIT)
+            split2(a, b, vh[k / 2], vl[k / 2]);
+        else
+            vh[k / 2] = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+    }
     *reinterpret_cast<u32x4*>(lds + 16 * l) = (u32x4){vh[0], vh[1], vh[2], 0u};
-    *reinterpret_cast<u32x4*>(lds + 16 * l + 8) = (u32x4){vl[0], vl[1], vl[2], 0u};
+    if (SPLIT) *reinterpret_cast<u32x4*>(lds + 16 * l + 8) = (u32x4){vl[0], vl[1], vl[2], 0u};
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     float part[4];
@@ -117,7 +127,7 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
         bf16x4 bh, bl;
         if (g < 2) {
             bh = *reinterpret_cast<const bf16x4*>(lds + 16 * node + 4 * g);
-            bl = *reinterpret_cast<const bf16x4*>(lds + 16 * node + 8 + 4 * g);
+            if (SPLIT) bl = *reinterpret_cast<const bf16x4*>(lds + 16 * node + 8 + 4 * g);
         } else {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -126,16 +136,23 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
             }
         }
         // layer 1 + activation -> layer-2 B operand (8 values per lane)
-        unsigned ph[4], pl[4];
+        unsigned ph[4], pl[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
         for (int mb = 0; mb < 2; mb++) {
             f32x4 c = {fr.b1[4 * mb], fr.b1[4 * mb + 1], fr.b1[4 * mb + 2], fr.b1[4 * mb + 3]};
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1h[mb], bh, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1h[mb], bl, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1l[mb], bh, c, 0, 0, 0);
+            if (SPLIT) {
+                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1h[mb], bl, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1l[mb], bh, c, 0, 0, 0);
+            }
 #pragma unroll
-            for (int r = 0; r < 4; r += 2)
-                split2(c[r] * sig2(c[r]), c[r + 1] * sig2(c[r + 1]), ph[2 * mb + r / 2], pl[2 * mb + r / 2]);
+            for (int r = 0; r < 4; r += 2) {
+                const float h0 = c[r] * sig2(c[r]), h1 = c[r + 1] * sig2(c[r + 1]);
+                if (SPLIT)
+                    split2(h0, h1, ph[2 * mb + r / 2], pl[2 * mb + r / 2]);
+                else
+                    ph[2 * mb + r / 2] = __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){h0, h1}, bf16x2));
+            }
         }
         const bf16x8 hh = __builtin_bit_cast(bf16x8, (u32x4){ph[0], ph[1], ph[2], ph[3]});
         const bf16x8 hl = __builtin_bit_cast(bf16x8, (u32x4){pl[0], pl[1], pl[2], pl[3]});
@@ -145,8 +162,10 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
         for (int mb = 0; mb < 2; mb++) {
             f32x4 c = {fr.b2[4 * mb], fr.b2[4 * mb + 1], fr.b2[4 * mb + 2], fr.b2[4 * mb + 3]};
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], hh, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], hl, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2l[mb], hh, c, 0, 0, 0);
+            if (SPLIT) {
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], hl, c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2l[mb], hh, c, 0, 0, 0);
+            }
 #pragma unroll
             for (int r = 0; r < 4; r++) acc += (fr.w3[4 * mb + r] * c[r]) * sig2(c[r]);
         }

@@ -138,3 +138,34 @@ def test_nn_guard_falls_back_to_les(ippi, precision):
             assert np.array_equal(a, b)
         else:
             assert rel_l2(a, b) <= 1e-5
+
+
+def test_nn_bf16_plain_products(oracle, ippi):
+    """IPPI_TAU_NN_BF16: single bf16 products with FP32 accumulation. Its
+    relaxation times stay within the bf16 rounding of the FP64 network
+    (operands carry 8 mantissa bits: 2e-3 absolute on tau here, where tau -
+    1/2 ~ 0.1), and the compensated path is closer by orders of magnitude;
+    in FP64 the mode is the FP64 network."""
+    blob = xavier_blob(2)
+    nx, ny, coll, bc, les, tau0, cs, u0, body, steps = CASES["bgk_nn_channel"]
+    o = oracle.Sim(nx, ny, coll, bc, les=les, tau_mode="nn", tau0=tau0, cs=cs, u_in=u0, threads=4)
+    o.init_fields(7, 1.0, u0, 1e-3)
+    o.set_mlp(blob)
+    o.step(steps)
+    o._blob = blob
+    t_ref = oracle_tau(oracle, o)
+    errs = {}
+    for mode, prec in (("nn_bf16", "fp32"), ("nn", "fp32"), ("nn_bf16", "fp64")):
+        g = ippi.Solver(nx, ny, coll, prec, bc, les=les, tau_mode=mode, tau0=tau0, cs=cs, u_in=u0)
+        g.init_fields(7, 1.0, u0, 1e-3)
+        g.set_mlp(blob)
+        g.step(steps)
+        errs[(mode, prec)] = np.abs(g.tau() - t_ref).max()
+        if prec == "fp64":
+            assert rel_l2(g.fields()[1], o.fields()[1]) <= 1e-12
+        g.close()
+    print("max |tau - tau_oracle|:", {f"{m}/{p}": f"{e:.2e}" for (m, p), e in errs.items()})
+    assert errs[("nn_bf16", "fp32")] <= 2e-3
+    assert errs[("nn", "fp32")] <= 2e-5
+    assert errs[("nn", "fp32")] < errs[("nn_bf16", "fp32")] / 10
+    assert errs[("nn_bf16", "fp64")] <= 1e-12
