@@ -1595,7 +1595,14 @@ __device__ void ib_in_step(const StepArgs& a, int body, const T* __restrict__ sr
             F[r][1] = Fy;
         }
     }
-    // every force of the block is stored before the flag
+    // every force of the block is stored before the flag. Ordering: forces
+    // and flag are agent-scope atomics, each coherent across the XCDs' L2s;
+    // s_waitcnt(0) holds the flag store until every force store of the block
+    // is acknowledged, and a reader issues its force loads (agent-scope
+    // atomics too, patch_force<true>) only after a poll has returned the
+    // flag. A release store and acquire loads would order the same but add
+    // an L2 writeback and an L2 invalidate at agent scope, which the lattice
+    // blocks' cached data would pay for.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (t == 0) __hip_atomic_store(a.ib_flag + body, a.ib_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

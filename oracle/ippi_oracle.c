@@ -26,16 +26,22 @@
  *   inlet / outlet NEE and free-slip rows: per-node closed forms (rho = 1
  *                and u = u_in at the inlet, u copied at the outlet, the
  *                specular mirror at the walls).
+ *   stream_flatplate  each rule checked node by node on a noisy flow (NEE
+ *                inlet/outlet/top, 'm' mirror upstream of the plate, 'd'
+ *                no-slip NEE on it, interior pull = shift, frozen bottom
+ *                corners); uniform inflow with no plate is a fixed point to
+ *                1e-14; a boundary layer grows behind the leading edge.
  *   or_noise     the published splitmix64 outputs for seed 0.
  *   MLP          torch autograd (FP64) and central finite differences.
  *   or_adjoint_drag  central finite differences (h = 1e-4) of the forward
  *                step for single nodes, random directions and the body
  *                centre, on horizons of 1 to 50 steps; exact zeros outside
  *                the domain of dependence; translation equivariance.
- * Not restated, so no pin applies: the reference's whole step on its own
- * flat-plate boundary set (mesh.cpp:228-356 with the 'm'/'d' split of
- * mesh.cpp:1413) and its libtorch models (mesh.cpp:2767-3260): parity
- * unpinned for those, and the HIP library does not implement them.
+ * The step as a whole (mesh.cpp) is not compiled (it needs MPI and
+ * libtorch), so the composition is pinned by the checks above rather than
+ * bit for bit. Not restated, so no pin applies: the reference's libtorch
+ * boundary models (mesh.cpp:2767-3260); parity unpinned for those, and the
+ * HIP library does not implement them.
  */
 #include "ippi_oracle.h"
 #include <math.h>
