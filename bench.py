@@ -39,7 +39,8 @@ Reported beside it:
                                FP64, 5,000 steps, per-body drag error;
                 accuracy_2048x1024  growth of the FP32-vs-FP64 velocity error
                                on the headline workload over 20,000 steps,
-                               beside FP64 vs FP64 from a perturbed start;
+                               beside FP64 arithmetic over FP32-rounded
+                               storage and FP64 from a perturbed start;
                 flatplate_2401x501  the reference's own run (flat plate,
                                BGK + LES, Re = 5e5): FP32 and FP64 MLUPS and
                                the same error comparison.
@@ -307,27 +308,36 @@ def leg_multibody(ippi, steps, warmup):
 
 
 def leg_accuracy(ippi, marks=(1000, 2000, 5000, 10000, 20000)):
-    """Velocity error growth on the headline workload: FP32 vs FP64, and,
-    as a control, FP64 vs FP64 started from initial noise whose amplitude
-    differs by 1 part in 1e6 (a perturbation of ~1e-9 in u, the size of
-    FP32 rounding). Equal growth rates mean the divergence is the flow's
-    own sensitivity (vortex shedding at Re = 1e4), not a drift of the
-    reduced-precision path."""
+    """Velocity error growth on the headline workload: FP32 vs FP64, next
+    to two FP64 runs against the same FP64 run:
+    - FP64 arithmetic whose every stored population is rounded as the FP32
+      path stores it (f - w to float; IPPI_FP64_ROUND_STORE=1): the error
+      of the storage format alone, whatever the arithmetic;
+    - FP64 started from initial noise whose amplitude differs by 1 part in
+      1e6 (a one-time perturbation of ~1e-9 in u), for the flow's own
+      sensitivity (vortex shedding at Re = 1e4)."""
     a32, a64 = make_headline(ippi, "fp32"), make_headline(ippi, "fp64")
     p64 = ippi.Solver(NX, NY, "mrt", "fp64", "inflow", les=True, tau0=TAU0, cs=CS, u_in=(U0, 0.0))
     p64.init_fields(SEED, 1.0, (U0, 0.0), 1e-3 * (1 + 1e-6))
     p64.add_body(CX, CY, DIAM, NMARK)
-    out = {"steps": list(marks), "fp32_vs_fp64": [], "fp64_perturbed_vs_fp64": []}
+    os.environ["IPPI_FP64_ROUND_STORE"] = "1"  # read when the handle is created
+    try:
+        r64s = make_headline(ippi, "fp64")
+    finally:
+        del os.environ["IPPI_FP64_ROUND_STORE"]
+    runs = (a32, a64, p64, r64s)
+    out = {"steps": list(marks), "fp32_vs_fp64": [], "fp64_fp32_storage_vs_fp64": [], "fp64_perturbed_vs_fp64": []}
     done = 0
     for m in marks:
         log(f"accuracy: to step {m}")
-        for s in (a32, a64, p64):
+        for s in runs:
             s.step(m - done)
         done = m
-        r32, r64, rp = a32.fields(), a64.fields(), p64.fields()
+        r32, r64, rp, rs = a32.fields(), a64.fields(), p64.fields(), r64s.fields()
         out["fp32_vs_fp64"].append(vel_err(r32, r64))
+        out["fp64_fp32_storage_vs_fp64"].append(vel_err(rs, r64))
         out["fp64_perturbed_vs_fp64"].append(vel_err(rp, r64))
-    for s in (a32, a64, p64):
+    for s in runs:
         s.close()
     return out
 

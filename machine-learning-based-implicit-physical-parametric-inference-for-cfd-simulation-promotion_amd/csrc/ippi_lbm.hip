@@ -52,6 +52,8 @@ struct StepArgs {
     int* flag;              // divergence flag
     int nbody;
     int nt;                 // non-temporal stores
+    int round32;            // FP64 path: each stored population rounded as the FP32 path stores it
+                            // (f - w to float and back): a diagnostic of the storage format alone
     double nn_max;          // upper bound of accepted network values
     float nn_maxtxf;        // nn_max - 1/2
     MlpDev mlp;             // relaxation-time network (tau_mode NN)
@@ -611,6 +613,10 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
         double v[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) v[k] = fresh ? o[k] : f[k];
+        if (a.round32) {  // block-uniform
+#pragma unroll
+            for (int k = 0; k < 9; k++) v[k] = (double)(float)(v[k] - WD[k]) + WD[k];
+        }
         st9<NT>(dst, N, n, v);
     }
 }
@@ -2116,6 +2122,8 @@ struct ippi_handle {
     bool ink_enabled = true;      // environment IPPI_IB_IN_STEP=0 turns it off
     bool ink_wait_skew = false;   // environment IPPI_IB_WAIT_SKEW=1: the band waits for a value never
                                   // stored, so its bounded wait times out (tests of that path only)
+    bool round32 = false;         // environment IPPI_FP64_ROUND_STORE=1: the FP64 path rounds each stored
+                                  // population as the FP32 path stores it (accuracy diagnostic)
     std::vector<int> h_act;      // active node -> patch node (host copy)
     std::vector<int> body_act0;  // first active node of each body (+ total)
     double* hist = nullptr;
@@ -2262,6 +2270,7 @@ StepArgs make_args(ippi_handle* h) {
     a.flag = h->flag;
     a.nbody = (int)h->bodies.size();
     a.nt = 2 * 9 * h->N * h->esz > ((size_t)512 << 20);
+    a.round32 = h->round32 && h->esz == 8;
     a.nn_max = h->c.tau_nn_max > 0 ? h->c.tau_nn_max : 1e30;
     a.nn_maxtxf = (float)(a.nn_max - 0.5);
     a.mlp.frag = h->d_frag;
@@ -2899,6 +2908,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     h->c = *cfg;
     if (const char* e = std::getenv("IPPI_IB_IN_STEP")) h->ink_enabled = std::strcmp(e, "0") != 0;
     if (const char* e = std::getenv("IPPI_IB_WAIT_SKEW")) h->ink_wait_skew = std::strcmp(e, "1") == 0;
+    if (const char* e = std::getenv("IPPI_FP64_ROUND_STORE")) h->round32 = std::strcmp(e, "1") == 0;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;
     auto bail = [&](int code) {

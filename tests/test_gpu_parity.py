@@ -234,3 +234,38 @@ def test_full_size_translation_equivariance(ippi, precision):
         g.close()
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(np.roll(a, s, axis=1), b)
+
+
+def test_fp64_storage_rounding_diagnostic(ippi, monkeypatch):
+    """IPPI_FP64_ROUND_STORE=1 (read when the handle is created): the FP64
+    path stores every population as the FP32 path stores it, f - w rounded
+    to float. From the same start, one step of it is the plain FP64 step's
+    state so rounded, bit for bit, wherever the read-back (post-streaming)
+    populations are copies of stored ones: every column but the inlet and
+    outlet, whose NEE rule recomputes them. Later steps differ from plain
+    FP64 by about what FP32 storage costs, and a handle created without the
+    variable is unaffected."""
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)[:, None, None]
+    runs = {}
+    for mode in ("plain", "round", "plain_after"):
+        if mode == "round":
+            monkeypatch.setenv("IPPI_FP64_ROUND_STORE", "1")
+        g = ippi.Solver(160, 80, "mrt", "fp64", "inflow", les=True, tau0=0.52, cs=0.1, u_in=(0.08, 0.0))
+        monkeypatch.delenv("IPPI_FP64_ROUND_STORE", raising=False)
+        g.init_fields(42, 1.0, (0.08, 0.0), 1e-3)
+        g.add_body(48, 40, 20, 63)
+        g.step(1)
+        f1 = g.populations()
+        g.step(199)
+        runs[mode] = (f1, g.fields())
+        g.close()
+    f_plain, f_round = runs["plain"][0], runs["round"][0]
+    expect = (f_plain - w).astype(np.float32).astype(np.float64) + w
+    assert np.array_equal(f_round[:, :, 1:-1], expect[:, :, 1:-1])
+    assert np.abs(f_round - expect).max() < 1e-8
+    assert not np.array_equal(f_round, f_plain)
+    (_, a), (_, b), (_, c) = runs["plain"], runs["round"], runs["plain_after"]
+    e = vel_err(b[1], b[2], a[1], a[2])
+    print(f"FP64 with FP32 storage vs FP64 after 200 steps: velocity rel L2 {e:.3e}")
+    assert 0 < e < 1e-6
+    assert all(np.array_equal(x, y) for x, y in zip(a, c))
