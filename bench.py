@@ -27,7 +27,8 @@ Reported beside it:
   legs          the other workloads, each measured the same way:
                 nn_4096x2048   MRT + LES + IB (D = 160) with the fused
                                network tau (seeded Xavier weights, guard
-                               active), FP32 vs FP64, 10,000 steps;
+                               active), FP32 vs FP64, 10,000 steps, and FP64
+                               over FP32-rounded storage vs FP64;
                 adjoint_1024x512  BGK + IB discrete adjoint of the mean drag
                                over steps 2001-4000, checkpoints every 100,
                                tau field 0.6 +- 0.02: wall time of forward +
@@ -194,6 +195,19 @@ def leg_nn(ippi, steps, warmup, profile_steps):
                         "fused_kernel_gbs": round(gbs, 1), "fused_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
                         "ib_kernel_avg_us": round(ib_us, 2)})
         s.close()
+    # the storage format alone: FP64 arithmetic whose stored populations are rounded as FP32 stores them
+    os.environ["IPPI_FP64_ROUND_STORE"] = "1"  # read when the handle is created
+    try:
+        s = ippi.Solver(nx, ny, "mrt", "fp64", "inflow", les=True, tau_mode="nn", tau0=tau0, cs=CS, u_in=(U, 0.0))
+    finally:
+        del os.environ["IPPI_FP64_ROUND_STORE"]
+    s.init_fields(SEED, 1.0, (U, 0.0), 1e-3)
+    s.add_body(nx / 4, ny / 2, D, workloads.markers_for(D))
+    s.set_mlp(blob)
+    log(f"nn 4096x2048 fp64 with FP32-rounded storage: {warmup + steps} steps")
+    s.step(warmup + steps)
+    out["rel_l2_u_fp64_fp32_storage_vs_fp64"] = vel_err(s.fields(), res["fp64"][1])
+    s.close()
     # error growth: FP32 against FP64 after 1,000 steps from the same start, beside FP64 against FP64
     # from a start perturbed by 1e-9 (the wake amplifies any perturbation; at `steps` only the first
     # comparison is made, above)
