@@ -32,9 +32,10 @@ Reported beside it:
                 adjoint_1024x512  BGK + IB discrete adjoint of the mean drag
                                over steps 2001-4000, checkpoints every 100,
                                tau field 0.6 +- 0.02: wall time of forward +
-                               reverse sweep, bandwidth of the all-node
-                               adjoint kernel, for FP32 and FP32 forward /
-                               FP64 adjoint;
+                               reverse sweep (recompute beside the sweep, and
+                               between the sweeps), bandwidth of the all-node
+                               adjoint kernel (timed alone), for FP32, FP32
+                               forward / FP64 adjoint and FP64;
                 multibody_8192x4096  MRT + LES + network tau, 16 cylinders
                                D ~ U[40, 120] (seeded placement), FP32 vs
                                FP64, 5,000 steps, per-body drag error;
@@ -261,12 +262,24 @@ def leg_adjoint(ippi):
         with case.solver(tau, c) as w:  # warm-up: first launches of every kernel
             w.adjoint_drag(20, 1, 10, fp64_adjoint=f64)
         with case.solver(tau, c) as s:
-            s.set_timing(True)
             t0 = time.perf_counter()
             J, gt, gc = s.adjoint_drag(case.nsteps, case.avg_from, case.ckpt, fp64_adjoint=f64)
             wall = time.perf_counter() - t0
+        # the all-node adjoint kernel timed alone: the recompute of the next segment, which otherwise
+        # runs beside the sweep on a second stream, runs between the sweeps (IPPI_ADJ_OVERLAP=0)
+        os.environ["IPPI_ADJ_OVERLAP"] = "0"  # read when the handle is created
+        try:
+            s = case.solver(tau, c)
+        finally:
+            del os.environ["IPPI_ADJ_OVERLAP"]
+        with s:
+            s.set_timing(True)
+            t0 = time.perf_counter()
+            J1, gt1, gc1 = s.adjoint_drag(case.nsteps, case.avg_from, case.ckpt, fp64_adjoint=f64)
+            wall_serial = time.perf_counter() - t0
             ms, n = s.adjoint_timing()
             s.set_timing(False)
+        assert J1 == J and np.array_equal(gt1, gt) and np.array_equal(gc1, gc), "overlap changed the adjoint"
         per = 36 if prec == "fp32" else 72  # forward state read per node
         lam = 72 if (prec == "fp64" or f64) else 36
         tau_b = 4 if prec == "fp32" else 8
@@ -275,6 +288,7 @@ def leg_adjoint(ippi):
         us = ms / max(n, 1) * 1e3
         gbs = bytes_node * nx * ny / (us * 1e-6) / 1e9
         out[name] = {"J": J, "dJ_dcenter": [float(gc[0][0]), float(gc[0][1])], "wall_s": round(wall, 3),
+                     "wall_s_recompute_not_overlapped": round(wall_serial, 3),
                      "adjoint_kernel_avg_us": round(us, 2), "adjoint_kernel_bytes_per_node": bytes_node,
                      "adjoint_kernel_gbs": round(gbs, 1), "adjoint_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
                      "_gt": gt}

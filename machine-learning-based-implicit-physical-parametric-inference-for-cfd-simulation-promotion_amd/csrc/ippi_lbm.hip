@@ -2122,6 +2122,8 @@ struct ippi_handle {
     bool ink_enabled = true;      // environment IPPI_IB_IN_STEP=0 turns it off
     bool ink_wait_skew = false;   // environment IPPI_IB_WAIT_SKEW=1: the band waits for a value never
                                   // stored, so its bounded wait times out (tests of that path only)
+    bool adj_overlap = true;      // environment IPPI_ADJ_OVERLAP=0: the adjoint recomputes each segment on
+                                  // the handle's stream, between the sweeps (to time the sweep's kernels alone)
     bool round32 = false;         // environment IPPI_FP64_ROUND_STORE=1: the FP64 path rounds each stored
                                   // population as the FP32 path stores it (accuracy diagnostic)
     std::vector<int> h_act;      // active node -> patch node (host copy)
@@ -2457,17 +2459,19 @@ int upload_ib(ippi_handle* h) {
 // IB kernel on the handle's stream; slot: read the pulled populations from
 // the slots the previous step kernel wrote (pull steps only).
 template <typename T>
-void launch_ib_to(ippi_handle* h, const T* src, bool pull, const IbDev& d, bool slot = false, bool gather = false) {
+void launch_ib_to(ippi_handle* h, const T* src, bool pull, const IbDev& d, bool slot = false, bool gather = false,
+                  hipStream_t st = nullptr) {
     dim3 g(h->nblk_ib), b(IB_T);
+    if (!st) st = h->s;
     const int nx = h->c.nx, ny = h->c.ny, na = h->nact, ga = gather ? 1 : 0;
     const T* sl = (const T*)h->d_slot;
 #define IPPI_IB_LAUNCH(E)                                                                                     \
     if (slot)                                                                                                 \
-        hipLaunchKernelGGL((k_ib<T, true, E, true>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na, ga);        \
+        hipLaunchKernelGGL((k_ib<T, true, E, true>), g, b, 0, st, src, h->mac, d, nx, ny, sl, na, ga);          \
     else if (pull)                                                                                            \
-        hipLaunchKernelGGL((k_ib<T, true, E, false>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na, ga);       \
+        hipLaunchKernelGGL((k_ib<T, true, E, false>), g, b, 0, st, src, h->mac, d, nx, ny, sl, na, ga);         \
     else                                                                                                      \
-        hipLaunchKernelGGL((k_ib<T, false, E, false>), g, b, 0, h->s, src, h->mac, d, nx, ny, sl, na, ga);
+        hipLaunchKernelGGL((k_ib<T, false, E, false>), g, b, 0, st, src, h->mac, d, nx, ny, sl, na, ga);
     if (h->ell == 8) {
         IPPI_IB_LAUNCH(8)
     } else {
@@ -2836,48 +2840,115 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     const dim3 grid_adj((nx + BX - 1) / BX, ny), grid_act((h->nact + 255) / 256);
     const StepKernel<T> kpull = pick_step<T, true, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB);
     const StepKernel<T> kfirst = pick_step<T, false, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB);
-    int cl = 0;
-    for (int sg = nseg - 1; sg >= 0; sg--) {
+    // Two buffer sets for the recomputed segments (states, patch forces,
+    // marker forces): while the reverse sweep of segment sg runs on the
+    // handle's stream, segment sg - 1 is recomputed into the other set on a
+    // second stream (default priority). The sweep's IB kernels are small and
+    // leave most of the GPU idle; the recompute fills it. Used while the
+    // second set of states fits in 8 GB; otherwise, or with one segment, the
+    // two phases alternate on the handle's stream. The results are the same
+    // either way: the same kernels read the same inputs.
+    struct Overlap {
+        hipStream_t s = nullptr;
+        hipEvent_t rec[2] = {nullptr, nullptr}, rev[2] = {nullptr, nullptr};
+        ~Overlap() {
+            if (s) {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+            for (hipEvent_t e : rec)
+                if (e) (void)hipEventDestroy(e);
+            for (hipEvent_t e : rev)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } ov;
+    const bool overlap = h->adj_overlap && nseg > 1 && sizeof(T) * SZ * (K - 1) <= ((size_t)8 << 30);
+    T* segs[2] = {seg, seg};
+    double* pfs[2] = {pf, pf};
+    double* Fss[2] = {Fs, Fs};
+    if (overlap) {
+        HIPC(h, mem.alloc(&segs[1], sizeof(T) * SZ * (K - 1)));
+        HIPC(h, mem.alloc(&pfs[1], sizeof(double) * 2 * (size_t)h->npn * K));
+        HIPC(h, mem.alloc(&Fss[1], sizeof(double) * 2 * (size_t)nm * K));
+        HIPC(h, hipMemsetAsync(pfs[1], 0, sizeof(double) * 2 * (size_t)h->npn * K, h->s));  // inactive patch nodes
+        HIPC(h, hipStreamCreateWithFlags(&ov.s, hipStreamNonBlocking));
+        for (int b = 0; b < 2; b++) {
+            HIPC(h, hipEventCreateWithFlags(&ov.rec[b], hipEventDisableTiming));
+            HIPC(h, hipEventCreateWithFlags(&ov.rev[b], hipEventDisableTiming));
+        }
+        // the second stream starts after the forward pass and the clears
+        HIPC(h, hipEventRecord(ov.rev[0], h->s));
+        HIPC(h, hipStreamWaitEvent(ov.s, ov.rev[0], 0));
+    }
+    // state i of segment sg: its checkpoint, then the recomputed states
+    auto state = [&](int sg, int i) { return i == 0 ? ck + SZ * sg : segs[sg & 1] + SZ * (i - 1); };
+    // recompute segment sg on stream st: states, patch forces and marker forces
+    auto recompute = [&](int sg, hipStream_t st) {
         const int t0 = sg * K, Ks = std::min(K, nsteps - t0);
-        auto state = [&](int i) { return i == 0 ? ck + SZ * sg : seg + SZ * (i - 1); };
-        // recompute the segment: states, patch forces and marker forces
+        double* pfb = pfs[sg & 1];
+        double* Fsb = Fss[sg & 1];
+        StepArgs sr = sa;
         for (int i = 0; i < Ks; i++) {
             const bool pull = !(t0 + i == 0 && phase0 == 0);
-            IbDev d{h->d_w, h->d_pidx, h->d_ud, ui, Fs + 2 * (size_t)nm * i, h->d_blk, h->d_act, h->d_ecnt,
-                    h->d_emk, h->d_ecw, pf + 2 * (size_t)h->npn * i, nullptr};
-            launch_ib_to<T>(h, state(i), pull, d);
+            IbDev d{h->d_w, h->d_pidx, h->d_ud, ui, Fsb + 2 * (size_t)nm * i, h->d_blk, h->d_act, h->d_ecnt,
+                    h->d_emk, h->d_ecw, pfb + 2 * (size_t)h->npn * i, nullptr};
+            launch_ib_to<T>(h, state(sg, i), pull, d, false, false, st);
             if (i + 1 < Ks) {
-                sa.patch_f = pf + 2 * (size_t)h->npn * i;
-                hipLaunchKernelGGL(pull ? kpull : kfirst, pull ? grid_pull : grid_first, dim3(BX), 0, h->s,
-                                   (const T*)state(i), state(i + 1), h->mac, sa);
+                sr.patch_f = pfb + 2 * (size_t)h->npn * i;
+                hipLaunchKernelGGL(pull ? kpull : kfirst, pull ? grid_pull : grid_first, dim3(BX), 0, st,
+                                   (const T*)state(sg, i), state(sg, i + 1), h->mac, sr);
             }
         }
+    };
+    int cl = 0;
+    if (overlap) {
+        recompute(nseg - 1, ov.s);
+        HIPC(h, hipEventRecord(ov.rec[(nseg - 1) & 1], ov.s));
+    }
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int t0 = sg * K, Ks = std::min(K, nsteps - t0);
+        if (overlap) {
+            if (sg > 0) {
+                // set (sg - 1) & 1 was last read by the sweep of segment sg + 1
+                if (sg + 1 < nseg) HIPC(h, hipStreamWaitEvent(ov.s, ov.rev[(sg + 1) & 1], 0));
+                recompute(sg - 1, ov.s);
+                HIPC(h, hipEventRecord(ov.rec[(sg - 1) & 1], ov.s));
+            }
+            HIPC(h, hipStreamWaitEvent(h->s, ov.rec[sg & 1], 0));
+        } else {
+            recompute(sg, h->s);
+        }
         HIPC(h, hipGetLastError());
+        const double* pfb = pfs[sg & 1];
+        const double* Fsb = Fss[sg & 1];
         for (int i = Ks - 1; i >= 0; i--) {
             const int t = t0 + i;
             const bool pull = !(t == 0 && phase0 == 0);
             const double cj = t + 1 >= avg_from ? 1.0 / nwin : 0.0;
             const A* lin = lam + SZ * cl;
             A* lout = lam + SZ * (cl ^ 1);
-            aa.patch_f = pf + 2 * (size_t)h->npn * i;
+            aa.patch_f = pfb + 2 * (size_t)h->npn * i;
             aa.store = t > 0;
-            ad.F = Fs + 2 * (size_t)nm * i;
+            ad.F = Fsb + 2 * (size_t)nm * i;
             if (pull)
-                hipLaunchKernelGGL((k_adj_act<T, A, true>), grid_act, dim3(256), 0, h->s, lin, state(i), h->mac, ad, aa);
+                hipLaunchKernelGGL((k_adj_act<T, A, true>), grid_act, dim3(256), 0, h->s, lin, state(sg, i), h->mac,
+                                   ad, aa);
             else
-                hipLaunchKernelGGL((k_adj_act<T, A, false>), grid_act, dim3(256), 0, h->s, lin, state(i), h->mac, ad, aa);
+                hipLaunchKernelGGL((k_adj_act<T, A, false>), grid_act, dim3(256), 0, h->s, lin, state(sg, i), h->mac,
+                                   ad, aa);
             hipLaunchKernelGGL(k_adj_mk, dim3(nb), dim3(IB_T), 0, h->s, ad, cj);
             if (h->timing) h->ev_adj.push_back(new_event(h));
             if (pull)
-                hipLaunchKernelGGL((k_adj_step<T, A, true>), grid_adj, dim3(BX), 0, h->s, lin, lout, state(i), h->mac,
-                                   gacc, aa);
+                hipLaunchKernelGGL((k_adj_step<T, A, true>), grid_adj, dim3(BX), 0, h->s, lin, lout, state(sg, i),
+                                   h->mac, gacc, aa);
             else
-                hipLaunchKernelGGL((k_adj_step<T, A, false>), grid_adj, dim3(BX), 0, h->s, lin, lout, state(i), h->mac,
-                                   gacc, aa);
+                hipLaunchKernelGGL((k_adj_step<T, A, false>), grid_adj, dim3(BX), 0, h->s, lin, lout, state(sg, i),
+                                   h->mac, gacc, aa);
             if (h->timing) h->ev_adj.push_back(new_event(h));
             cl ^= 1;
         }
         hipLaunchKernelGGL(k_adj_flush<A>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->s, gacc, gtot, N);
+        if (overlap) HIPC(h, hipEventRecord(ov.rev[sg & 1], h->s));
         HIPC(h, hipGetLastError());
     }
     if (g_tau) HIPC(h, hipMemcpyAsync(g_tau, gtot, sizeof(double) * N, hipMemcpyDeviceToHost, h->s));
@@ -2909,6 +2980,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (const char* e = std::getenv("IPPI_IB_IN_STEP")) h->ink_enabled = std::strcmp(e, "0") != 0;
     if (const char* e = std::getenv("IPPI_IB_WAIT_SKEW")) h->ink_wait_skew = std::strcmp(e, "1") == 0;
     if (const char* e = std::getenv("IPPI_FP64_ROUND_STORE")) h->round32 = std::strcmp(e, "1") == 0;
+    if (const char* e = std::getenv("IPPI_ADJ_OVERLAP")) h->adj_overlap = std::strcmp(e, "0") != 0;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;
     auto bail = [&](int code) {
