@@ -71,6 +71,22 @@ def test_checkpoint_interval_does_not_change_result(ippi):
         np.testing.assert_allclose(gc, ref[2], rtol=1e-14)
 
 
+@pytest.mark.parametrize("precision,ckpt", [("fp64", 7), ("fp32", 4), ("fp32", 1)])
+def test_recompute_overlap_is_bitwise_neutral(ippi, monkeypatch, precision, ckpt):
+    """The next segment's recompute on a second stream beside the reverse
+    sweep (the default) and between the sweeps (IPPI_ADJ_OVERLAP=0, read
+    when the handle is created) give bit-identical objective and gradients,
+    down to one-step segments."""
+    print(f"[adjoint] recompute overlap {precision} ckpt={ckpt}", flush=True)
+    tau = small_tau()
+    on = small_gpu(ippi, precision, tau, 60, 31, ckpt)
+    monkeypatch.setenv("IPPI_ADJ_OVERLAP", "0")
+    off = small_gpu(ippi, precision, tau, 60, 31, ckpt)
+    monkeypatch.delenv("IPPI_ADJ_OVERLAP")
+    assert on[0] == off[0]
+    assert np.array_equal(on[1], off[1]) and np.array_equal(on[2], off[2])
+
+
 def test_fp32_adjoint_small(ippi, oracle):
     print("[adjoint] fp32 vs oracle, small", flush=True)
     tau = small_tau()
