@@ -1,51 +1,52 @@
 """Benchmark of the MI355X D2Q9 time step of IPPI-LBM (one JSON line).
 
-Headline workload (`value`): MRT collision + hybrid Smagorinsky LES
-(Cs = 0.1), 2048 x 1024 nodes, one immersed-boundary cylinder of diameter
-80 lu (251 markers) at (512, 512), Re = 1e4 (tau0 = 0.5 + 3 U D / Re =
-0.50192), inflow U = 0.08 with seeded +-1e-3 velocity noise, non-
-equilibrium-extrapolation inlet/outlet and free-slip walls; populations
-stored in FP32 as f - w. A "step" is one full time step (one fused
-stream-collide kernel, which also computes the immersed boundary), 20,000
-steps after 50 warm-up steps.
+Headline workload (`value`): the FP32 fused MRT + hybrid Smagorinsky LES
+(Cs = 0.1) + immersed-boundary step with the relaxation time inferred by the
+fused network (6 -> 32 -> 32 -> 1, SiLU, softplus + 0.5, bf16 MFMA with
+FP32 accumulation, guard >= 0.501) at every node, on 4096 x 2048 nodes: one
+cylinder of diameter 160 lu (503 markers) at (1024, 1024), Re = 1e4
+(tau0 = 0.5 + 3 U D / Re = 0.50384), inflow U = 0.08 with seeded +-1e-3
+velocity noise, non-equilibrium-extrapolation inlet/outlet and free-slip
+walls, populations stored in FP32 as f - w. The network's weights are the
+seeded Xavier draw of workloads.xavier_blob(7). A "step" is one full time
+step: one fused stream-collide kernel, which also computes the immersed
+boundary and the network.
 
 Reported beside it:
-  fp64_mlups    the same workload on the plain FP64 path, and the ratio;
-  accuracy      relative L2 error of velocity and density of the FP32 path
-                against the FP64 GPU path after the same steps (same seeded
-                start), and of the drag record;
-  roofline      achieved HBM bandwidth of the fused stream-collide kernel:
-                72 algorithmic bytes per node (9 FP32 loads + 9 FP32
-                stores) x nodes / average launch time, from one HIP event
-                pair on the library's stream around the timed steps (the
-                launches back to back; each launch bracketed by its own
-                events is reported beside it); `traffic` = HBM bytes per launch from
-                the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary,
-                scaled by the same counters on a plain copy of known size;
-  cpu_baseline  the FP64 CPU restatement (oracle/, OpenMP) on a bounded
-                sample of the same workload on this host's cores;
-  legs          the other workloads, each measured the same way:
-                nn_4096x2048   MRT + LES + IB (D = 160) with the fused
-                               network tau (seeded Xavier weights, guard
-                               active), FP32 vs FP64, 10,000 steps, and FP64
-                               over FP32-rounded storage vs FP64;
-                adjoint_1024x512  BGK + IB discrete adjoint of the mean drag
-                               over steps 2001-4000, checkpoints every 100,
-                               tau field 0.6 +- 0.02: wall time of forward +
-                               reverse sweep (recompute beside the sweep, and
-                               between the sweeps), bandwidth of the all-node
-                               adjoint kernel (timed alone), for FP32, FP32
-                               forward / FP64 adjoint and FP64;
-                multibody_8192x4096  MRT + LES + network tau, 16 cylinders
-                               D ~ U[40, 120] (seeded placement), FP32 vs
-                               FP64, 5,000 steps, per-body drag error;
-                accuracy_2048x1024  growth of the FP32-vs-FP64 velocity error
-                               on the headline workload over 20,000 steps,
-                               beside FP64 arithmetic over FP32-rounded
-                               storage and FP64 from a perturbed start;
-                flatplate_2401x501  the reference's own run (flat plate,
-                               BGK + LES, Re = 5e5): FP32 and FP64 MLUPS and
-                               the same error comparison.
+  fp64_mlups        the plain FP64 path on the same grid and body (MRT + LES +
+                    IB, no network), and speedup_vs_fp64 = value / it;
+  fp64_nn_mlups     the FP64 path with the same network (evaluated in FP64);
+  rel_l2_u_vs_fp64  FP32 against the FP64 path with the network after the
+                    timed steps (same seeded start); accuracy_gate_met
+                    uses the 10,000-step figure of the accuracy leg when
+                    that leg ran, else this one (accuracy_gate_steps says which);
+  roofline          achieved HBM bandwidth of the fused kernel: 72 algorithmic
+                    bytes per node (9 FP32 loads + 9 FP32 stores) x nodes /
+                    average launch time, from one HIP event pair on the
+                    library's stream around the timed steps; `traffic` = HBM
+                    bytes per launch from the committed rocprofv3 FETCH_SIZE /
+                    WRITE_SIZE summary, scaled by the same counters on a plain
+                    copy of known size;
+  cpu_baseline      the FP64 CPU restatement (oracle/, OpenMP) on a bounded
+                    sample of the same workload on this host's cores;
+  legs              the other workloads, each measured the same way:
+                    accuracy_nn_4096x2048  the headline workload over 10,000
+                                   steps: FP32 vs FP64 (both with the
+                                   network), FP64 over FP32-rounded storage,
+                                   tau error, and the plain-bf16 network;
+                    mrt_les_ib_2048x1024   MRT + LES + IB without the network,
+                                   D = 80, Re = 1e4: FP32 and FP64 MLUPS,
+                                   roofline, and the FP32-vs-FP64 error over
+                                   20,000 steps;
+                    adjoint_1024x512  BGK + IB discrete adjoint of the mean drag
+                                   over steps 2001-4000, checkpoints every 100,
+                                   tau field 0.6 +- 0.02;
+                    multibody_8192x4096  MRT + LES + network tau, 16 cylinders
+                                   D ~ U[40, 120] (seeded placement), FP32 vs
+                                   FP64, 5,000 steps, per-body drag error;
+                    flatplate_2401x501  the reference's own run (flat plate,
+                                   BGK + LES, Re = 5e5);
+                    tausearch      FindAndWriteTau's candidate search.
 
 `python bench.py --copy-probe MB` runs only a plain device copy of MB
 megabytes (the calibration program for the rocprofv3 traffic counters).
@@ -64,16 +65,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "machine-learning-based-implicit-physical-parametric-inference-for-cfd-simulation-promotion_amd")
 METRIC = "MLUPS + % HBM roofline, FP32/NN-tau vs plain FP64 at ≤1e-5 rel L2 velocity"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_step_f32_ib_mrt_les_ib_2048x1024.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_step_f32_ib_nn_4096x2048.md")
 
-NX, NY = 2048, 1024
+# headline: MRT + LES + IB + network tau, 4096 x 2048
+NX, NY = 4096, 2048
 U0 = 0.08
-DIAM, NMARK = 80.0, 251
-CX, CY = 512.0, 512.0
+DIAM = 160.0
+NMARK = int(round(3.141592653589793 * DIAM))  # 503 markers, ~1 lu apart
+CX, CY = NX / 4.0, NY / 2.0
 RE = 1e4
 TAU0 = 0.5 + 3.0 * U0 * DIAM / RE
 CS = 0.1
 SEED = 20240601
+NET_SEED = 7
+
+# MRT + LES + IB without the network, 2048 x 1024 (leg mrt_les_ib_2048x1024)
+NX2, NY2, DIAM2, NMARK2, CX2, CY2 = 2048, 1024, 80.0, 251, 512.0, 512.0
+TAU02 = 0.5 + 3.0 * U0 * DIAM2 / RE
 
 
 def load_pkg():
@@ -132,10 +140,19 @@ def timed_run_events(s, steps, warmup):
 
 
 # ------------------------------------------------------------------ headline
-def make_headline(ippi, precision):
-    s = ippi.Solver(NX, NY, "mrt", precision, "inflow", les=True, tau0=TAU0, cs=CS, u_in=(U0, 0.0))
-    s.init_fields(SEED, 1.0, (U0, 0.0), 1e-3)
+def blob():
+    from ippi_amd import workloads
+    return workloads.xavier_blob(NET_SEED)
+
+
+def make_headline(ippi, precision, nn=True, tau_mode="nn", noise=1e-3, force_history=0):
+    """The headline workload; nn=False gives the plain path (no network)."""
+    s = ippi.Solver(NX, NY, "mrt", precision, "inflow", les=True, tau_mode=tau_mode if nn else "const",
+                    tau0=TAU0, cs=CS, u_in=(U0, 0.0), force_history=force_history)
+    s.init_fields(SEED, 1.0, (U0, 0.0), noise)
     s.add_body(CX, CY, DIAM, NMARK)
+    if nn:
+        s.set_mlp(blob())
     return s
 
 
@@ -150,14 +167,16 @@ def read_traffic():
         return None
 
 
-def cpu_baseline(seconds_target=12.0):
+def cpu_baseline(seconds_target=15.0):
     """FP64 oracle (OpenMP) on the headline workload, bounded number of steps."""
     sys.path.insert(0, ROOT)
     from oracle import pyoracle
     threads = min(16, os.cpu_count() or 1)
-    o = pyoracle.Sim(NX, NY, "mrt", "inflow", les=True, tau0=TAU0, cs=CS, u_in=(U0, 0.0), threads=threads)
+    o = pyoracle.Sim(NX, NY, "mrt", "inflow", les=True, tau_mode="nn", tau0=TAU0, cs=CS, u_in=(U0, 0.0),
+                     threads=threads)
     o.init_fields(SEED, 1.0, (U0, 0.0), 1e-3)
     o.add_body(CX, CY, DIAM, NMARK)
+    o.set_mlp(blob())
     t0 = time.perf_counter()
     o.step(1)
     one = time.perf_counter() - t0
@@ -167,83 +186,106 @@ def cpu_baseline(seconds_target=12.0):
     dt = time.perf_counter() - t0
     o.close()
     return {"value": round(NX * NY * n / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"oracle/ FP64 MRT+LES+IB, {NX}x{NY}, {n} steps after 1 warm-up step, {threads} OpenMP threads"}
+            "sample": f"oracle/ FP64 MRT+LES+IB+network tau, {NX}x{NY}, {n} steps after 1 warm-up step, "
+                      f"{threads} OpenMP threads"}
 
 
 # ---------------------------------------------------------------- other legs
-def leg_nn(ippi, steps, warmup, profile_steps):
-    from ippi_amd import workloads
-    nx, ny, D = 4096, 2048, 160.0
-    U, tau0 = 0.08, 0.5 + 3 * 0.08 * 160.0 / 1e4
-    blob = workloads.xavier_blob(7)
-    out = {"workload": f"MRT+LES(Cs=0.1)+IB cylinder D=160 ({workloads.markers_for(D)} markers) + fused network "
-                       "tau (seeded Xavier 6-32-32-1, guard >= 0.501), 4096x2048, inflow 0.08 +-1e-3, Re=1e4",
-           "steps": steps}
-    res = {}
-    for prec in ("fp32", "fp64"):
-        s = ippi.Solver(nx, ny, "mrt", prec, "inflow", les=True, tau_mode="nn", tau0=tau0, cs=CS, u_in=(U, 0.0))
-        s.init_fields(SEED, 1.0, (U, 0.0), 1e-3)
-        s.add_body(nx / 4, ny / 2, D, workloads.markers_for(D))
-        s.set_mlp(blob)
-        log(f"nn 4096x2048 {prec}: {steps} steps")
-        dt, k_timed = timed_run_events(s, steps, warmup)
-        res[prec] = (nx * ny * steps / dt / 1e6, s.fields(), s.tau())
-        if prec == "fp32":
-            k_each, ib_us = kernel_avg_us(s, profile_steps)
-            k_us = k_timed if k_timed is not None else k_each
-            gbs = 72 * nx * ny / (k_us * 1e-6) / 1e9
-            out.update({"fused_kernel_avg_us": round(k_us, 2), "fused_kernel_avg_us_timed_individually": round(k_each, 2),
-                        "fused_kernel_gbs": round(gbs, 1), "fused_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
-                        "ib_kernel_avg_us": round(ib_us, 2)})
-        s.close()
-    # the storage format alone: FP64 arithmetic whose stored populations are rounded as FP32 stores them
+def leg_accuracy_nn(ippi, steps):
+    """The headline workload over `steps` steps: FP32 against FP64 (both
+    with the network), beside FP64 arithmetic over FP32-rounded storage
+    (IPPI_FP64_ROUND_STORE=1: the storage format alone), the relaxation
+    time's error, the network's acceptance, and the plain-bf16 network
+    variant after 1,000 steps."""
+    out = {"workload": "headline workload (MRT+LES+IB+network tau, 4096x2048)", "steps": steps}
+    runs = {"fp32": make_headline(ippi, "fp32"), "fp64": make_headline(ippi, "fp64")}
     os.environ["IPPI_FP64_ROUND_STORE"] = "1"  # read when the handle is created
     try:
-        s = ippi.Solver(nx, ny, "mrt", "fp64", "inflow", les=True, tau_mode="nn", tau0=tau0, cs=CS, u_in=(U, 0.0))
+        runs["fp64_fp32_storage"] = make_headline(ippi, "fp64")
     finally:
         del os.environ["IPPI_FP64_ROUND_STORE"]
-    s.init_fields(SEED, 1.0, (U, 0.0), 1e-3)
-    s.add_body(nx / 4, ny / 2, D, workloads.markers_for(D))
-    s.set_mlp(blob)
-    log(f"nn 4096x2048 fp64 with FP32-rounded storage: {warmup + steps} steps")
-    s.step(warmup + steps)
-    out["rel_l2_u_fp64_fp32_storage_vs_fp64"] = vel_err(s.fields(), res["fp64"][1])
-    s.close()
-    # error growth: FP32 against FP64 after 1,000 steps from the same start, beside FP64 against FP64
-    # from a start perturbed by 1e-9 (the wake amplifies any perturbation; at `steps` only the first
-    # comparison is made, above)
-    early, etau = {}, {}
-    for name, prec, mode, noise in (("fp32", "fp32", "nn", 1e-3), ("fp64", "fp64", "nn", 1e-3),
-                                    ("fp64_perturbed", "fp64", "nn", 1e-3 + 1e-9),
-                                    ("fp32_bf16_plain", "fp32", "nn_bf16", 1e-3)):
-        s = ippi.Solver(nx, ny, "mrt", prec, "inflow", les=True, tau_mode=mode, tau0=tau0, cs=CS, u_in=(U, 0.0))
-        s.init_fields(SEED, 1.0, (U, 0.0), noise)
-        s.add_body(nx / 4, ny / 2, D, workloads.markers_for(D))
-        s.set_mlp(blob)
-        s.step(1000)
-        early[name] = s.fields()
-        etau[name] = s.tau()
-        if name == "fp32_bf16_plain":  # speed of the plain-bf16 network (IPPI_TAU_NN_BF16)
-            dtb, kb = timed_run_events(s, min(steps, 2000), warmup)
-            gbs_b = 72 * nx * ny / (kb * 1e-6) / 1e9 if kb else None
-            out["bf16_plain"] = {"mlups_fp32": round(nx * ny * min(steps, 2000) / dtb / 1e6, 1),
-                                 "fused_kernel_avg_us": round(kb, 2) if kb else None,
-                                 "fused_kernel_roofline_frac": round(gbs_b / HBM_PEAK_GBS, 4) if gbs_b else None}
+    marks = [m for m in (1000, 2000, 5000, 10000, 20000) if m < steps] + [steps]
+    out["marks"] = marks
+    out["fp32_vs_fp64"], out["fp64_fp32_storage_vs_fp64"], out["tau_fp32_vs_fp64"] = [], [], []
+    done = 0
+    for m in marks:
+        log(f"accuracy (network, {NX}x{NY}): to step {m}")
+        for s in runs.values():
+            s.step(m - done)
+        done = m
+        f = {k: s.fields() for k, s in runs.items()}
+        out["fp32_vs_fp64"].append(vel_err(f["fp32"], f["fp64"]))
+        out["fp64_fp32_storage_vs_fp64"].append(vel_err(f["fp64_fp32_storage"], f["fp64"]))
+        t32, t64 = runs["fp32"].tau(), runs["fp64"].tau()
+        out["tau_fp32_vs_fp64"].append(float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64)))
+    out["network_tau_accepted_frac"] = float(np.mean(runs["fp64"].tau() > 0.501))
+    r32, r64 = runs["fp32"].body_forces(), runs["fp64"].body_forces()
+    n = min(len(r32), len(r64)) // 2
+    out["mean_drag_rel_err_last_half"] = float(abs(r32[-n:, 0, 0].mean() - r64[-n:, 0, 0].mean()) /
+                                               abs(r64[-n:, 0, 0].mean()))
+    for s in runs.values():
         s.close()
-    out["rel_l2_u_vs_fp64_at_1000"] = vel_err(early["fp32"], early["fp64"])
-    out["rel_l2_u_fp64_perturbed_vs_fp64_at_1000"] = vel_err(early["fp64_perturbed"], early["fp64"])
-    out["bf16_plain"].update({
-        "rel_l2_u_vs_fp64_at_1000": vel_err(early["fp32_bf16_plain"], early["fp64"]),
-        "rel_l2_tau_vs_fp64_at_1000": float(np.linalg.norm(etau["fp32_bf16_plain"] - etau["fp64"]) /
-                                            np.linalg.norm(etau["fp64"])),
-        "rel_l2_tau_compensated_vs_fp64_at_1000": float(np.linalg.norm(etau["fp32"] - etau["fp64"]) /
-                                                        np.linalg.norm(etau["fp64"]))})
-    t32, t64 = res["fp32"][2], res["fp64"][2]
-    out.update({"mlups_fp32": round(res["fp32"][0], 1), "mlups_fp64": round(res["fp64"][0], 1),
-                "speedup_vs_fp64": round(res["fp32"][0] / res["fp64"][0], 3),
-                "rel_l2_u_vs_fp64": vel_err(res["fp32"][1], res["fp64"][1]),
-                "network_tau_accepted_frac": float(np.mean(t64 > 0.501)),
-                "rel_l2_tau_vs_fp64": float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))})
+    out["rel_l2_u_vs_fp64"] = out["fp32_vs_fp64"][-1]
+    # the plain-bf16 network (IPPI_TAU_NN_BF16): speed and error after 1,000 steps
+    a, c = make_headline(ippi, "fp32", tau_mode="nn_bf16"), make_headline(ippi, "fp64")
+    a.step(1000)
+    c.step(1000)
+    e_u, e_t = vel_err(a.fields(), c.fields()), float(np.linalg.norm(a.tau() - c.tau()) / np.linalg.norm(c.tau()))
+    dtb, kb = timed_run_events(a, 500, 10)
+    a.close()
+    c.close()
+    out["bf16_plain"] = {"rel_l2_u_vs_fp64_at_1000": e_u, "rel_l2_tau_vs_fp64_at_1000": e_t,
+                         "mlups_fp32": round(NX * NY * 500 / dtb / 1e6, 1),
+                         "fused_kernel_avg_us": round(kb, 2) if kb else None}
+    return out
+
+
+def leg_mrt_les_ib(ippi, steps, warmup, profile_steps, marks=(1000, 2000, 5000, 10000, 20000)):
+    """MRT + LES + IB without the network on 2048 x 1024 (D = 80, Re = 1e4):
+    FP32 and FP64 MLUPS, the fused kernel's roofline, and the FP32-vs-FP64
+    velocity error over 20,000 steps beside FP64 over FP32-rounded storage
+    and FP64 from a start perturbed by 1 part in 1e6 of the noise."""
+    def make(prec, noise=1e-3):
+        s = ippi.Solver(NX2, NY2, "mrt", prec, "inflow", les=True, tau0=TAU02, cs=CS, u_in=(U0, 0.0))
+        s.init_fields(SEED, 1.0, (U0, 0.0), noise)
+        s.add_body(CX2, CY2, DIAM2, NMARK2)
+        return s
+    out = {"workload": "MRT+LES(Cs=0.1)+IB cylinder D=80 (251 markers), 2048x1024, inflow 0.08 +-1e-3, Re=1e4",
+           "steps": steps}
+    for prec in ("fp32", "fp64"):
+        s = make(prec)
+        log(f"mrt+les+ib 2048x1024 {prec}: {steps} steps")
+        dt, k_timed = timed_run_events(s, steps, warmup)
+        out[f"mlups_{prec}"] = round(NX2 * NY2 * steps / dt / 1e6, 1)
+        if prec == "fp32":
+            k_each, _ = kernel_avg_us(s, profile_steps)
+            k_us = k_timed if k_timed is not None else k_each
+            gbs = 72 * NX2 * NY2 / (k_us * 1e-6) / 1e9
+            out.update({"fused_kernel_avg_us": round(k_us, 3), "fused_kernel_gbs": round(gbs, 1),
+                        "fused_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4)})
+        s.close()
+    out["speedup_vs_fp64"] = round(out["mlups_fp32"] / out["mlups_fp64"], 3)
+    runs = {"fp32": make("fp32"), "fp64": make("fp64"), "fp64_perturbed": make("fp64", 1e-3 * (1 + 1e-6))}
+    os.environ["IPPI_FP64_ROUND_STORE"] = "1"  # read when the handle is created
+    try:
+        runs["fp64_fp32_storage"] = make("fp64")
+    finally:
+        del os.environ["IPPI_FP64_ROUND_STORE"]
+    out["error_steps"] = list(marks)
+    for k in ("fp32_vs_fp64", "fp64_fp32_storage_vs_fp64", "fp64_perturbed_vs_fp64"):
+        out[k] = []
+    done = 0
+    for m in marks:
+        log(f"accuracy (2048x1024): to step {m}")
+        for s in runs.values():
+            s.step(m - done)
+        done = m
+        f = {k: s.fields() for k, s in runs.items()}
+        out["fp32_vs_fp64"].append(vel_err(f["fp32"], f["fp64"]))
+        out["fp64_fp32_storage_vs_fp64"].append(vel_err(f["fp64_fp32_storage"], f["fp64"]))
+        out["fp64_perturbed_vs_fp64"].append(vel_err(f["fp64_perturbed"], f["fp64"]))
+    for s in runs.values():
+        s.close()
     return out
 
 
@@ -335,41 +377,6 @@ def leg_multibody(ippi, steps, warmup):
     return out
 
 
-def leg_accuracy(ippi, marks=(1000, 2000, 5000, 10000, 20000)):
-    """Velocity error growth on the headline workload: FP32 vs FP64, next
-    to two FP64 runs against the same FP64 run:
-    - FP64 arithmetic whose every stored population is rounded as the FP32
-      path stores it (f - w to float; IPPI_FP64_ROUND_STORE=1): the error
-      of the storage format alone, whatever the arithmetic;
-    - FP64 started from initial noise whose amplitude differs by 1 part in
-      1e6 (a one-time perturbation of ~1e-9 in u), for the flow's own
-      sensitivity (vortex shedding at Re = 1e4)."""
-    a32, a64 = make_headline(ippi, "fp32"), make_headline(ippi, "fp64")
-    p64 = ippi.Solver(NX, NY, "mrt", "fp64", "inflow", les=True, tau0=TAU0, cs=CS, u_in=(U0, 0.0))
-    p64.init_fields(SEED, 1.0, (U0, 0.0), 1e-3 * (1 + 1e-6))
-    p64.add_body(CX, CY, DIAM, NMARK)
-    os.environ["IPPI_FP64_ROUND_STORE"] = "1"  # read when the handle is created
-    try:
-        r64s = make_headline(ippi, "fp64")
-    finally:
-        del os.environ["IPPI_FP64_ROUND_STORE"]
-    runs = (a32, a64, p64, r64s)
-    out = {"steps": list(marks), "fp32_vs_fp64": [], "fp64_fp32_storage_vs_fp64": [], "fp64_perturbed_vs_fp64": []}
-    done = 0
-    for m in marks:
-        log(f"accuracy: to step {m}")
-        for s in runs:
-            s.step(m - done)
-        done = m
-        r32, r64, rp, rs = a32.fields(), a64.fields(), p64.fields(), r64s.fields()
-        out["fp32_vs_fp64"].append(vel_err(r32, r64))
-        out["fp64_fp32_storage_vs_fp64"].append(vel_err(rs, r64))
-        out["fp64_perturbed_vs_fp64"].append(vel_err(rp, r64))
-    for s in runs:
-        s.close()
-    return out
-
-
 def leg_tausearch(ippi, n=65536, n_cpu=48):
     """FindAndWriteTau's candidate search (20,001 candidates, FP64) for n
     nodes on the GPU, next to the oracle's sequential loop on a sample of
@@ -457,10 +464,10 @@ def pattern_probe(ippi):
     XCD-grouped block order, next to a plain nine-array copy, at the
     headline and network-leg sizes."""
     out = {}
-    for (nx, ny) in ((NX, NY), (4096, 2048)):
+    for (nx, ny) in ((2048, 1024), (4096, 2048), (8192, 4096)):
         for prec in ("fp32", "fp64"):
             bpn = 72 if prec == "fp32" else 144
-            for mode in ("pull", "pull_xcd", "copy"):
+            for mode in ("pull", "pull_xcd", "copy", "copy16"):
                 us = ippi.pattern_probe(nx, ny, prec, mode, reps=300)
                 gbs = bpn * nx * ny / (us * 1e-6) / 1e9
                 out[f"{nx}x{ny}_{prec}_{mode}"] = {"us": round(us, 3), "gbs": round(gbs, 1),
@@ -469,44 +476,23 @@ def pattern_probe(ippi):
     print(json.dumps({"pattern_probe": out}), flush=True)
 
 
-def issue_probe(ippi):
-    """Cost of the instruction kinds the network variant issues, from
-    register-only loops (ippi_issue_probe): per instruction relative to an
-    FP32 FMA, and MFMA mixed with FMAs against the sum of the two alone
-    (a ratio of 1 means the two do not overlap)."""
-    us = {k: ippi.issue_probe(k) for k in ippi.ISSUE_KINDS}
-    per = {"fma": 64, "exp2": 64, "add_rcp": 64, "silu": 16, "mfma_bf16": 16, "mfma_f32": 16}
-    unit = us["fma"] / 64
-    out = {"us_per_launch": {k: round(v, 2) for k, v in us.items()},
-           "cost_in_fma": {"exp2": round(us["exp2"] / 64 / unit, 2),
-                           "rcp": round(us["add_rcp"] / 64 / unit - 1.0, 2),
-                           "silu_exp_add_rcp_mul": round(us["silu"] / 16 / unit, 2),
-                           "mfma_bf16_16x16x32": round(us["mfma_bf16"] / 16 / unit, 2),
-                           "mfma_f32_16x16x4": round(us["mfma_f32"] / 16 / unit, 2)},
-           "mixed_over_sum": {"mfma_bf16+fma": round(us["mfma_bf16+fma"] / (us["mfma_bf16"] + us["fma"]), 3),
-                              "mfma_f32+fma": round(us["mfma_f32+fma"] / (us["mfma_f32"] + us["fma"]), 3)}}
-    log(f"issue probe: {out}")
-    print(json.dumps({"issue_probe": out}), flush=True)
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison of the headline")
-    ap.add_argument("--legs", default="nn,adjoint,multibody,accuracy,flatplate,tausearch",
-                    help="comma list of extra workloads (nn, adjoint, multibody, accuracy, flatplate, tausearch) "
-                         "or 'none'")
-    ap.add_argument("--nn-steps", type=int, default=10000)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparisons of the headline")
+    ap.add_argument("--legs", default="accuracy,mrtles,adjoint,multibody,flatplate,tausearch",
+                    help="comma list of extra workloads (accuracy, mrtles, adjoint, multibody, flatplate, "
+                         "tausearch) or 'none'")
+    ap.add_argument("--acc-steps", type=int, default=10000, help="steps of the headline accuracy leg")
+    ap.add_argument("--mrtles-steps", type=int, default=2000)
     ap.add_argument("--mb-steps", type=int, default=5000)
-    ap.add_argument("--profile-steps", type=int, default=500,
-                    help="steps of the event-timed pass used for the roofline figure")
+    ap.add_argument("--profile-steps", type=int, default=200,
+                    help="steps of the per-launch event-timed pass reported beside the roofline figure")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (counter calibration)")
-    ap.add_argument("--issue-probe", action="store_true",
-                    help="print the instruction issue-cost probe and exit")
     ap.add_argument("--pattern-probe", action="store_true",
                     help="only time the fused step's memory pattern with no arithmetic (bandwidth ceiling)")
     args = ap.parse_args()
@@ -519,30 +505,34 @@ def main():
         return
     ippi = load_pkg()
     ippi.library()
-    if args.issue_probe:
-        issue_probe(ippi)
-        return
     if args.pattern_probe:
         pattern_probe(ippi)
         return
     K, W = args.steps, args.warmup
 
     s32 = make_headline(ippi, "fp32")
-    log(f"fp32 MRT+LES+IB {NX}x{NY}: warmup {W}, timed {K} steps")
+    log(f"fp32 MRT+LES+IB+network {NX}x{NY}: warmup {W}, timed {K} steps")
     dt32, k_us_timed = timed_run_events(s32, K, W)
     mlups32 = NX * NY * K / dt32 / 1e6
     extra = {}
     if not args.no_fp64:
+        p64 = make_headline(ippi, "fp64", nn=False)
+        log(f"fp64 plain MRT+LES+IB {NX}x{NY}: warmup {W}, timed {K} steps")
+        dtp = timed_run(p64, K, W)
+        p64.close()
+        mlups_p64 = NX * NY * K / dtp / 1e6
         s64 = make_headline(ippi, "fp64")
-        log(f"fp64 MRT+LES+IB {NX}x{NY}: warmup {W}, timed {K} steps")
+        log(f"fp64 MRT+LES+IB+network {NX}x{NY}: warmup {W}, timed {K} steps")
         dt64 = timed_run(s64, K, W)
         mlups64 = NX * NY * K / dt64 / 1e6
         r32, r64 = s32.fields(), s64.fields()
         d32, d64 = s32.body_forces(), s64.body_forces()
         n = min(len(d32), len(d64))
-        extra = {"fp64_mlups": round(mlups64, 1), "speedup_vs_fp64": round(mlups32 / mlups64, 3),
-                 "rel_l2_u_vs_fp64": vel_err(r32, r64),
+        extra = {"fp64_mlups": round(mlups_p64, 1), "speedup_vs_fp64": round(mlups32 / mlups_p64, 3),
+                 "fp64_nn_mlups": round(mlups64, 1), "speedup_vs_fp64_nn": round(mlups32 / mlups64, 3),
+                 "rel_l2_u_vs_fp64": vel_err(r32, r64), "rel_l2_u_vs_fp64_steps": K + W,
                  "rel_l2_rho_vs_fp64": float(np.linalg.norm(r32[0] - r64[0]) / np.linalg.norm(r64[0])),
+                 "rel_l2_tau_vs_fp64": float(np.linalg.norm(s32.tau() - s64.tau()) / np.linalg.norm(s64.tau())),
                  "drag_rel_err_vs_fp64": float(np.abs(d32[-n:, 0, 0] - d64[-n:, 0, 0]).mean() /
                                                np.abs(d64[-n:, 0, 0]).mean())}
         s64.close()
@@ -554,33 +544,37 @@ def main():
     s32.close()
     algo_bytes = 72 * NX * NY
     achieved = algo_bytes / (k_us * 1e-6) / 1e9
-    # the same memory pattern with no arithmetic, in the kernel's block order (XCD-grouped rows): the
-    # ceiling the kernel's traffic can reach here; the plain order beside it
-    probe_us = ippi.pattern_probe(NX, NY, "fp32", "pull_xcd", reps=300)
-    probe_plain_us = ippi.pattern_probe(NX, NY, "fp32", "pull", reps=300)
+    # the same memory pattern with no arithmetic (nine pulled loads, nine non-temporal stores): the
+    # ceiling the kernel's traffic can reach at this size
+    probe_us = ippi.pattern_probe(NX, NY, "fp32", "pull", reps=200)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
-                "kernel": "k_step_f32_ib<MRT,INFLOW> (fused step, IB inside)" if ib_us == 0.0
-                          else "k_step_f32<MRT,INFLOW,pull>", "avg_launch_us": round(k_us, 3),
+                "kernel": "k_step_f32_ib<MRT,INFLOW,NN> (fused step, IB and network inside)" if ib_us == 0.0
+                          else "k_step_f32<MRT,INFLOW,pull,NN>", "avg_launch_us": round(k_us, 3),
                 "avg_launch_us_timed_individually": round(k_us_each, 3),
                 "algorithmic_bytes_per_launch": algo_bytes, "ib_kernel_avg_us": round(ib_us, 3),
-                "pattern_probe_us": round(probe_us, 3), "frac_of_pattern_probe": round(probe_us / k_us, 4),
-                "pattern_probe_plain_order_us": round(probe_plain_us, 3)}
+                "pattern_probe_us": round(probe_us, 3), "frac_of_pattern_probe": round(probe_us / k_us, 4)}
 
     legs = {}
     names = [x for x in args.legs.split(",") if x and x != "none"]
-    if "nn" in names:
-        legs["nn_4096x2048"] = leg_nn(ippi, args.nn_steps, W, min(args.profile_steps, 200))
+    if "accuracy" in names:
+        legs["accuracy_nn_4096x2048"] = leg_accuracy_nn(ippi, args.acc_steps)
+    if "mrtles" in names:
+        legs["mrt_les_ib_2048x1024"] = leg_mrt_les_ib(ippi, args.mrtles_steps, W, args.profile_steps)
     if "adjoint" in names:
         legs["adjoint_1024x512"] = leg_adjoint(ippi)
     if "multibody" in names:
         legs["multibody_8192x4096"] = leg_multibody(ippi, args.mb_steps, W)
-    if "accuracy" in names:
-        legs["accuracy_2048x1024"] = leg_accuracy(ippi)
     if "flatplate" in names:
         legs["flatplate_2401x501"] = leg_flatplate(ippi)
     if "tausearch" in names:
         legs["tausearch"] = leg_tausearch(ippi)
+    if "accuracy_nn_4096x2048" in legs:
+        gate_err, gate_steps = legs["accuracy_nn_4096x2048"]["rel_l2_u_vs_fp64"], args.acc_steps
+    else:
+        gate_err, gate_steps = extra.get("rel_l2_u_vs_fp64"), K + W
+    extra["accuracy_gate"] = {"rel_l2_u_vs_fp64": gate_err, "steps": gate_steps, "bar": 1e-5,
+                              "met": None if gate_err is None else bool(gate_err <= 1e-5)}
 
     cpu = None
     if not args.no_cpu:
@@ -591,10 +585,11 @@ def main():
         "metric": METRIC, "value": round(mlups32, 1), "unit": "MLUPS", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(dt32 / K * 1e3, 5), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "MRT+LES(Cs=0.1)+IB cylinder D=80 (251 markers), 2048x1024, inflow 0.08 +-1e-3 "
-                               "noise, Re=1e4, FP32 shifted populations (f-w)",
-                   "nx": NX, "ny": NY, "collision": "mrt", "les_cs": CS, "tau0": TAU0, "u_in": U0,
-                   "bodies": 1, "markers": NMARK, "bc": "inflow"},
+        "config": {"workload": "FP32 fused MRT+LES(Cs=0.1)+IB step with network tau at every node (6-32-32-1 "
+                               "SiLU, bf16 MFMA, seeded Xavier weights, guard >= 0.501), cylinder D=160 (503 "
+                               "markers), 4096x2048, inflow 0.08 +-1e-3 noise, Re=1e4, populations stored as f-w",
+                   "nx": NX, "ny": NY, "collision": "mrt", "les_cs": CS, "tau0": TAU0, "tau_mode": "nn",
+                   "u_in": U0, "bodies": 1, "markers": NMARK, "bc": "inflow"},
         **extra,
         "roofline": roofline, "cpu_baseline": cpu, "legs": legs,
     }

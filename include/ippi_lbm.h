@@ -263,20 +263,11 @@ int ippi_get_stencil(ippi_handle* h, int n, const int* xy, double* fnew, double*
  * its own buffers and stream, times `reps` launches of a kernel with the
  * fused step's memory pattern and no arithmetic on an nx x ny lattice:
  * mode 1 = the nine pulled loads of an inflow step and nine stores, mode 2
- * = mode 1 with the rows grouped by XCD, mode 0 = a
- * nine-array copy; precision IPPI_FP64 or IPPI_FP32_SHIFTED. Writes the
+ * = mode 1 with the rows grouped by XCD, mode 0 = a nine-array copy, mode
+ * 3 = a flat copy of the same 9*nx*ny elements with 16 bytes per lane (the
+ * plain HBM stream); precision IPPI_FP64 or IPPI_FP32_SHIFTED. Writes the
  * mean time per launch (microseconds). */
 int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch);
-
-/* Issue-rate probe (no counterpart in the reference: measurement only). On
- * its own buffer and stream, times `reps` launches of 4096 x 256 threads,
- * each running 256 iterations of one instruction mix on registers: kind 0:
- * 64 FP32 FMAs; 1: 64 exp2; 2: 64 (add, rcp); 3: 16 SiLU activations
- * (exp2, add, rcp, mul); 4: 16 bf16 MFMAs 16x16x32; 5: kinds 4 and 0
- * together; 6: 16 FP32 MFMAs 16x16x4; 7: kinds 6 and 0 together. The
- * times give the cost of each kind against an FMA, and whether MFMA and
- * VALU work overlap (kind 5 against kinds 4 + 0). */
-int ippi_issue_probe(int device, int kind, int reps, double* us_per_launch);
 
 /* Checkpoint: the full device state as a flat byte image. *size gives the
  * required size when buf is NULL. */

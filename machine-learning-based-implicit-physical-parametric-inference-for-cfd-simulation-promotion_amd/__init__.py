@@ -88,7 +88,6 @@ _SIGS = {
     "ippi_adjoint_drag": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D],
                           ctypes.c_int),
     "ippi_get_adjoint_timing": ([ctypes.c_void_p, _D, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
-    "ippi_issue_probe": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, _D], ctypes.c_int),
     "ippi_pattern_probe": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D],
                            ctypes.c_int),
     "ippi_get_stencil": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _D, _D, _D], ctypes.c_int),
@@ -395,25 +394,14 @@ def pattern_probe(nx, ny, precision="fp32", mode="pull", reps=200, device=0):
     """Microseconds per launch of a kernel with the fused step's memory
     pattern and no arithmetic (mode "pull": the nine shifted loads of an
     inflow step; "pull_xcd": the same with the rows grouped by XCD;
-    "copy": a nine-array copy) on an nx x ny lattice."""
+    "copy": a nine-array copy; "copy16": a flat copy of the same elements
+    with 16 bytes per lane) on an nx x ny lattice."""
     us = ctypes.c_double()
     rc = library().ippi_pattern_probe(int(device), int(nx), int(ny), PRECISION[precision],
-                                      {"copy": 0, "pull": 1, "pull_xcd": 2}[mode], int(reps), ctypes.byref(us))
+                                      {"copy": 0, "pull": 1, "pull_xcd": 2, "copy16": 3}[mode], int(reps),
+                                      ctypes.byref(us))
     if rc != 0:
         raise IppiError(rc, "ippi_pattern_probe failed")
-    return us.value
-
-
-ISSUE_KINDS = ("fma", "exp2", "add_rcp", "silu", "mfma_bf16", "mfma_bf16+fma", "mfma_f32", "mfma_f32+fma")
-
-
-def issue_probe(kind, reps=10, device=0):
-    """Microseconds per launch of the issue-rate probe for one instruction
-    mix (a name of ISSUE_KINDS): 4096 x 256 threads, 256 iterations each."""
-    us = ctypes.c_double()
-    rc = library().ippi_issue_probe(int(device), ISSUE_KINDS.index(kind), int(reps), ctypes.byref(us))
-    if rc != 0:
-        raise IppiError(rc, "ippi_issue_probe failed")
     return us.value
 
 

@@ -13,7 +13,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = [os.path.join(HERE, "csrc", f) for f in ("ippi_lbm.hip", "ippi_train.hip", "ippi_tausearch.hip", "ippi_probe.hip")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("ippi_lbm.hip", "ippi_train.hip", "ippi_tausearch.hip")]
 OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libippi_lbm.so")
 

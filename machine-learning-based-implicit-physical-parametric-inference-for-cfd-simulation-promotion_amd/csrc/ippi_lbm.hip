@@ -10,10 +10,12 @@
 // i.e. the streaming of step t and the collision of step t+1, reading and
 // writing 9 populations per node once (A/B buffers, SoA f[k][y][x]).
 // The immersed boundary (mesh.cpp:514-832, 1244-1281, wired before the
-// collision) runs as one small kernel before each fused step: velocity at
-// the marker supports, marker interpolation + direct forcing, then a
-// gather-style spread into a dense per-body patch that the fused kernel
-// reads.
+// collision) runs inside the fused step kernel where the bodies allow it
+// (k_step_f32_ib / k_step_f64_ib: one block per body computes the marker
+// velocities and forces while the lattice rows stream, and the rows of the
+// bodies' patches wait for them), otherwise as one small kernel before each
+// fused step (k_ib). Either way the step kernel gathers each patch node's
+// force from the marker forces (the spread, mesh.cpp:659-741).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -899,6 +901,20 @@ __global__ void __launch_bounds__(BX) k_probe(const T* __restrict__ src, T* __re
     if (active) {
 #pragma unroll
         for (int k = 0; k < 9; k++) st<NT>(dst + k * N, n, f[k]);
+    }
+}
+
+// Bandwidth ceiling: a flat copy of one array with 16 bytes per lane (grid
+// stride), the simplest HBM stream the chip runs (pattern probe mode 3).
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+template <bool NT>
+__global__ void __launch_bounds__(256) k_copy16(const f32x4v* __restrict__ s, f32x4v* __restrict__ d, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) {
+        const f32x4v v = s[i];
+        if (NT)
+            __builtin_nontemporal_store(v, d + i);
+        else
+            d[i] = v;
     }
 }
 
@@ -3332,7 +3348,7 @@ int ippi_get_timing(ippi_handle* h, double* step_ms, int64_t* step_launches, dou
 }
 
 int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int reps, double* us_per_launch) {
-    if (nx < 4 || ny < 4 || precision < 0 || precision > 1 || mode < 0 || mode > 2 ||
+    if (nx < 4 || ny < 4 || precision < 0 || precision > 1 || mode < 0 || mode > 3 ||
         reps < 1 || !us_per_launch)
         return IPPI_ERR_ARG;
     if (9 * (uint64_t)nx * (uint64_t)ny >= ((uint64_t)1 << 32)) return IPPI_ERR_ARG;
@@ -3365,7 +3381,14 @@ int ippi_pattern_probe(int device, int nx, int ny, int precision, int mode, int 
     else if (mode == 1) IPPI_PROBE(TT, 1, false);      \
     else if (nt) IPPI_PROBE(TT, 0, true);              \
     else IPPI_PROBE(TT, 0, false);
-            if (esz == 8) {
+            if (mode == 3) {  // 16 bytes per lane, one flat array of 9 N elements
+                const size_t n4 = 9 * N * esz / 16;
+                const dim3 gc((unsigned)std::min<size_t>((n4 + 255) / 256, 65536));
+                if (nt)
+                    hipLaunchKernelGGL(k_copy16<true>, gc, dim3(256), 0, st, (const f32x4v*)s0, (f32x4v*)d0, n4);
+                else
+                    hipLaunchKernelGGL(k_copy16<false>, gc, dim3(256), 0, st, (const f32x4v*)s0, (f32x4v*)d0, n4);
+            } else if (esz == 8) {
                 IPPI_PROBE_T(double)
             } else {
                 IPPI_PROBE_T(float)
