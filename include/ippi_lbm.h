@@ -49,6 +49,7 @@ extern "C" {
 #define IPPI_ERR_DIVERGED 3  /* a node's density left [0, 10] or became NaN (ErrorCatch) */
 #define IPPI_ERR_STATE 4     /* call not valid in the handle's current state */
 #define IPPI_ERR_NOMEM 5     /* device allocation failed */
+#define IPPI_ERR_CAPACITY 6  /* input larger than a fixed capacity of the call */
 
 enum ippi_collision { IPPI_COLL_BGK = 0, IPPI_COLL_MRT = 1, IPPI_COLL_RR = 2 };
 enum ippi_precision { IPPI_FP64 = 0, IPPI_FP32_SHIFTED = 1 };
@@ -246,6 +247,21 @@ int ippi_get_adjoint_timing(ippi_handle* h, double* ms, int64_t* launches);
  * step = 1e-5. */
 int ippi_tau_search(int device, int n, const double* fnew, const double* feq, const double* tau,
                     const double* target, int m_lo, int m_hi, double step, double* best_dtau, double* best_err);
+
+/* ippi_tau_search without device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving
+ * loops, code between captured graph launches, harnesses that forbid
+ * hipMalloc). Same arguments and results as ippi_tau_search, bit for bit;
+ * the device copies of the inputs and outputs live in one fixed array per
+ * GPU (256 bytes per node, 8 MiB, held until the process ends), so n is at
+ * most IPPI_TAU_SEARCH_NOALLOC_MAX (IPPI_ERR_CAPACITY beyond). Calls that
+ * overlap on one GPU (from several host threads) share that array and
+ * race; serialise them. Runs on the device's null stream and returns when
+ * the results are in best_dtau / best_err. */
+#define IPPI_TAU_SEARCH_NOALLOC_MAX 32768
+int ippi_tau_search_noalloc(int device, int n, const double* fnew, const double* feq, const double* tau,
+                            const double* target, int m_lo, int m_hi, double step, double* best_dtau,
+                            double* best_err);
 
 /* Inputs of the label search for n nodes (xy[i] = (x, y), two or more
  * nodes inside the domain), as Mesh::FindAndWriteTau gathers them

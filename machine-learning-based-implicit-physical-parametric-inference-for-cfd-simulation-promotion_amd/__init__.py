@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libippi_lbm.so")
 
-IPPI_OK, IPPI_ERR_ARG, IPPI_ERR_HIP, IPPI_ERR_DIVERGED, IPPI_ERR_STATE, IPPI_ERR_NOMEM = range(6)
+IPPI_OK, IPPI_ERR_ARG, IPPI_ERR_HIP, IPPI_ERR_DIVERGED, IPPI_ERR_STATE, IPPI_ERR_NOMEM, IPPI_ERR_CAPACITY = range(7)
 COLLISION = {"bgk": 0, "mrt": 1, "rr": 2}
 PRECISION = {"fp64": 0, "fp32": 1}
 BC = {"periodic_bb": 0, "inflow": 1, "flatplate": 2}
@@ -93,6 +93,8 @@ _SIGS = {
     "ippi_get_stencil": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _D, _D, _D], ctypes.c_int),
     "ippi_tau_search": ([ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                          _D, _D], ctypes.c_int),
+    "ippi_tau_search_noalloc": ([ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, _D, _D], ctypes.c_int),
     "ippi_accumulate_mean": ([ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "ippi_get_mean": ([ctypes.c_void_p, _D, _D, _D], ctypes.c_int),
     "ippi_save": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
@@ -405,17 +407,22 @@ def pattern_probe(nx, ny, precision="fp32", mode="pull", reps=200, device=0):
     return us.value
 
 
-def tau_search(fnew, feq, tau, target, m_lo=-10000, m_hi=10000, step=1e-5, device=0):
+TAU_SEARCH_NOALLOC_MAX = 32768
+
+
+def tau_search(fnew, feq, tau, target, m_lo=-10000, m_hi=10000, step=1e-5, device=0, noalloc=False):
     """The candidate loop of FindAndWriteTau (mesh.cpp:2667-2704) for n nodes
     on the GPU: fnew, feq, tau [n, 9] of the node each population streams in
-    from, target [n, 3] = (rho, u, v). Returns (best dtau [n], score [n])."""
+    from, target [n, 3] = (rho, u, v). Returns (best dtau [n], score [n]).
+    noalloc: ippi_tau_search_noalloc (fixed device staging, n at most
+    TAU_SEARCH_NOALLOC_MAX)."""
     fnew, feq, tau, target = (np.ascontiguousarray(a, dtype=np.float64) for a in (fnew, feq, tau, target))
     n = fnew.shape[0]
     if fnew.shape != (n, 9) or feq.shape != (n, 9) or tau.shape != (n, 9) or target.shape != (n, 3):
         raise ValueError("fnew, feq, tau must be [n, 9] and target [n, 3]")
     bd, be = np.zeros(n), np.zeros(n)
-    rc = library().ippi_tau_search(int(device), n, _p(fnew), _p(feq), _p(tau), _p(target), int(m_lo),
-                                   int(m_hi), float(step), _p(bd), _p(be))
+    fn = library().ippi_tau_search_noalloc if noalloc else library().ippi_tau_search
+    rc = fn(int(device), n, _p(fnew), _p(feq), _p(tau), _p(target), int(m_lo), int(m_hi), float(step), _p(bd), _p(be))
     if rc != 0:
-        raise IppiError(rc, "ippi_tau_search failed")
+        raise IppiError(rc, "ippi_tau_search_noalloc failed" if noalloc else "ippi_tau_search failed")
     return bd, be

@@ -34,13 +34,24 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <mutex>
 #include <string>
 
 #include "../../include/ippi_lbm.h"
 
+// Staging of ippi_tau_search_noalloc: one fixed device array per GPU that
+// the library holds until the process ends, carved per call into the
+// inputs fnew, feq, tau [n][9], target [n][3] and the outputs best_dtau,
+// best_err [n] (256 bytes per node). Host code reaches it only through
+// the address hipGetSymbolAddress returns for the current device.
+constexpr size_t TS_BYTES_PER_NODE = 8 * (9 + 9 + 9 + 3 + 1 + 1);
+constexpr size_t TS_SCRATCH_BYTES = TS_BYTES_PER_NODE * IPPI_TAU_SEARCH_NOALLOC_MAX;
+static __device__ __align__(256) unsigned char scratch[TS_SCRATCH_BYTES];
+
 namespace {
 
 constexpr int TS_NODES = 4;  // nodes (waves) per block
+
 constexpr double TS_TINY = 0x1p-200, TS_HUGE = 0x1p200;
 __constant__ double TS_EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 __constant__ double TS_EY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -161,5 +172,67 @@ int ippi_tau_search(int device, int n, const double* fnew, const double* feq, co
     }
     for (double* p : {d_fn, d_fq, d_tk, d_tg, d_bd, d_be})
         if (p) (void)hipFree(p);
+    return rc;
+}
+
+namespace {
+
+constexpr int TS_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[TS_MAX_DEVICES] = {};
+
+// Device address of this GPU's staging array (device must be current),
+// looked up once per device.
+int scratch_ptr(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return IPPI_ERR_HIP;
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[device]);
+    return IPPI_OK;
+}
+
+}  // namespace
+
+int ippi_tau_search_noalloc(int device, int n, const double* fnew, const double* feq, const double* tau,
+                            const double* target, int m_lo, int m_hi, double step, double* best_dtau,
+                            double* best_err) {
+    if (n < 0 || (n > 0 && (!fnew || !feq || !tau || !target || !best_dtau || !best_err)) || m_hi < m_lo ||
+        (long long)m_hi - (long long)m_lo > (1LL << 30) || m_hi > INT_MAX - 64)
+        return IPPI_ERR_ARG;
+    if (n > IPPI_TAU_SEARCH_NOALLOC_MAX) return IPPI_ERR_CAPACITY;
+    if (n == 0) return IPPI_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) return IPPI_ERR_HIP;
+    if (device < 0 || device >= ndev || device >= TS_MAX_DEVICES) return IPPI_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return IPPI_ERR_HIP;
+    unsigned char* base = nullptr;
+    int rc = scratch_ptr(device, &base);
+    if (rc) return rc;
+    const size_t s9 = sizeof(double) * 9 * (size_t)n, s3 = sizeof(double) * 3 * (size_t)n, s1 = sizeof(double) * n;
+    double* d_fn = reinterpret_cast<double*>(base);
+    double* d_fq = d_fn + 9 * (size_t)n;
+    double* d_tk = d_fq + 9 * (size_t)n;
+    double* d_tg = d_tk + 9 * (size_t)n;
+    double* d_bd = d_tg + 3 * (size_t)n;
+    double* d_be = d_bd + (size_t)n;
+    hipStream_t st = nullptr;  // the device's null stream: ordered after earlier calls of this function
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && rc == IPPI_OK) rc = IPPI_ERR_HIP;
+        return e == hipSuccess;
+    };
+    if (ok(hipMemcpyAsync(d_fn, fnew, s9, hipMemcpyHostToDevice, st)) &&
+        ok(hipMemcpyAsync(d_fq, feq, s9, hipMemcpyHostToDevice, st)) &&
+        ok(hipMemcpyAsync(d_tk, tau, s9, hipMemcpyHostToDevice, st)) &&
+        ok(hipMemcpyAsync(d_tg, target, s3, hipMemcpyHostToDevice, st))) {
+        const dim3 g((unsigned)((n + TS_NODES - 1) / TS_NODES));
+        hipLaunchKernelGGL(k_tau_search, g, dim3(64 * TS_NODES), 0, st, n, d_fn, d_fq, d_tk, d_tg, m_lo, m_hi, step,
+                           d_bd, d_be);
+        if (ok(hipGetLastError()) && ok(hipMemcpyAsync(best_dtau, d_bd, s1, hipMemcpyDeviceToHost, st)) &&
+            ok(hipMemcpyAsync(best_err, d_be, s1, hipMemcpyDeviceToHost, st)))
+            ok(hipStreamSynchronize(st));
+    }
     return rc;
 }
