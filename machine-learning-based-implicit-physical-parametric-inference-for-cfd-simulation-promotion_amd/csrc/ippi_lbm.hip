@@ -34,6 +34,13 @@
 using namespace lbm;
 
 #define IPPI_MAXB 32
+// waves per SIMD the network variants of the step kernels are compiled for
+#ifndef IPPI_NN_WAVES
+#define IPPI_NN_WAVES 4
+#endif
+#ifndef IPPI_INK_MAXACT_NN
+#define IPPI_INK_MAXACT_NN 3072
+#endif
 
 namespace {
 
@@ -664,7 +671,7 @@ __global__ void __launch_bounds__(BX) k_step_f64(const double* __restrict__ src,
 template <int COLL, int BC, bool PULL, int NN, bool NT, bool COH = false>
 __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, const float* __restrict__ src,
                                          const double* __restrict__ mac, float* __restrict__ dst, const StepArgs& a,
-                                         bool active, __bf16* lds, int bx0) {
+                                         bool active, __bf16* lds, int bx0, const u32x4* lf = nullptr) {
 #pragma clang fp contract(fast)
     const int nx = a.nx;
     const unsigned n = (unsigned)y * nx + x;
@@ -735,7 +742,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         feat[3] = 1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy));
         feat[4] = nxx - nyy;
         feat[5] = nxy;
-        const float t = mlp_tau_wave<NN == 1>(feat, a.mlp, lds);  // tau - 1/2
+        const float t = mlp_tau_nn<NN>(feat, a.mlp, lds, lf);  // tau - 1/2
         nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
         if (nn_ok) tx = t;
     }
@@ -811,7 +818,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
 // population loads (caps them at 128 VGPRs). The others are latency-bound
 // streams and keep the full 8 waves (64 VGPRs).
 template <int COLL, int BC, bool PULL, int NPT, int NN, bool NT>
-__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? 4 : 8)))
+__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? IPPI_NN_WAVES : 8)))
 k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
                                                  const double* __restrict__ mac, StepArgs a) {
     const int nx = a.nx, ny = a.ny;
@@ -823,11 +830,14 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
     const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const int y0 = byi * NPT;
     __shared__ __bf16 lds_mlp[NN ? BX * 16 : 1];
+    __shared__ u32x4 lds_frag[NN == 3 ? MLP_FRAGE_CHUNKS * 64 : 1];
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
     const unsigned N = (unsigned)nx * ny;
     constexpr bool RAW = !(PULL && BC == IPPI_BC_FLATPLATE);  // the flat-plate rules load for themselves
+    if (NN == 3) mlp_frage_to_lds<BX>(a.mlp, lds_frag);  // in flight with the population loads
     float cur[9];
     if (RAW) load_raw<float, BC, PULL>(src, x, y0, nx, ny, N, cur);
+    if (NN == 3) __syncthreads();
 #pragma unroll
     for (int j = 0; j < NPT; j++) {
         const int y = y0 + j;
@@ -835,7 +845,7 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
         const bool more = j + 1 < NPT && y + 1 < ny;
         float nxt[9];
         if (RAW && j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
-        node_f32<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX);
+        node_f32<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX, lds_frag);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; k++) cur[k] = nxt[k];
@@ -930,6 +940,11 @@ __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, dou
     const int y = blockIdx.y;
     const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
     __shared__ __bf16 lds_mlp[NN ? BX * 16 : 1];
+    __shared__ u32x4 lds_frag[NN == 3 ? MLP_FRAGE_CHUNKS * 64 : 1];
+    if (NN == 3) {
+        mlp_frage_to_lds<BX>(a.mlp, lds_frag);
+        __syncthreads();
+    }
     float g[9];
     Mo o;
     if constexpr (BC == IPPI_BC_FLATPLATE) {
@@ -958,7 +973,7 @@ __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, dou
     if (NN) {
         float feat[6] = {o.rho, o.ux, o.uy,
                          1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy)), nxx - nyy, nxy};
-        const float t = mlp_tau_wave<NN == 1>(feat, a.mlp, lds_mlp + (threadIdx.x & ~63) * 16);
+        const float t = mlp_tau_nn<NN>(feat, a.mlp, lds_mlp + (threadIdx.x & ~63) * 16, lds_frag);
         nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
         if (nn_ok) tx = t;
     }
@@ -1535,7 +1550,7 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
 // record's tree are those of k_ib, so the results are bit for bit those of
 // the separate kernel.
 constexpr int INK_MAXACT = 2048;     // active nodes per body, lattice-only kernel (16 KB of float velocities)
-constexpr int INK_MAXACT_NN = 3072;  // with the network (fewer resident blocks, more LDS each)
+constexpr int INK_MAXACT_NN = IPPI_INK_MAXACT_NN;  // with the network (fewer resident blocks, more LDS each)
 constexpr int INK_MAXACT64 = 1280;   // FP64 (20 KB of double velocities)
 // wait limit in polls of the flag; each poll is a load from memory (~0.5-1
 // us) and a short sleep, so the limit is some 30-70 ms: hundreds of times
@@ -1683,9 +1698,12 @@ __device__ __forceinline__ void wait_ib_forces(const StepArgs& a) {
 }
 
 template <int COLL, int BC, int NN, bool NT>
-__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? 4 : 8)))
+__global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? IPPI_NN_WAVES : 8)))
 k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const double* __restrict__ mac, StepArgs a) {
-    __shared__ float U[2 * (NN ? INK_MAXACT_NN : INK_MAXACT)];
+    // the IB block's velocities; the lattice blocks of the exact-weight
+    // network keep the network's fragments in the same LDS
+    __shared__ __attribute__((aligned(16))) float U[2 * (NN ? INK_MAXACT_NN : INK_MAXACT)];
+    static_assert(NN != 3 || sizeof(U) >= 16 * MLP_FRAGE_CHUNKS * 64, "LDS for the network's fragments");
     __shared__ __attribute__((aligned(16))) __bf16 lds_mlp[NN ? BX * 16 : 1];
     const int nx = a.nx, ny = a.ny;
     if (blockIdx.y == 0) {
@@ -1700,10 +1718,13 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     const bool active = (int)(bxi * BX + threadIdx.x) < nx;
     const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny;
+    u32x4* lds_frag = reinterpret_cast<u32x4*>(U);
+    if (NN == 3) mlp_frage_to_lds<BX>(a.mlp, lds_frag);  // in flight with the population loads
     float cur[9];
     load_raw<float, BC, true>(src, x, y, nx, ny, N, cur);
+    if (NN == 3) __syncthreads();
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
-    node_f32<COLL, BC, true, NN, NT, true>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX);
+    node_f32<COLL, BC, true, NN, NT, true>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX, lds_frag);
 }
 
 // FP64 variant: four rows per thread as k_step_f64, the rotation and the
@@ -2150,8 +2171,10 @@ struct ippi_handle {
     // relaxation-time network
     bool has_mlp = false;
     MlpFrag* d_frag = nullptr;
+    u32x4* d_frage = nullptr;  // exact-weight network fragments, [MLP_FRAGE_CHUNKS][64]
     double* d_blob = nullptr;
-    float mlp_mu[6] = {0}, mlp_isig[6] = {0}, mlp_b3 = 0.f;
+    float mlp_mu[6] = {0}, mlp_isig[6] = {0}, mlp_isig_e[6] = {0}, mlp_b3 = 0.f;
+    bool mlp_exact = false;  // W1 and W2 hold bf16 values only (the FP32 path's exact-weight network)
     // optional per-launch timing (HIP events on this handle's stream)
     int timing = 0;                      // 1: events around each launch; 2: around each ippi_step call
     int64_t batch_steps = 0, batch_ib = 0;  // kernels launched inside the timed calls (mode 2)
@@ -2181,9 +2204,10 @@ void dfree(P*& p) {
 }
 
 // Network mode of the step kernels: 0 none, 1 compensated bf16 products,
-// 2 plain bf16 products (FP32 only; FP64 evaluates the network in FP64).
+// 2 plain bf16 products, 3 bf16-exact weights (2 and 3 FP32 only; FP64
+// evaluates the network in FP64).
 int nn_kind(const ippi_handle* h) {
-    if (h->c.tau_mode == IPPI_TAU_NN) return 1;
+    if (h->c.tau_mode == IPPI_TAU_NN) return h->esz == 4 && h->mlp_exact ? 3 : 1;
     if (h->c.tau_mode == IPPI_TAU_NN_BF16) return h->esz == 4 ? 2 : 1;
     return 0;
 }
@@ -2231,6 +2255,7 @@ StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
 
 template <int COLL, int BC>
 StepKernel<float> pick_step_ib_bc(int nn, bool nt) {
+    if (nn == 3) return nt ? k_step_f32_ib<COLL, BC, 3, true> : k_step_f32_ib<COLL, BC, 3, false>;
     if (nn == 2) return nt ? k_step_f32_ib<COLL, BC, 2, true> : k_step_f32_ib<COLL, BC, 2, false>;
     if (nn == 1) return nt ? k_step_f32_ib<COLL, BC, 1, true> : k_step_f32_ib<COLL, BC, 1, false>;
     return nt ? k_step_f32_ib<COLL, BC, 0, true> : k_step_f32_ib<COLL, BC, 0, false>;
@@ -2292,10 +2317,12 @@ StepArgs make_args(ippi_handle* h) {
     a.nn_max = h->c.tau_nn_max > 0 ? h->c.tau_nn_max : 1e30;
     a.nn_maxtxf = (float)(a.nn_max - 0.5);
     a.mlp.frag = h->d_frag;
+    a.mlp.frage = h->d_frage;
     a.mlp.blob = h->d_blob;
     for (int i = 0; i < 6; i++) {
         a.mlp.mu[i] = h->mlp_mu[i];
         a.mlp.isig[i] = h->mlp_isig[i];
+        a.mlp.isig_e[i] = h->mlp_isig_e[i];
     }
     a.mlp.b3 = h->mlp_b3;
     a.plate_x = h->c.plate_x;
@@ -2652,10 +2679,12 @@ int run_steps(ippi_handle* h, int nsteps) {
             if (h->timing == 1) h->ev_ib.push_back(new_event(h));
             h->batch_ib++;
         }
-        StepKernel<T> k = pull ? (nk == 2   ? pick_step<T, true, 2>(h->c.collision, h->c.bc, a.nt)
+        StepKernel<T> k = pull ? (nk == 3   ? pick_step<T, true, 3>(h->c.collision, h->c.bc, a.nt)
+                                  : nk == 2 ? pick_step<T, true, 2>(h->c.collision, h->c.bc, a.nt)
                                   : nk == 1 ? pick_step<T, true, 1>(h->c.collision, h->c.bc, a.nt)
                                             : pick_step<T, true, 0>(h->c.collision, h->c.bc, a.nt))
-                               : (nk == 2   ? pick_step<T, false, 2>(h->c.collision, h->c.bc)
+                               : (nk == 3   ? pick_step<T, false, 3>(h->c.collision, h->c.bc)
+                                  : nk == 2 ? pick_step<T, false, 2>(h->c.collision, h->c.bc)
                                   : nk == 1 ? pick_step<T, false, 1>(h->c.collision, h->c.bc)
                                             : pick_step<T, false, 0>(h->c.collision, h->c.bc));
         if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
@@ -3026,6 +3055,7 @@ int ippi_destroy(ippi_handle* h) {
     dfree(h->buf[0]); dfree(h->buf[1]); dfree(h->mac); dfree(h->corner); dfree(h->mean); dfree(h->tau); dfree(h->flag); dfree(h->part);
     free_ib(h);
     dfree(h->d_frag);
+    dfree(h->d_frage);
     dfree(h->d_blob);
     if (h->s) (void)hipStreamDestroy(h->s);
     delete h;
@@ -3271,6 +3301,9 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
     // constants folded as described at MlpFrag
     std::vector<MlpFrag> fr(64);
     const double L = 1.4426950408889634, LN2 = 0.6931471805599453;
+    bool exact = true;  // every W1 and W2 entry a bf16 value
+    for (int i = OFF_W1; i < OFF_B1; i++) exact = exact && (double)(float)(__bf16)(float)blob[i] == blob[i];
+    for (int i = OFF_W2; i < OFF_B2; i++) exact = exact && (double)(float)(__bf16)(float)blob[i] == blob[i];
     auto hi = [](double v) { return (__bf16)(float)v; };
     auto lo = [](double v) { return (__bf16)((float)v - (float)(__bf16)(float)v); };
     for (int l = 0; l < 64; l++) {
@@ -3283,6 +3316,10 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
                 const double w = k < MLP_NIN ? -L * blob[OFF_W1 + o * MLP_NIN + k] : 0.0;
                 F.a1h[mb][j] = hi(w);
                 F.a1l[mb][j] = lo(w);
+            }
+            for (int j = 0; j < 4; j++) {  // [W1, 0, 0, W1, 0, 0] for the exact-weight network
+                const int k = (4 * g + j) % 8;
+                F.a1e[mb][j] = (__bf16)(float)(k < MLP_NIN ? blob[OFF_W1 + o * MLP_NIN + k] : 0.0);
             }
             for (int j = 0; j < 8; j++) {
                 const int k = (j / 4) * 16 + 4 * g + (j % 4);  // order of the layer-1 outputs in the lanes
@@ -3297,16 +3334,36 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
             }
         }
     }
+    // chunk-major copy of the exact-weight network's per-lane fragments
+    std::vector<u32x4> fe(MLP_FRAGE_CHUNKS * 64);
+    for (int l = 0; l < 64; l++) {
+        struct {
+            bf16x4 a1e[2];
+            bf16x8 a2h[2];
+            float b1[8], b2[8], w3[8];
+        } e;
+        static_assert(sizeof(e) == 16 * MLP_FRAGE_CHUNKS, "fragment layout");
+        memcpy(e.a1e, fr[l].a1e, sizeof(e.a1e));
+        memcpy(e.a2h, fr[l].a2h, sizeof(e.a2h));
+        memcpy(e.b1, fr[l].b1, sizeof(e.b1));
+        memcpy(e.b2, fr[l].b2, sizeof(e.b2));
+        memcpy(e.w3, fr[l].w3, sizeof(e.w3));
+        for (int c = 0; c < MLP_FRAGE_CHUNKS; c++) memcpy(&fe[c * 64 + l], reinterpret_cast<const char*>(&e) + 16 * c, 16);
+    }
     if (!h->d_frag) HIPC(h, hipMalloc(&h->d_frag, sizeof(MlpFrag) * 64));
+    if (!h->d_frage) HIPC(h, hipMalloc(&h->d_frage, sizeof(u32x4) * fe.size()));
     if (!h->d_blob) HIPC(h, hipMalloc(&h->d_blob, sizeof(double) * MLP_BLOB));
     HIPC(h, hipStreamSynchronize(h->s));
     HIPC(h, hipMemcpy(h->d_frag, fr.data(), sizeof(MlpFrag) * 64, hipMemcpyHostToDevice));
+    HIPC(h, hipMemcpy(h->d_frage, fe.data(), sizeof(u32x4) * fe.size(), hipMemcpyHostToDevice));
     HIPC(h, hipMemcpy(h->d_blob, blob, sizeof(double) * MLP_BLOB, hipMemcpyHostToDevice));
     for (int i = 0; i < 6; i++) {
         h->mlp_mu[i] = (float)blob[OFF_MU + i];
         h->mlp_isig[i] = (float)blob[OFF_IS + i];
+        h->mlp_isig_e[i] = (float)(-L * blob[OFF_IS + i]);
     }
     h->mlp_b3 = (float)blob[OFF_B3];
+    h->mlp_exact = exact;
     h->has_mlp = true;
     return IPPI_OK;
 }
@@ -3545,7 +3602,9 @@ int ippi_get_tau(ippi_handle* h, double* tau) {
     }
 #define IPPI_TAU32_LAUNCH(B)                                                                               \
     if (bc == B) {                                                                                         \
-        if (nk == 2)                                                                                       \
+        if (nk == 3)                                                                                       \
+            hipLaunchKernelGGL((k_tau32<B, 3>), grid, dim3(BX), 0, h->s, (const float*)h->buf[h->cur], d_out, a); \
+        else if (nk == 2)                                                                                  \
             hipLaunchKernelGGL((k_tau32<B, 2>), grid, dim3(BX), 0, h->s, (const float*)h->buf[h->cur], d_out, a); \
         else if (nk == 1)                                                                                  \
             hipLaunchKernelGGL((k_tau32<B, 1>), grid, dim3(BX), 0, h->s, (const float*)h->buf[h->cur], d_out, a); \

@@ -24,6 +24,15 @@
 // only), FP32 accumulation: a third of the MFMAs and no lo parts, about 8
 // mantissa bits per operand.
 //
+// Exact weights (mlp_tau_wave_exact): when every entry of W1 and W2 is a
+// bf16 value, the products need no lo part of the weights: a product is
+// W x_hi + W x_lo, exact in FP32 up to the 16-bit representation of x.
+// Layer 1 then takes one 16x16x16 MFMA per 16 outputs, its K holding
+// [x_hi(6), 0, 0, x_lo(6), 0, 0] against [W(6), 0, 0, W(6), 0, 0]; layer 2
+// two 16x16x32 MFMAs (h_hi, h_lo). 24 MFMAs per wave instead of 48. The
+// -log2(e) of the activations is folded into the feature scaling (isig)
+// and the biases, so W1 stays exact.
+//
 // FP64 path: the same network on VALU in FP64 with the oracle's loop order.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -53,12 +62,28 @@ struct MlpFrag {
     bf16x4 a1h[2], a1l[2];  // layer-1 A tiles (two 16-output blocks), hi / lo
     bf16x8 a2h[2], a2l[2];  // layer-2 A tiles, columns in lane order
     float b1[8], b2[8], w3[8];  // scaled bias / output weight of this lane's 8 outputs
+    bf16x4 a1e[2];          // exact weights: layer-1 A tiles [W, W] (unscaled W1)
 };
+// Exact-weight network: the per-lane fragment (144 bytes) is read from LDS,
+// where each workgroup copies it once (global layout chunk-major
+// [MLP_FRAGE_CHUNKS][64 lanes] of 16-byte chunks, so that a wave's reads of
+// one chunk are contiguous): chunk 0 = a1e[0..1], 1..2 = a2h[0..1] (exact
+// W2), 3..4 = b1, 5..6 = b2, 7..8 = w3, with the foldings of MlpFrag.
+constexpr int MLP_FRAGE_CHUNKS = 9;
 struct MlpDev {
     const MlpFrag* frag;  // [64 lanes]
     float mu[6], isig[6], b3;
+    float isig_e[6];      // exact weights: -log2(e) isig
+    const u32x4* frage;   // exact weights: [MLP_FRAGE_CHUNKS][64]
     const double* blob;   // FP64 path
 };
+
+// Copy of the exact-weight fragments into the workgroup's LDS (every thread
+// of the block takes part; the caller synchronises before the first use).
+template <int NT>
+__device__ __forceinline__ void mlp_frage_to_lds(const MlpDev& m, u32x4* lf) {
+    for (int i = threadIdx.x; i < MLP_FRAGE_CHUNKS * 64; i += NT) lf[i] = m.frage[i];
+}
 
 // 1 / (1 + 2^t) = sigma(z) for t = -log2(e) z
 __device__ __forceinline__ float sig2(float t) {
@@ -179,6 +204,82 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
     }
     const float o = (g == 0 ? part[0] : g == 1 ? part[1] : g == 2 ? part[2] : part[3]) + m.b3;
     return softplus_f(o);
+}
+
+// Network with bf16-exact W1 and W2 (see the header): same layouts and
+// folding as mlp_tau_wave, two MFMAs per product in layer 2 and one in
+// layer 1.
+__device__ __forceinline__ float mlp_tau_wave_exact(const float* feat, const MlpDev& m, __bf16* lds,
+                                                    const u32x4* lf) {
+#pragma clang fp contract(fast)
+    const int l = threadIdx.x & 63;
+    const int g = l >> 4;
+    struct {
+        bf16x4 a1e[2];
+        bf16x8 a2h[2];
+        float b1[8], b2[8], w3[8];
+    } fr;
+    static_assert(sizeof(fr) == 16 * MLP_FRAGE_CHUNKS, "fragment layout");
+    u32x4* fc = reinterpret_cast<u32x4*>(&fr);
+#pragma unroll
+    for (int c = 0; c < MLP_FRAGE_CHUNKS; c++) fc[c] = lf[c * 64 + l];
+    unsigned vh[3], vl[3];
+#pragma unroll
+    for (int k = 0; k < 6; k += 2) {
+        const float a = (feat[k] - m.mu[k]) * m.isig_e[k], b = (feat[k + 1] - m.mu[k + 1]) * m.isig_e[k + 1];
+        split2(a, b, vh[k / 2], vl[k / 2]);
+    }
+    *reinterpret_cast<u32x4*>(lds + 16 * l) = (u32x4){vh[0], vh[1], vh[2], 0u};
+    *reinterpret_cast<u32x4*>(lds + 16 * l + 8) = (u32x4){vl[0], vl[1], vl[2], 0u};
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    float part[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        const int node = nb * 16 + (l & 15);
+        // K rows 4g..4g+3 of [x_hi, 0, 0, x_lo, 0, 0]: the node's 16 bf16 in LDS as written above
+        const bf16x4 bx = *reinterpret_cast<const bf16x4*>(lds + 16 * node + 4 * g);
+        unsigned ph[4], pl[4];
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++) {
+            f32x4 c = {fr.b1[4 * mb], fr.b1[4 * mb + 1], fr.b1[4 * mb + 2], fr.b1[4 * mb + 3]};
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1e[mb], bx, c, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {
+                const float h0 = c[r] * sig2(c[r]), h1 = c[r + 1] * sig2(c[r + 1]);
+                split2(h0, h1, ph[2 * mb + r / 2], pl[2 * mb + r / 2]);
+            }
+        }
+        const bf16x8 hh = __builtin_bit_cast(bf16x8, (u32x4){ph[0], ph[1], ph[2], ph[3]});
+        const bf16x8 hl = __builtin_bit_cast(bf16x8, (u32x4){pl[0], pl[1], pl[2], pl[3]});
+        float acc = 0.0f;
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++) {
+            f32x4 c = {fr.b2[4 * mb], fr.b2[4 * mb + 1], fr.b2[4 * mb + 2], fr.b2[4 * mb + 3]};
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], hh, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fr.a2h[mb], hl, c, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc += (fr.w3[4 * mb + r] * c[r]) * sig2(c[r]);
+        }
+        part[nb] = acc;
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        part[nb] += __shfl_xor(part[nb], 16);
+        part[nb] += __shfl_xor(part[nb], 32);
+    }
+    const float o = (g == 0 ? part[0] : g == 1 ? part[1] : g == 2 ? part[2] : part[3]) + m.b3;
+    return softplus_f(o);
+}
+
+// Network mode of the FP32 kernels: 1 compensated bf16 products, 2 plain
+// bf16 products, 3 bf16-exact weights.
+template <int NN>
+__device__ __forceinline__ float mlp_tau_nn(const float* feat, const MlpDev& m, __bf16* lds, const u32x4* lf) {
+    if constexpr (NN == 3)
+        return mlp_tau_wave_exact(feat, m, lds, lf);
+    else
+        return mlp_tau_wave<NN == 1>(feat, m, lds);
 }
 
 // FP64 network on VALU, the oracle's loop order (weights read wave-uniformly).

@@ -1,7 +1,8 @@
 """Seeded inputs shared by the benchmark and the tests.
 
 - `xavier_blob`: relaxation-time network 6 -> 32 -> 32 -> 1 with
-  Xavier-uniform weights (numpy PCG64 seeded), in the parameter layout of
+  Xavier-uniform weights (numpy PCG64 seeded; W1 and W2 rounded to bf16
+  values by default), in the parameter layout of
   ippi_set_mlp (W1[32][6], b1[32], W2[32][32], b2[32], W3[32], b3, mu[6],
   1/sigma[6]);
 - `place_cylinders`: cylinders with diameters uniform in [dmin, dmax] at
@@ -22,11 +23,20 @@ MLP_PARAMS = 6 * 32 + 32 + 32 * 32 + 32 + 32 + 1
 MLP_BLOB = MLP_PARAMS + 12
 
 
+def round_bf16(x):
+    """Nearest bf16 values (ties to even) of x, as float64."""
+    b = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    b = ((b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return b.view(np.float32).astype(np.float64)
+
+
 def xavier_blob(seed, b3=-5.0, mu=(1.0, 0.05, 0.0, 2e-3, 0.0, 0.0), sig=(2e-3, 0.02, 0.02, 2e-3, 5e-4, 5e-4),
-                w3_scale=0.3):
+                w3_scale=0.3, bf16_weights=True):
     """Seeded Xavier-uniform network. With the default b3 the output
     softplus(.) + 0.5 sits a few 1e-3 above 0.5, so the >= 0.501 guard
-    accepts it at typical nodes."""
+    accepts it at typical nodes. bf16_weights: W1 and W2 rounded to bf16
+    values, which the FP32 path multiplies exactly (two MFMAs per product
+    instead of three)."""
     rng = np.random.default_rng(seed)
     p = np.zeros(MLP_BLOB)
 
@@ -35,6 +45,9 @@ def xavier_blob(seed, b3=-5.0, mu=(1.0, 0.05, 0.0, 2e-3, 0.0, 0.0), sig=(2e-3, 0
         return rng.uniform(-a, a, (n_out, n_in))
     p[0:192] = xav(6, 32).ravel()
     p[224:1248] = xav(32, 32).ravel()
+    if bf16_weights:
+        p[0:192] = round_bf16(p[0:192])
+        p[224:1248] = round_bf16(p[224:1248])
     p[1280:1312] = xav(32, 1).ravel() * w3_scale
     p[1312] = b3
     p[1313:1319] = mu

@@ -37,6 +37,18 @@ def xavier_blob(seed, b3=-2.25, mu=(1.0, 0.05, 0.0, 2e-3, 0.0, 0.0), sig=(2e-3, 
     return p
 
 
+def blob_for(seed, weights, **kw):
+    """weights "any": W1, W2 as drawn (the compensated network, three MFMAs
+    per product); "bf16": W1, W2 rounded to bf16 values (the exact-weight
+    network, two MFMAs per product)."""
+    p = xavier_blob(seed, **kw)
+    if weights == "bf16":
+        from ippi_amd import workloads
+        p[0:192] = workloads.round_bf16(p[0:192])
+        p[224:1248] = workloads.round_bf16(p[224:1248])
+    return p
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
@@ -91,25 +103,28 @@ def test_nn_fp64_matches_oracle(oracle, ippi, case):
     assert vel_err(a, b) <= 1e-12
 
 
+@pytest.mark.parametrize("weights", ["any", "bf16"])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_nn_fp32_within_budget(oracle, ippi, case):
-    blob = xavier_blob(1)
+def test_nn_fp32_within_budget(oracle, ippi, case, weights):
+    blob = blob_for(1, weights)
     o, g = run(oracle, ippi, case, "fp32", blob)
     a, b = g.fields(), o.fields()
     e = vel_err(a, b)
-    print(f"{case}: FP32 + bf16-MFMA MLP vs FP64 oracle: rel L2 u {e:.3e}")
+    print(f"{case} ({weights} weights): FP32 + bf16-MFMA MLP vs FP64 oracle: rel L2 u {e:.3e}")
     assert e <= 1e-5
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
-def test_nn_tau_field(oracle, ippi, precision, tol):
-    blob = xavier_blob(2)
+@pytest.mark.parametrize("precision,tol,weights", [("fp64", 1e-12, "any"), ("fp32", 2e-5, "any"),
+                                                   ("fp64", 1e-12, "bf16"), ("fp32", 2e-5, "bf16")])
+def test_nn_tau_field(oracle, ippi, precision, tol, weights):
+    blob = blob_for(2, weights)
     o, g = run(oracle, ippi, "bgk_nn_channel", precision, blob)
     o._blob = blob
     t_ref = oracle_tau(oracle, o)
     t_gpu = g.tau()
     err = np.abs(t_gpu - t_ref).max()
-    print(f"{precision}: max |tau_gpu - tau_oracle| = {err:.3e}, tau range {t_ref.min():.4f}..{t_ref.max():.4f}")
+    print(f"{precision} ({weights} weights): max |tau_gpu - tau_oracle| = {err:.3e}, "
+          f"tau range {t_ref.min():.4f}..{t_ref.max():.4f}")
     assert err <= tol
     assert t_ref.std() > 1e-4  # the network output actually varies over the field
 
