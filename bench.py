@@ -587,7 +587,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "FP32 fused MRT+LES(Cs=0.1)+IB step with network tau at every node (6-32-32-1 "
                                "SiLU, bf16 MFMA, seeded Xavier weights, guard >= 0.501), cylinder D=160 (503 "
-                               "markers), 4096x2048, inflow 0.08 +-1e-3 noise, Re=1e4, populations stored as f-w",
+                               "markers), 4096x2048, inflow 0.08 +-1e-3 noise, Re=1e4, FP32 populations stored as f - feq(1, (0.08, 0))",
                    "nx": NX, "ny": NY, "collision": "mrt", "les_cs": CS, "tau0": TAU0, "tau_mode": "nn",
                    "u_in": U0, "bodies": 1, "markers": NMARK, "bc": "inflow"},
         **extra,

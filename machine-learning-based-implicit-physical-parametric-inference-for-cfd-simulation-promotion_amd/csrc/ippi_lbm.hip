@@ -50,8 +50,23 @@ struct Patch {
     int m0;    // first marker of the body
 };
 
+// Reference state of the FP32 storage (s = f - c, c = feq(1, (U, 0)); see
+// lbm_node.hpp): U in both precisions, c in both precisions, and the
+// correction bb = c_7 - c_5 that halfway bounce-back adds where it returns
+// a diagonal population reversed (c_5 = c_8 and c_6 = c_7, so a reversed
+// population changes its offset by +-bb; the axis ones keep theirs).
+struct RefState {
+    double U;
+    float Uf;
+    float bb;
+    double c[9];
+    float cf[9];
+};
+
 struct StepArgs {
     int nx, ny;
+    RefState ref;           // FP32 storage offsets
+    float duinxf;           // FP32 inlet velocity minus U
     double tau0, cs, uinx, uiny;
     float tx0f, csf, uinxf, uinyf;  // FP32 path: tx0f = tau0 - 1/2
     int les;
@@ -142,8 +157,12 @@ __device__ __forceinline__ void st9(T* __restrict__ dst, unsigned N, unsigned n,
 //     above row ny-1 (the population leaving the node comes back reversed).
 //   IPPI_BC_INFLOW: rows 0 and ny-1 are specular free-slip rows
 //     (mesh.cpp:325-337 and its mirror image).
+// FP32 storage (s = f - c): a population returned reversed by bounce-back
+// changes offset (bb = c_7 - c_5, see RefState); the mirror of the
+// free-slip rows keeps it (c is symmetric in y).
 template <typename T, int BC>
-__device__ __forceinline__ void pull(const T* __restrict__ s, int x, int y, int nx, int ny, unsigned N, T* f) {
+__device__ __forceinline__ void pull(const T* __restrict__ s, int x, int y, int nx, int ny, unsigned N, T* f,
+                                     float bb = 0.0f) {
     const bool bot = y == 0, top = y == ny - 1;
     unsigned xc[3];  // lane columns for dx = -1, 0, +1
     if (BC == IPPI_BC_PERIODIC_BB) {
@@ -188,6 +207,16 @@ __device__ __forceinline__ void pull(const T* __restrict__ s, int x, int y, int 
         const unsigned row = (unsigned)(y + dy) * nx;
         f[k] = ld(s + (unsigned)pk * N, row + xc[dx + 1]);
     }
+    if constexpr (sizeof(T) == 4 && BC == IPPI_BC_PERIODIC_BB) {
+        if (bot) {
+            f[5] += bb;
+            f[6] -= bb;
+        }
+        if (top) {
+            f[7] -= bb;
+            f[8] += bb;
+        }
+    }
 }
 
 // Post-streaming state of node (x, y): populations (full f for FP64, f - w
@@ -221,14 +250,17 @@ __device__ __forceinline__ void node_state(const double* __restrict__ s, int x, 
     }
 }
 
-// FP32 node state: shifted populations g and their moments. `Mo` keeps the
-// partial sums shared by the moments, the stress and the MRT transform.
+// FP32 node state: stored deviations g = f - c and their moments (see
+// lbm_node.hpp). `Mo` keeps the partial sums shared by the moments, the
+// stress and the MRT transform, the deviations drho, (dux, duy) of the
+// density and velocity from the reference state (1, (U, 0)), and the full
+// velocity ux = U + dux, uy = duy.
 struct Mo {
-    float drho, rho, ir, ux, uy, jx, jy;
+    float drho, rho, ir, dux, duy, ux, uy;
     float s13, s24, d13, d24, s57, s68, sx, sy;
 };
 
-__device__ __forceinline__ void moments32(const float* g, Mo& o) {
+__device__ __forceinline__ void moments32(const float* g, float U, Mo& o) {
 #pragma clang fp contract(fast)
     o.s13 = g[1] + g[3];
     o.s24 = g[2] + g[4];
@@ -240,48 +272,70 @@ __device__ __forceinline__ void moments32(const float* g, Mo& o) {
     o.sx = d57 + (g[8] - g[6]);  // g5 - g6 - g7 + g8
     o.sy = d57 + (g[6] - g[8]);  // g5 + g6 - g7 - g8
     o.drho = g[0] + ((o.s13 + o.s24) + (o.s57 + o.s68));
-    o.jx = o.d13 + o.sx;
-    o.jy = o.d24 + o.sy;
+    const float djx = o.d13 + o.sx, djy = o.d24 + o.sy;
     o.rho = 1.0f + o.drho;
     if (o.rho < 1e-6f) {
         o.ir = 0.f;
-        o.ux = 0.f;
-        o.uy = 0.f;
+        o.dux = -U;
+        o.duy = 0.f;
     } else {
         o.ir = sh::rcp(o.rho);
-        o.ux = o.jx * o.ir;
-        o.uy = o.jy * o.ir;
+        o.dux = (djx - U * o.drho) * o.ir;
+        o.duy = djy * o.ir;
     }
+    o.ux = U + o.dux;
+    o.uy = o.duy;
+}
+
+// Non-equilibrium extrapolation in deviations: g = q(b) + (g - q(n)), the
+// node taking rho_b = 1 + drb and u_b = (U + dbx, dby).
+__device__ __forceinline__ void nee32(float* g, Mo& o, float drb, float dbx, float dby, const RefState& r) {
+#pragma clang fp contract(fast)
+    float qb[9], qn[9];
+    sh::geq_dev(drb, 1.0f + drb, dbx, dby, r.Uf, r.cf, qb);
+    sh::geq_dev(o.drho, o.rho, o.dux, o.duy, r.Uf, r.cf, qn);
+#pragma unroll
+    for (int k = 0; k < 9; k++) g[k] = qb[k] + (g[k] - qn[k]);
+    moments32(g, r.Uf, o);  // partial sums of the new populations
+    o.drho = drb;
+    o.rho = 1.0f + drb;
+    o.ir = sh::rcp(o.rho);
+    o.dux = dbx;
+    o.duy = dby;
+    o.ux = r.Uf + dbx;
+    o.uy = dby;
+}
+
+// Non-equilibrium second moments sum e e f - (rho u u + rho/3 I) from the
+// deviations: the stored sums less (drho U^2 + rho X + drho/3, ...) with
+// X = dux (2 U + dux), i.e. rho u_x^2 - U^2 (c's own moments cancel).
+__device__ __forceinline__ void neq32(const Mo& o, float U, float& nxx, float& nxy, float& nyy) {
+#pragma clang fp contract(fast)
+    const float d3 = o.drho * (1.0f / 3.0f);
+    const float X = o.dux * (2.0f * U + o.dux), Jy = o.rho * o.duy;
+    nxx = (o.s13 + (o.s57 + o.s68)) - ((o.drho * (U * U) + o.rho * X) + d3);
+    nyy = (o.s24 + (o.s57 + o.s68)) - (Jy * o.duy + d3);
+    nxy = (o.s57 - o.s68) - Jy * (U + o.dux);
 }
 
 template <int BC>
 __device__ __forceinline__ void node_state32(const float* __restrict__ s, int x, int y, int nx, int ny, unsigned N,
-                                             float uinx, float uiny, float* g, Mo& o) {
+                                             float duinx, float uiny, const RefState& r, float* g, Mo& o) {
 #pragma clang fp contract(fast)
     if (BC == IPPI_BC_PERIODIC_BB) {
-        pull<float, IPPI_BC_PERIODIC_BB>(s, x, y, nx, ny, N, g);
-        moments32(g, o);
+        pull<float, IPPI_BC_PERIODIC_BB>(s, x, y, nx, ny, N, g, r.bb);
+        moments32(g, r.Uf, o);
         return;
     }
     const bool nee = (x == 0) || (x == nx - 1);
     const int xs = x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x);
     pull<float, IPPI_BC_INFLOW>(s, xs, y, nx, ny, N, g);
-    moments32(g, o);
+    moments32(g, r.Uf, o);
     if (nee) {
-        float bx = x == 0 ? uinx : o.ux, by = x == 0 ? uiny : o.uy;
-        float qb[9], qn[9];
-        sh::geq(0.f, 1.f, bx, by, qb);
-        sh::geq(o.drho, o.rho, o.ux, o.uy, qn);
-#pragma unroll
-        for (int k = 0; k < 9; k++) g[k] = qb[k] + (g[k] - qn[k]);
-        moments32(g, o);  // partial sums of the new populations
-        o.drho = 0.f;
-        o.rho = 1.f;
-        o.ir = 1.f;
-        o.ux = bx;
-        o.uy = by;
-        o.jx = bx;
-        o.jy = by;
+        if (x == 0)
+            nee32(g, o, 0.f, duinx, uiny, r);
+        else
+            nee32(g, o, 0.f, o.dux, o.duy, r);
     }
 }
 
@@ -341,45 +395,30 @@ __device__ __forceinline__ bool fp_state64(const double* __restrict__ s, int x, 
     return true;
 }
 
-// FP32 (shifted populations) counterpart of fp_state64.
+// FP32 (deviation) counterpart of fp_state64.
 __device__ __forceinline__ bool fp_state32(const float* __restrict__ s, int x, int y, int nx, int ny, unsigned N,
-                                           float uinx, float uiny, int plate_x, float* g, Mo& o) {
+                                           float duinx, float uiny, int plate_x, const RefState& r, float* g, Mo& o) {
 #pragma clang fp contract(fast)
-    auto nee = [&](float drb, float rb, float bx, float by) {
-        float qb[9], qn[9];
-        sh::geq(drb, rb, bx, by, qb);
-        sh::geq(o.drho, o.rho, o.ux, o.uy, qn);
-#pragma unroll
-        for (int k = 0; k < 9; k++) g[k] = qb[k] + (g[k] - qn[k]);
-        moments32(g, o);  // partial sums of the new populations
-        o.drho = drb;
-        o.rho = rb;
-        o.ir = sh::rcp(rb);
-        o.ux = bx;
-        o.uy = by;
-        o.jx = rb * bx;
-        o.jy = rb * by;
-    };
     if (y == 0) {
         if (x == 0 || x == nx - 1) {
 #pragma unroll
             for (int k = 0; k < 9; k++) g[k] = ld(s + k * N, (unsigned)x);
-            moments32(g, o);
+            moments32(g, r.Uf, o);
             return false;
         }
         pull<float, IPPI_BC_INFLOW>(s, x, x < plate_x ? 0 : 1, nx, ny, N, g);
-        moments32(g, o);
-        if (x >= plate_x) nee(o.drho, o.rho, 0.f, 0.f);
+        moments32(g, r.Uf, o);
+        if (x >= plate_x) nee32(g, o, o.drho, -r.Uf, 0.f, r);  // 'd': rho from (x, 1), u = 0
         return true;
     }
     const int xs = x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x);
     pull<float, IPPI_BC_INFLOW>(s, xs, y == ny - 1 ? ny - 2 : y, nx, ny, N, g);
-    moments32(g, o);
+    moments32(g, r.Uf, o);
     if (x == 0)
-        nee(0.f, 1.f, uinx, uiny);
+        nee32(g, o, 0.f, duinx, uiny, r);
     else if (x == nx - 1)
-        nee(0.f, 1.f, o.ux, o.uy);
-    if (y == ny - 1) nee(0.f, 1.f, o.ux, o.uy);
+        nee32(g, o, 0.f, o.dux, o.duy, r);
+    if (y == ny - 1) nee32(g, o, 0.f, o.dux, o.duy, r);
     return true;
 }
 
@@ -515,10 +554,11 @@ constexpr int step_npt() {
 // Raw pulled populations of node (x, y) (for IPPI_BC_INFLOW the NEE columns
 // pull the neighbouring column), or the stored fnew state when !PULL.
 template <typename T, int BC, bool PULL>
-__device__ __forceinline__ void load_raw(const T* __restrict__ src, int x, int y, int nx, int ny, unsigned N, T* f) {
+__device__ __forceinline__ void load_raw(const T* __restrict__ src, int x, int y, int nx, int ny, unsigned N, T* f,
+                                         float bb = 0.0f) {
     if (PULL) {
         const int xs = BC == IPPI_BC_INFLOW ? (x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x)) : x;
-        pull<T, BC>(src, xs, y, nx, ny, N, f);
+        pull<T, BC>(src, xs, y, nx, ny, N, f, bb);
     } else {
         const unsigned n = (unsigned)y * nx + x;
 #pragma unroll
@@ -624,7 +664,7 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
         for (int k = 0; k < 9; k++) v[k] = fresh ? o[k] : f[k];
         if (a.round32) {  // block-uniform
 #pragma unroll
-            for (int k = 0; k < 9; k++) v[k] = (double)(float)(v[k] - WD[k]) + WD[k];
+            for (int k = 0; k < 9; k++) v[k] = (double)(float)(v[k] - a.ref.c[k]) + a.ref.c[k];
         }
         st9<NT>(dst, N, n, v);
     }
@@ -680,34 +720,28 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     Mo o;
     bool fresh = true;  // false: flat-plate corner, stored populations copied through
     float keep[BC == IPPI_BC_FLATPLATE ? 9 : 1];
+    const RefState& r = a.ref;
+    const float U = r.Uf;
     if (PULL && BC == IPPI_BC_FLATPLATE) {
-        fresh = fp_state32(src, x, y, nx, a.ny, N, a.uinxf, a.uinyf, a.plate_x, g, o);
+        fresh = fp_state32(src, x, y, nx, a.ny, N, a.duinxf, a.uinyf, a.plate_x, r, g, o);
 #pragma unroll
         for (int k = 0; k < (BC == IPPI_BC_FLATPLATE ? 9 : 1); k++) keep[k] = g[k];
     } else {
-        moments32(g, o);
+        moments32(g, U, o);
     }
     if (PULL && BC != IPPI_BC_FLATPLATE) {
         if (BC == IPPI_BC_INFLOW && (x == 0 || x == nx - 1)) {  // NEE, mesh.cpp:239-286
-            float bx = x == 0 ? a.uinxf : o.ux, by = x == 0 ? a.uinyf : o.uy;
-            float qb[9], qn[9];
-            sh::geq(0.f, 1.f, bx, by, qb);
-            sh::geq(o.drho, o.rho, o.ux, o.uy, qn);
-#pragma unroll
-            for (int k = 0; k < 9; k++) g[k] = qb[k] + (g[k] - qn[k]);
-            moments32(g, o);  // partial sums of the new populations
-            o.drho = 0.f;
-            o.rho = 1.f;
-            o.ir = 1.f;
-            o.ux = bx;
-            o.uy = by;
+            if (x == 0)
+                nee32(g, o, 0.f, a.duinxf, a.uinyf, r);
+            else
+                nee32(g, o, 0.f, o.dux, o.duy, r);
         }
     } else if (!PULL) {
         o.rho = (float)mac[n];
         o.drho = (float)(mac[n] - 1.0);
         o.ir = sh::rcp(o.rho);
-        o.ux = (float)mac[N + n];
-        o.uy = (float)mac[2 * N + n];
+        o.dux = (float)(mac[N + n] - r.U);
+        o.duy = (float)mac[2 * N + n];
     }
     if (!(o.rho >= 0.f && o.rho <= 10.f)) *a.flag = 1;
     float fx = 0.f, fy = 0.f;
@@ -719,26 +753,29 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
             fx = (float)fxd;
             fy = (float)fyd;
             const float h = 0.5f * o.ir;
-            o.ux += fx * h;
-            o.uy += fy * h;
+            o.dux += fx * h;
+            o.duy += fy * h;
         }
     }
     // relaxation as tx = tau - 1/2 (see lbm_node.hpp)
     const float txb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tx0f;
     float tx = txb;
-    const float rho = o.rho, ux = o.ux, uy = o.uy;
-    const float rux = rho * ux, ruy = rho * uy;
-    // non-equilibrium second moments: sum e e g - (rho u u + drho/3 I)
-    const float d3 = o.drho * (1.0f / 3.0f);
-    const float nxx = (o.s13 + (o.s57 + o.s68)) - (rux * ux + d3);
-    const float nyy = (o.s24 + (o.s57 + o.s68)) - (ruy * uy + d3);
-    const float nxy = (o.s57 - o.s68) - rux * uy;
+    const float rho = o.rho, drho = o.drho, dux = o.dux, duy = o.duy;
+    const float ux = U + dux, uy = duy;
+    // deviations from the reference state's: momentum J = rho u - (U, 0),
+    // rho |u|^2 - U^2 = drho U^2 + rho Q, rho u_x^2 - U^2 = drho U^2 + rho X
+    const float Jx = drho * U + rho * dux, Jy = rho * duy;
+    const float X = dux * (2.0f * U + dux), Q = X + duy * duy;
+    const float dU2 = drho * (U * U);
+    float nxx, nxy, nyy;  // non-equilibrium second moments
+    neq32(o, U, nxx, nxy, nyy);
     bool nn_ok = false;
     if (NN) {  // guard of mesh.cpp:2821-2827: NN value if >= 0.501, else LES / base
+        // features as deviations (rho - 1, u - (U, 0)); the host shifts mu to match
         float feat[6];
-        feat[0] = rho;
-        feat[1] = ux;
-        feat[2] = uy;
+        feat[0] = drho;
+        feat[1] = dux;
+        feat[2] = duy;
         feat[3] = 1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy));
         feat[4] = nxx - nyy;
         feat[5] = nxy;
@@ -750,7 +787,7 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
     const float dl = sh::dl_of(tx);  // 2 - omega
     if (COLL == IPPI_COLL_RR) {
         float q[9];
-        sh::geq(o.drho, rho, ux, uy, q);
+        sh::geq_dev(drho, rho, dux, duy, U, r.cf, q);
         sh::rr(g, q, nxx, nxy, nyy, dl, fx, fy);
     } else {
         // moments of g (Lallemand-Luo basis, NodeAndLattice.cpp:18-27)
@@ -761,14 +798,15 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         const float m6 = -2.0f * o.d24 + o.sy;
         const float m7 = o.s13 - o.s24;
         const float m8 = o.s57 - o.s68;
-        const float t1 = ux * ux + uy * uy, t2 = ux * ux - uy * uy;
-        const float e1 = -2.0f * o.drho + 3.0f * rho * t1;
-        const float e2 = o.drho - 3.0f * rho * t1;
+        // equilibrium moments less those of c: energy drho U^2 + rho Q
+        // terms, p_xx rho u_x^2 - rho u_y^2 - U^2, p_xy rho u_x u_y
+        const float e1 = -2.0f * drho + 3.0f * (dU2 + rho * Q);
+        const float e2 = drho - 3.0f * (dU2 + rho * Q);
         // c_l = s_l (m_l - meq_l) / |M_l|^2; a rate s_l = omega = 2 - dl is
         // applied as 2 x - dl x (one rounding, no rounded omega)
         auto om_x = [dl](float x) { return 2.0f * x - dl * x; };
         const float x1 = (m1 - e1) * (1.0f / 36.0f), x2 = (m2 - e2) * (1.0f / 36.0f);
-        const float x4 = (m4 + rux) * (1.0f / 12.0f), x6 = (m6 + ruy) * (1.0f / 12.0f);
+        const float x4 = (m4 + Jx) * (1.0f / 12.0f), x6 = (m6 + Jy) * (1.0f / 12.0f);
         float c1, c2, c4, c6;
         if (COLL == IPPI_COLL_MRT) {  // fixed rates, NodeAndLattice.cpp:306
             c1 = 1.2f * x1;
@@ -781,8 +819,8 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
             c4 = om_x(x4);
             c6 = om_x(x6);
         }
-        const float c7 = om_x((m7 - rho * t2) * 0.25f);
-        const float c8 = om_x((m8 - rux * uy) * 0.25f);
+        const float c7 = om_x((m7 - ((dU2 + rho * X) - Jy * duy)) * 0.25f);
+        const float c8 = om_x((m8 - Jy * ux) * 0.25f);
         // BGK relaxes the momentum towards rho u (shifted by F/2 under IB
         // forcing): exact target -omega F/2; MRT does not relax it.
         float c3 = 0.f, c5 = 0.f;
@@ -836,7 +874,7 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
     constexpr bool RAW = !(PULL && BC == IPPI_BC_FLATPLATE);  // the flat-plate rules load for themselves
     if (NN == 3) mlp_frage_to_lds<BX>(a.mlp, lds_frag);  // in flight with the population loads
     float cur[9];
-    if (RAW) load_raw<float, BC, PULL>(src, x, y0, nx, ny, N, cur);
+    if (RAW) load_raw<float, BC, PULL>(src, x, y0, nx, ny, N, cur, a.ref.bb);
     if (NN == 3) __syncthreads();
 #pragma unroll
     for (int j = 0; j < NPT; j++) {
@@ -844,7 +882,7 @@ k_step_f32(const float* __restrict__ src, float* __restrict__ dst,
         if (y >= ny) break;
         const bool more = j + 1 < NPT && y + 1 < ny;
         float nxt[9];
-        if (RAW && j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt);
+        if (RAW && j + 1 < NPT) load_raw<float, BC, PULL>(src, x, min(y + 1, ny - 1), nx, ny, N, nxt, a.ref.bb);
         node_f32<COLL, BC, PULL, NN, NT>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX, lds_frag);
         if (more) {
 #pragma unroll
@@ -874,7 +912,7 @@ __global__ void k_stencil(const T* __restrict__ prev, const int* __restrict__ xy
 #pragma unroll
     for (int q = 0; q < 9; q++) {
         const T v = ld(prev + q * N, (unsigned)(sy - EY[q]) * nx + (sx - EX[q]));
-        f[q] = sizeof(T) == 8 ? (double)v : (double)v + WD[q];
+        f[q] = sizeof(T) == 8 ? (double)v : (double)v + a.ref.c[q];
     }
     double rho, ux, uy, fq[9];
     ref::macro(f, rho, ux, uy);
@@ -950,28 +988,26 @@ __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, dou
     if constexpr (BC == IPPI_BC_FLATPLATE) {
         if (y == 0 && (x == 0 || x == nx - 1)) {  // corner: initial populations, macros from mac
 #pragma unroll
-            for (int k = 0; k < 9; k++) g[k] = (float)(a.corner[9 * (x != 0) + k] - WD[k]);
-            moments32(g, o);
+            for (int k = 0; k < 9; k++) g[k] = (float)(a.corner[9 * (x != 0) + k] - a.ref.c[k]);
+            moments32(g, a.ref.Uf, o);
             o.rho = (float)a.mac[n];
             o.drho = (float)(a.mac[n] - 1.0);
             o.ir = sh::rcp(o.rho);
-            o.ux = (float)a.mac[N + n];
-            o.uy = (float)a.mac[2 * N + n];
+            o.dux = (float)(a.mac[N + n] - a.ref.U);
+            o.duy = (float)a.mac[2 * N + n];
         } else {
-            fp_state32(src, x, y, nx, ny, N, a.uinxf, a.uinyf, a.plate_x, g, o);
+            fp_state32(src, x, y, nx, ny, N, a.duinxf, a.uinyf, a.plate_x, a.ref, g, o);
         }
     } else {
-        node_state32<BC>(src, x, y, nx, ny, N, a.uinxf, a.uinyf, g, o);
+        node_state32<BC>(src, x, y, nx, ny, N, a.duinxf, a.uinyf, a.ref, g, o);
     }
     const float txb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tx0f;
-    const float d3 = o.drho * (1.0f / 3.0f);
-    const float nxx = (o.s13 + (o.s57 + o.s68)) - (o.rho * o.ux * o.ux + d3);
-    const float nyy = (o.s24 + (o.s57 + o.s68)) - (o.rho * o.uy * o.uy + d3);
-    const float nxy = (o.s57 - o.s68) - o.rho * o.ux * o.uy;
+    float nxx, nxy, nyy;
+    neq32(o, a.ref.Uf, nxx, nxy, nyy);
     float tx = txb;
     bool nn_ok = false;
-    if (NN) {
-        float feat[6] = {o.rho, o.ux, o.uy,
+    if (NN) {  // features as deviations, as in node_f32
+        float feat[6] = {o.drho, o.dux, o.duy,
                          1.5f * o.ir * sh::fsqrt(2.0f * (nxx * nxx + 2.0f * nxy * nxy + nyy * nyy)), nxx - nyy, nxy};
         const float t = mlp_tau_nn<NN>(feat, a.mlp, lds_mlp + (threadIdx.x & ~63) * 16, lds_frag);
         nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
@@ -1029,7 +1065,7 @@ __global__ void __launch_bounds__(BX) k_tau64(const double* __restrict__ src, do
 template <typename T, int BC>
 __global__ void __launch_bounds__(BX) k_restrict(const T* __restrict__ src, double* __restrict__ fc,
                                                  double* __restrict__ mc, int nx, int ny, double uinx,
-                                                 double uiny, double s, int want_f) {
+                                                 double uiny, double s, int want_f, RefState ref) {
 #pragma clang fp contract(off)
     const int nxc = nx / 2, nyc = ny / 2;
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y;
@@ -1045,9 +1081,9 @@ __global__ void __launch_bounds__(BX) k_restrict(const T* __restrict__ src, doub
             } else {
                 float g[9];
                 Mo o;
-                node_state32<BC>(src, x, y, nx, ny, N, (float)uinx, (float)uiny, g, o);
+                node_state32<BC>(src, x, y, nx, ny, N, (float)(uinx - ref.U), (float)uiny, ref, g, o);
 #pragma unroll
-                for (int k = 0; k < 9; k++) f[k] = (double)g[k] + WD[k];
+                for (int k = 0; k < 9; k++) f[k] = (double)g[k] + ref.c[k];
                 ref::macro(f, r, vx, vy);
             }
             double q[9];
@@ -1164,7 +1200,7 @@ __global__ void __launch_bounds__(BX) k_labels(const double* __restrict__ fc, co
 // (mode 1); f = feq (InitialLatticeFeq form), stored as the fnew state.
 template <typename T>
 __global__ void k_init(T* __restrict__ dst, double* __restrict__ mac, size_t N, int mode,
-                       unsigned long long seed, double rho0, double ux0, double uy0, double amp) {
+                       unsigned long long seed, double rho0, double ux0, double uy0, double amp, RefState ref) {
 #pragma clang fp contract(off)
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -1188,7 +1224,7 @@ __global__ void k_init(T* __restrict__ dst, double* __restrict__ mac, size_t N, 
         if constexpr (sizeof(T) == 8)
             dst[k * N + n] = f[k];
         else
-            dst[k * N + n] = (float)(f[k] - WD[k]);
+            dst[k * N + n] = (float)(f[k] - ref.c[k]);
     }
 }
 
@@ -1207,19 +1243,19 @@ __global__ void k_mean_phase0(const double* __restrict__ mac, double* __restrict
 // from the stored initial state (full f, FP64): no rule updates them, so
 // they are the populations these nodes keep.
 template <typename T>
-__global__ void k_corner(const T* __restrict__ src, double* __restrict__ corner, int nx, unsigned N) {
+__global__ void k_corner(const T* __restrict__ src, double* __restrict__ corner, int nx, unsigned N, RefState ref) {
     const int t = threadIdx.x;
     if (t >= 18) return;
     const int k = t % 9;
     double v = (double)src[(size_t)k * N + (t < 9 ? 0u : (unsigned)(nx - 1))];
-    if constexpr (sizeof(T) == 4) v += WD[k];
+    if constexpr (sizeof(T) == 4) v += ref.c[k];
     corner[t] = v;
 }
 
 template <typename T, int BC>
 __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, double* __restrict__ f_out, int nx,
                         int ny, double uinx, double uiny, int plate_x, const double* __restrict__ corner,
-                        double* __restrict__ mean, double kk) {
+                        double* __restrict__ mean, double kk, RefState ref) {
 #pragma clang fp contract(off)
     const int x = blockIdx.x * BX + threadIdx.x;
     const int y = blockIdx.y;
@@ -1244,7 +1280,7 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
         } else {
             float g[9];
             Mo o;
-            fp_state32(src, x, y, nx, ny, N, (float)uinx, (float)uiny, plate_x, g, o);
+            fp_state32(src, x, y, nx, ny, N, (float)(uinx - ref.U), (float)uiny, plate_x, ref, g, o);
             const bool assigned = x == 0 || x == nx - 1 || y == ny - 1 || (y == 0 && x >= plate_x);
             double gd[9], sm = 0.0;
 #pragma unroll
@@ -1252,18 +1288,18 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
             for (int k = 0; k < 9; k++) sm += gd[k];
             const double jx = ((gd[1] - gd[3]) + (gd[5] - gd[7])) + (gd[8] - gd[6]);
             const double jy = ((gd[2] - gd[4]) + (gd[5] - gd[7])) + (gd[6] - gd[8]);
-            rho = assigned ? (double)o.rho : 1.0 + sm;
-            ux = assigned ? (double)o.ux : jx / rho;
-            uy = assigned ? (double)o.uy : jy / rho;
+            rho = assigned ? 1.0 + (double)o.drho : 1.0 + sm;
+            ux = assigned ? ref.U + (double)o.dux : (ref.U + jx) / rho;
+            uy = assigned ? (double)o.duy : jy / rho;
 #pragma unroll
-            for (int k = 0; k < 9; k++) f[k] = gd[k] + WD[k];
+            for (int k = 0; k < 9; k++) f[k] = gd[k] + ref.c[k];
         }
     } else if constexpr (sizeof(T) == 8) {
         node_state(src, x, y, nx, ny, N, BC, uinx, uiny, f, rho, ux, uy);
     } else {
         float g[9];
         Mo o;
-        node_state32<BC>(src, x, y, nx, ny, N, (float)uinx, (float)uiny, g, o);
+        node_state32<BC>(src, x, y, nx, ny, N, (float)(uinx - ref.U), (float)uiny, ref, g, o);
         double s = 0.0, jx, jy;
         double gd[9];
 #pragma unroll
@@ -1273,10 +1309,10 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
         jy = ((gd[2] - gd[4]) + (gd[5] - gd[7])) + (gd[6] - gd[8]);
         const bool nee = BC == IPPI_BC_INFLOW && (x == 0 || x == nx - 1);
         rho = nee ? 1.0 : 1.0 + s;
-        ux = nee ? (double)o.ux : jx / rho;
-        uy = nee ? (double)o.uy : jy / rho;
+        ux = nee ? ref.U + (double)o.dux : (ref.U + jx) / rho;
+        uy = nee ? (double)o.duy : jy / rho;
 #pragma unroll
-        for (int k = 0; k < 9; k++) f[k] = gd[k] + WD[k];
+        for (int k = 0; k < 9; k++) f[k] = gd[k] + ref.c[k];
     }
     mac[n] = rho;
     mac[N + n] = ux;
@@ -1334,11 +1370,15 @@ struct IbDev {
     const double* ecw;   // [active][E] weight (0 in padding)
     double* patch_f;     // [patch node][2]
     double* hist;        // [body][2] of this step's record
+    RefState ref;        // FP32 storage offsets (the interpolated deviation gets U back)
 };
 
+// Velocity of node (X, Y) for the interpolation: FP64 the full velocity;
+// FP32 its deviation (u - U, u_y) from the reference state, to which the
+// interpolation adds U at the end (the 16 weights of a marker sum to 1).
 template <typename T, bool PULL>
 __device__ __forceinline__ void support_u(const T* __restrict__ src, const double* __restrict__ mac, int X, int Y,
-                                          int nx, unsigned N, double& ux, double& uy) {
+                                          int nx, unsigned N, double& ux, double& uy, const RefState& rs) {
 #pragma clang fp contract(off)
     // supports lie in the interior, so both boundary kinds pull the plain stencil
     const unsigned n = (unsigned)Y * nx + X;
@@ -1359,16 +1399,16 @@ __device__ __forceinline__ void support_u(const T* __restrict__ src, const doubl
             ref::macro(f, r, ux, uy);
         } else {
             Mo o;
-            moments32(f, o);
-            ux = o.ux;
-            uy = o.uy;
+            moments32(f, rs.Uf, o);
+            ux = o.dux;
+            uy = o.duy;
         }
     } else {
         if constexpr (sizeof(T) == 8) {
             ux = mac[N + n];
             uy = mac[2 * N + n];
-        } else {
-            ux = (double)(float)mac[N + n];
+        } else {  // as the FP32 step's first collision reads them
+            ux = (double)(float)(mac[N + n] - rs.U);
             uy = (double)(float)mac[2 * N + n];
         }
     }
@@ -1379,16 +1419,16 @@ __device__ __forceinline__ void support_u(const T* __restrict__ src, const doubl
 // load each instead of the node list and the stencil gathers).
 // Velocity from the nine pulled populations, as support_u computes it.
 template <typename T>
-__device__ __forceinline__ void slot_u(const T* f, double& ux, double& uy) {
+__device__ __forceinline__ void slot_u(const T* f, double& ux, double& uy, const RefState& rs) {
 #pragma clang fp contract(off)
     if constexpr (sizeof(T) == 8) {
         double r;
         ref::macro(f, r, ux, uy);
     } else {
         Mo o;
-        moments32(f, o);
-        ux = o.ux;
-        uy = o.uy;
+        moments32(f, rs.Uf, o);
+        ux = o.dux;
+        uy = o.duy;
     }
 }
 
@@ -1435,7 +1475,7 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
                 T f[9];
 #pragma unroll
                 for (int k = 0; k < 9; k++) f[k] = slot[(size_t)k * nact_all + i];
-                slot_u(f, ux[j], uy[j]);
+                slot_u(f, ux[j], uy[j], d.ref);
             }
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -1455,7 +1495,7 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int r = nd[j] - bk.pad;
-            support_u<T, PULL>(src, mac, bk.x0 + r % bk.w, bk.y0 + r / bk.w, nx, N, ux[j], uy[j]);
+            support_u<T, PULL>(src, mac, bk.x0 + r % bk.w, bk.y0 + r / bk.w, nx, N, ux[j], uy[j], d.ref);
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -1485,6 +1525,7 @@ __global__ void __launch_bounds__(IB_T) k_ib(const T* __restrict__ src, const do
             uix += (U[2 * pi[s]] * pw[s]);
             uiy += (U[2 * pi[s] + 1] * pw[s]);
         }
+        if constexpr (sizeof(T) == 4) uix = uix + d.ref.U;  // FP32: deviations were interpolated
         const double Fx = 2 * (ux0 - uix), Fy = 2 * (uy0 - uiy);
         FB[2 * ml] = Fx;
         FB[2 * ml + 1] = Fy;
@@ -1583,8 +1624,8 @@ __device__ void ib_in_step(const StepArgs& a, int body, const T* __restrict__ sr
         for (int j = 0; j < NA; j++) {
             const int r = nd[j] - bk.pad;
             double vx, vy;
-            support_u<T, true>(src, nullptr, bk.x0 + r % bk.w, bk.y0 + r / bk.w, a.nx, N, vx, vy);
-            ux[j] = (T)vx;  // exact: the FP32 path's moments are FP32
+            support_u<T, true>(src, nullptr, bk.x0 + r % bk.w, bk.y0 + r / bk.w, a.nx, N, vx, vy, a.ref);
+            ux[j] = (T)vx;  // exact: the FP32 path's velocity deviations are FP32
             uy[j] = (T)vy;
         }
 #pragma unroll
@@ -1622,6 +1663,7 @@ __device__ void ib_in_step(const StepArgs& a, int body, const T* __restrict__ sr
                     uiy += ((double)U[2 * pi[s] + 1] * pw[s]);
                 }
             }
+            if constexpr (sizeof(T) == 4) uix = uix + a.ref.U;  // FP32: deviations were interpolated
             const double Fx = 2 * (a.ib_ud[2 * m] - uix), Fy = 2 * (a.ib_ud[2 * m + 1] - uiy);
             a.ib_ui[2 * m] = uix;
             a.ib_ui[2 * m + 1] = uiy;
@@ -1721,7 +1763,7 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     u32x4* lds_frag = reinterpret_cast<u32x4*>(U);
     if (NN == 3) mlp_frage_to_lds<BX>(a.mlp, lds_frag);  // in flight with the population loads
     float cur[9];
-    load_raw<float, BC, true>(src, x, y, nx, ny, N, cur);
+    load_raw<float, BC, true>(src, x, y, nx, ny, N, cur, a.ref.bb);
     if (NN == 3) __syncthreads();
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
     node_f32<COLL, BC, true, NN, NT, true>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX, lds_frag);
@@ -1818,6 +1860,7 @@ __device__ __forceinline__ void pull_adj(const T* __restrict__ s, int x, int y, 
 
 struct AdjArgs {
     int nx, ny;
+    RefState ref;           // FP32 storage offsets
     double tau0;
     int tau_field;
     const void* tau_ptr;
@@ -1862,20 +1905,25 @@ __device__ __forceinline__ int adj_patch(const AdjArgs& a, int x, int y, int bx0
     return -1;
 }
 
-template <typename T>
-__device__ __forceinline__ void adj_eq(T drho, T rho, T ux, T uy, T* q) {
+// Equilibrium at the corrected velocity (usx, usy) in the representation of
+// the stored populations: full feq (FP64), or its deviation from c (FP32).
+template <typename T, typename A>
+__device__ __forceinline__ void adj_eq(T drho, T rho, A usx, A usy, const RefState& rs, T* q) {
     if constexpr (sizeof(T) == 8)
-        ref::feq(rho, ux, uy, q);
+        ref::feq(rho, (double)usx, (double)usy, q);
+    else if constexpr (sizeof(A) == 8)
+        sh::geq_dev(drho, rho, (float)(usx - rs.U), (float)usy, rs.Uf, rs.cf, q);
     else
-        sh::geq(drho, rho, ux, uy, q);
+        sh::geq_dev(drho, rho, usx - rs.Uf, usy, rs.Uf, rs.cf, q);
 }
 
 // Forward state of the step at node (x, y): pulled populations (or the
 // stored first state with the macro arrays), density and velocity.
 template <typename T, bool PULL>
 __device__ __forceinline__ void adj_fwd(const T* __restrict__ src, const double* __restrict__ mac, int x, int y,
-                                        int nx, int ny, unsigned N, T* f, T& drho, T& rho, T& ux, T& uy) {
-    load_raw<T, IPPI_BC_PERIODIC_BB, PULL>(src, x, y, nx, ny, N, f);
+                                        int nx, int ny, unsigned N, const RefState& rs, T* f, T& drho, T& rho, T& ux,
+                                        T& uy) {
+    load_raw<T, IPPI_BC_PERIODIC_BB, PULL>(src, x, y, nx, ny, N, f, rs.bb);
     const unsigned n = (unsigned)y * nx + x;
     if (PULL) {
         T s = (f[0] + (f[1] + f[3])) + ((f[2] + f[4]) + ((f[5] + f[7]) + (f[6] + f[8])));
@@ -1891,8 +1939,11 @@ __device__ __forceinline__ void adj_fwd(const T* __restrict__ src, const double*
         if (rho < T(1e-6)) {
             ux = T(0);
             uy = T(0);
-        } else {
+        } else if constexpr (sizeof(T) == 8) {
             ux = jx / rho;
+            uy = jy / rho;
+        } else {  // deviations of the momentum from (U, 0)
+            ux = rs.Uf + (jx - rs.Uf * drho) / rho;
             uy = jy / rho;
         }
     } else {
@@ -1924,23 +1975,39 @@ __global__ void __launch_bounds__(256) k_adj_act(const A* __restrict__ lin, cons
     const unsigned N = (unsigned)nx * ny;
     const int node = d.act_node[i], r = d.act[i];
     const int x = node % nx, y = node / nx;
-    A l[9];
-    S f[9], drho, rho, ux, uy;
-    pull_adj(lin, x, y, nx, ny, N, l);
+    // FP64 throughout (a few thousand nodes): the centre gradient sums the
+    // force adjoint and the velocity against the delta's derivative, whose
+    // weights sum to zero, so the rounding of FP32 arithmetic on these
+    // quantities would not cancel with the rest
+    A la[9];
+    pull_adj(lin, x, y, nx, ny, N, la);
     const A c0 = lin[0];
+    double l[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) l[k] -= c0;
-    adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, f, drho, rho, ux, uy);
-    const adj::NodeFwd<A> s = adj::node_fwd<A>((A)rho, (A)ux, (A)uy, (A)a.patch_f[2 * r],
-                                                (A)a.patch_f[2 * r + 1], adj_tx<S, A>(a, (unsigned)node));
-    const adj::WMom<A> m = adj::wmoments(l);
-    const adj::UsAdj<A> ua = adj::us_adjoint(m, s, true);
-    A lfx, lfy;
+    for (int k = 0; k < 9; k++) l[k] = (double)(la[k] - c0);
+    double rho, ux, uy;
+    if (PULL) {
+        S fs[9];
+        load_raw<S, IPPI_BC_PERIODIC_BB, true>(src, x, y, nx, ny, N, fs, a.ref.bb);
+        double fd[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) fd[k] = sizeof(S) == 8 ? (double)fs[k] : (double)fs[k] + a.ref.c[k];
+        ref::macro(fd, rho, ux, uy);
+    } else {
+        rho = mac[node];
+        ux = mac[N + node];
+        uy = mac[2 * N + node];
+    }
+    const adj::NodeFwd<double> s = adj::node_fwd<double>(rho, ux, uy, a.patch_f[2 * r], a.patch_f[2 * r + 1],
+                                                         adj_tx<S, double>(a, (unsigned)node));
+    const adj::WMom<double> m = adj::wmoments(l);
+    const adj::UsAdj<double> ua = adj::us_adjoint(m, s, true);
+    double lfx, lfy;
     adj::force_adjoint(ua, s, lfx, lfy);
-    d.lf[2 * i] = (double)lfx;
-    d.lf[2 * i + 1] = (double)lfy;
-    d.u[2 * i] = (double)ux;
-    d.u[2 * i + 1] = (double)uy;
+    d.lf[2 * i] = lfx;
+    d.lf[2 * i + 1] = lfy;
+    d.u[2 * i] = ux;
+    d.u[2 * i + 1] = uy;
 }
 
 // One block per body. cj = weight of this step's force record in the
@@ -2073,7 +2140,7 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
     const A c0 = lin[0];  // constant-mode removal
 #pragma unroll
     for (int k = 0; k < 9; k++) l[k] -= c0;
-    adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, f, drho, rho, ux, uy);
+    adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, a.ref, f, drho, rho, ux, uy);
     A fx = A(0), fy = A(0), lix = A(0), liy = A(0);
     const int bx0 = bxi * BX;
     const int r = a.nbody > 0 ? adj_patch(a, x, y, bx0, bx0 + BX - 1) : -1;
@@ -2089,7 +2156,7 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
     const adj::UsAdj<A> ua = adj::us_adjoint(m, s, forced);
     // non-equilibrium part in the representation of the stored populations
     S q[9];
-    adj_eq(drho, rho, (S)ua.usx, (S)ua.usy, q);
+    adj_eq(drho, rho, ua.usx, ua.usy, a.ref, q);
     A dd[9], nu[9], dt;
 #pragma unroll
     for (int k = 0; k < 9; k++) dd[k] = (A)(f[k] - q[k]);
@@ -2116,6 +2183,7 @@ __global__ void k_adj_flush(T* __restrict__ part, double* __restrict__ total, si
 
 struct ippi_handle {
     ippi_config c{};
+    RefState ref{};  // FP32 storage offsets: s = f - feq(1, (u_in_x, 0))
     size_t N = 0;
     size_t esz = 8;
     hipStream_t s = nullptr;
@@ -2298,6 +2366,8 @@ StepArgs make_args(ippi_handle* h) {
     StepArgs a{};
     a.nx = h->c.nx;
     a.ny = h->c.ny;
+    a.ref = h->ref;
+    a.duinxf = (float)(h->c.u_in_x - h->ref.U);
     a.tau0 = h->c.tau0;
     a.cs = h->c.cs;
     a.uinx = h->c.u_in_x;
@@ -2527,7 +2597,7 @@ template <typename T>
 int launch_ib(ippi_handle* h, bool pull) {
     const int nb = (int)h->bodies.size();
     IbDev d{h->d_w, h->d_pidx, h->d_ud, h->d_ui, h->d_F, h->d_blk, h->d_act, h->d_ecnt, h->d_emk, h->d_ecw,
-            h->d_pf, h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap)};
+            h->d_pf, h->hist + 2 * (size_t)nb * (size_t)(h->nrec % h->hist_cap), h->ref};
     const T* src = (const T*)h->buf[h->cur];
     launch_ib_to<T>(h, src, pull, d, pull && h->slots_ok, true);
     HIPC(h, hipGetLastError());
@@ -2620,7 +2690,7 @@ int run_steps(ippi_handle* h, int nsteps) {
         const bool pull = h->phase == 1;
         if (!pull && h->c.bc == IPPI_BC_FLATPLATE) {
             hipLaunchKernelGGL(k_corner<T>, dim3(1), dim3(64), 0, h->s, (const T*)h->buf[h->cur], h->corner, nx,
-                               (unsigned)h->N);
+                               (unsigned)h->N, h->ref);
             HIPC(h, hipGetLastError());
         }
         const int nk = nn_kind(h);
@@ -2720,7 +2790,8 @@ int run_macro(ippi_handle* h, double* f_out, double* mean = nullptr, double kk =
             HIPC(h, hipStreamSynchronize(h->s));
             for (size_t k = 0; k < 9; k++)
                 for (size_t n = 0; n < h->N; n++)
-                    f_out[k * h->N + n] = sizeof(T) == 8 ? (double)tmp[k * h->N + n] : (double)tmp[k * h->N + n] + WD[k];
+                    f_out[k * h->N + n] =
+                        sizeof(T) == 8 ? (double)tmp[k * h->N + n] : (double)tmp[k * h->N + n] + h->ref.c[k];
         }
         return IPPI_OK;
     }
@@ -2732,13 +2803,13 @@ int run_macro(ippi_handle* h, double* f_out, double* mean = nullptr, double kk =
     const double ux = h->c.u_in_x, uy = h->c.u_in_y;
     if (h->c.bc == IPPI_BC_PERIODIC_BB)
         hipLaunchKernelGGL((k_macro<T, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
-                           px, h->corner, mean, kk);
+                           px, h->corner, mean, kk, h->ref);
     else if (h->c.bc == IPPI_BC_INFLOW)
         hipLaunchKernelGGL((k_macro<T, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
-                           px, h->corner, mean, kk);
+                           px, h->corner, mean, kk, h->ref);
     else
         hipLaunchKernelGGL((k_macro<T, IPPI_BC_FLATPLATE>), grid, dim3(BX), 0, h->s, src, h->mac, d_f, nx, ny, ux, uy,
-                           px, h->corner, mean, kk);
+                           px, h->corner, mean, kk, h->ref);
     HIPC(h, hipGetLastError());
     if (f_out) {
         HIPC(h, hipMemcpyAsync(f_out, d_f, sizeof(double) * 9 * h->N, hipMemcpyDeviceToHost, h->s));
@@ -2871,6 +2942,7 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     aa.nx = nx;
     aa.ny = ny;
     aa.tau0 = h->c.tau0;
+    aa.ref = h->ref;
     aa.tau_field = h->c.tau_mode == IPPI_TAU_FIELD;
     aa.tau_ptr = h->tau;
     aa.patch_l = pl;
@@ -2936,7 +3008,7 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
         for (int i = 0; i < Ks; i++) {
             const bool pull = !(t0 + i == 0 && phase0 == 0);
             IbDev d{h->d_w, h->d_pidx, h->d_ud, ui, Fsb + 2 * (size_t)nm * i, h->d_blk, h->d_act, h->d_ecnt,
-                    h->d_emk, h->d_ecw, pfb + 2 * (size_t)h->npn * i, nullptr};
+                    h->d_emk, h->d_ecw, pfb + 2 * (size_t)h->npn * i, nullptr, h->ref};
             launch_ib_to<T>(h, state(sg, i), pull, d, false, false, st);
             if (i + 1 < Ks) {
                 sr.patch_f = pfb + 2 * (size_t)h->npn * i;
@@ -3028,6 +3100,12 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (const char* e = std::getenv("IPPI_ADJ_OVERLAP")) h->adj_overlap = std::strcmp(e, "0") != 0;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;
+    // reference state of the FP32 storage: the free stream (u_in_x, 0)
+    h->ref.U = cfg->u_in_x;
+    h->ref.Uf = (float)cfg->u_in_x;
+    ref::feq(1.0, h->ref.U, 0.0, h->ref.c);
+    for (int k = 0; k < 9; k++) h->ref.cf[k] = (float)h->ref.c[k];
+    h->ref.bb = (float)(h->ref.c[7] - h->ref.c[5]);
     auto bail = [&](int code) {
         ippi_destroy(h);
         return code;
@@ -3066,10 +3144,10 @@ static int init_common(ippi_handle* h, int mode, uint64_t seed, double rho0, dou
     dim3 g((unsigned)((h->N + 255) / 256));
     if (h->esz == 8)
         hipLaunchKernelGGL(k_init<double>, g, dim3(256), 0, h->s, (double*)h->buf[0], h->mac, h->N, mode,
-                           (unsigned long long)seed, rho0, ux0, uy0, amp);
+                           (unsigned long long)seed, rho0, ux0, uy0, amp, h->ref);
     else
         hipLaunchKernelGGL(k_init<float>, g, dim3(256), 0, h->s, (float*)h->buf[0], h->mac, h->N, mode,
-                           (unsigned long long)seed, rho0, ux0, uy0, amp);
+                           (unsigned long long)seed, rho0, ux0, uy0, amp, h->ref);
     HIPC(h, hipGetLastError());
     HIPC(h, hipMemsetAsync(h->flag, 0, sizeof(int), h->s));
     h->cur = 0;
@@ -3149,18 +3227,18 @@ int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, f
             const double* src = (const double*)h->buf[h->cur];
             if (pb)
                 hipLaunchKernelGGL((k_restrict<double, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
-                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0);
+                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
             else
                 hipLaunchKernelGGL((k_restrict<double, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
-                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0);
+                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
         } else {
             const float* src = (const float*)h->buf[h->cur];
             if (pb)
                 hipLaunchKernelGGL((k_restrict<float, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
-                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0);
+                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
             else
                 hipLaunchKernelGGL((k_restrict<float, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
-                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0);
+                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
         }
         return hipGetLastError();
     };
@@ -3202,7 +3280,7 @@ int ippi_set_populations(ippi_handle* h, const double* f) {
     } else {
         std::vector<float> g(9 * h->N);
         for (size_t k = 0; k < 9; k++)
-            for (size_t n = 0; n < h->N; n++) g[k * h->N + n] = (float)(f[k * h->N + n] - WD[k]);
+            for (size_t n = 0; n < h->N; n++) g[k * h->N + n] = (float)(f[k * h->N + n] - h->ref.c[k]);
         HIPC(h, hipMemcpy(h->buf[0], g.data(), sizeof(float) * 9 * h->N, hipMemcpyHostToDevice));
     }
     HIPC(h, hipMemcpy(h->mac, m.data(), sizeof(double) * 3 * h->N, hipMemcpyHostToDevice));
@@ -3358,7 +3436,9 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
     HIPC(h, hipMemcpy(h->d_frage, fe.data(), sizeof(u32x4) * fe.size(), hipMemcpyHostToDevice));
     HIPC(h, hipMemcpy(h->d_blob, blob, sizeof(double) * MLP_BLOB, hipMemcpyHostToDevice));
     for (int i = 0; i < 6; i++) {
-        h->mlp_mu[i] = (float)blob[OFF_MU + i];
+        // the FP32 kernels pass rho - 1 and u - (U, 0) as the first three features
+        const double shift = i == 0 ? 1.0 : (i == 1 ? h->ref.U : 0.0);
+        h->mlp_mu[i] = (float)(blob[OFF_MU + i] - shift);
         h->mlp_isig[i] = (float)blob[OFF_IS + i];
         h->mlp_isig_e[i] = (float)(-L * blob[OFF_IS + i]);
     }

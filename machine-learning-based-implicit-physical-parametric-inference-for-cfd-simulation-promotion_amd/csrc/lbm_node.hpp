@@ -272,7 +272,18 @@ IPPI_HD double les_tau(const double* f, double rho, double ux, double uy, double
 }  // namespace ref
 
 // ------------------------------------------------------------------ FP32
-// g_k = f_k - w_k; rho = 1 + sum g; rho u = sum g e (sum w e = 0).
+// Stored: s_k = f_k - c_k with c = feq(1, (U, 0)), the equilibrium of the
+// free stream (U = the configured inflow velocity, 0 when there is none, in
+// which case c = w). In a flow that is mostly free stream the stored values
+// are small deviations, so their FP32 rounding is small in absolute terms:
+// the storage error that each step re-injects shrinks by the ratio of
+// |f - c| to |f - w|. Every FP32 formula works on deviations from that
+// reference state: with drho = sum s and dj = sum s e,
+//     rho = 1 + drho,   u = (U + dj_x, dj_y) / rho,
+//     du = u - (U, 0) = ((dj_x - U drho) / rho, dj_y / rho),
+// and the equilibrium, its moments and the non-equilibrium stress are
+// written as differences from those of c, whose terms are all small
+// (sum c = 1, sum c e = (U, 0), sum c e e = (U, 0)(U, 0) + I/3).
 //
 // Relaxation times are carried as tx = tau - 1/2, the quantity the
 // viscosity is proportional to (nu = tx / 3). Near tau = 1/2 (high
@@ -300,41 +311,28 @@ IPPI_HD float fsqrt(float x) {
 #endif
 }
 
-IPPI_HD void macro(const float* g, float& drho, float& rho, float& ux, float& uy) {
+// Equilibrium deviation q_k = feq_k(rho, u) - c_k for rho = 1 + drho and
+// u = (U + dux, duy). With a = e.(U, 0), b = e.du and
+// Q = |u|^2 - U^2 = dux (2 U + dux) + duy^2:
+//     q_k = c_k drho + w_k rho (b (3 + 4.5 (2 a + b)) - 1.5 Q),
+// every term small for a flow near the reference state. cf = c in FP32.
+IPPI_HD void geq_dev(float drho, float rho, float dux, float duy, float U, const float* cf, float* q) {
 #pragma clang fp contract(fast)
-    float a = (g[1] + g[3]) + (g[2] + g[4]);
-    float b = (g[5] + g[7]) + (g[6] + g[8]);
-    drho = g[0] + (a + b);
-    float jx = (g[1] - g[3]) + ((g[5] - g[7]) + (g[8] - g[6]));
-    float jy = (g[2] - g[4]) + ((g[5] - g[7]) + (g[6] - g[8]));
-    rho = 1.0f + drho;
-    if (rho < 1e-6f) {
-        ux = 0.f;
-        uy = 0.f;
-    } else {
-        float inv = rcp(rho);
-        ux = jx * inv;
-        uy = jy * inv;
-    }
-}
-
-// shifted equilibrium feq - w = w (drho + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u))
-IPPI_HD void geq(float drho, float rho, float ux, float uy, float* q) {
-#pragma clang fp contract(fast)
-    float uu = 1.5f * (ux * ux + uy * uy);
-    float s = ux + uy, d = ux - uy;
-    q[0] = WF[0] * (drho - rho * uu);
-    float ax = rho * (4.5f * ux * ux - uu), ay = rho * (4.5f * uy * uy - uu);
-    float as = rho * (4.5f * s * s - uu), ad = rho * (4.5f * d * d - uu);
-    float bx = 3.0f * rho * ux, by = 3.0f * rho * uy, bs = 3.0f * rho * s, bd = 3.0f * rho * d;
-    q[1] = WF[1] * (drho + ax + bx);
-    q[3] = WF[3] * (drho + ax - bx);
-    q[2] = WF[2] * (drho + ay + by);
-    q[4] = WF[4] * (drho + ay - by);
-    q[5] = WF[5] * (drho + as + bs);
-    q[7] = WF[7] * (drho + as - bs);
-    q[8] = WF[8] * (drho + ad + bd);
-    q[6] = WF[6] * (drho + ad - bd);
+    const float Q = dux * (2.0f * U + dux) + duy * duy;
+    const float hq = 1.5f * Q;
+    const float bs = dux + duy, bd = dux - duy;
+    auto one = [&](int k, float a, float b) {
+        q[k] = cf[k] * drho + (WF[k] * rho) * (b * (3.0f + 4.5f * (2.0f * a + b)) - hq);
+    };
+    one(0, 0.0f, 0.0f);
+    one(1, U, dux);
+    one(3, -U, -dux);
+    one(2, 0.0f, duy);
+    one(4, 0.0f, -duy);
+    one(5, U, bs);
+    one(7, -U, -bs);
+    one(8, U, bd);
+    one(6, -U, -bd);
 }
 
 // 2 - omega = 4 tx / (1 + 2 tx) for tau = 1/2 + tx: the hardware reciprocal

@@ -236,16 +236,17 @@ def test_full_size_translation_equivariance(ippi, precision):
         assert np.array_equal(np.roll(a, s, axis=1), b)
 
 
-def test_fp64_storage_rounding_diagnostic(ippi, monkeypatch):
+def test_fp64_storage_rounding_diagnostic(ippi, oracle, monkeypatch):
     """IPPI_FP64_ROUND_STORE=1 (read when the handle is created): the FP64
-    path stores every population as the FP32 path stores it, f - w rounded
-    to float. From the same start, one step of it is the plain FP64 step's
-    state so rounded, bit for bit, wherever the read-back (post-streaming)
-    populations are copies of stored ones: every column but the inlet and
-    outlet, whose NEE rule recomputes them. Later steps differ from plain
-    FP64 by about what FP32 storage costs, and a handle created without the
-    variable is unaffected."""
-    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)[:, None, None]
+    path stores every population as the FP32 path stores it, f - c rounded
+    to float, c = feq(1, (u_in, 0)) the free-stream equilibrium. From the
+    same start, one step of it is the plain FP64 step's state so rounded,
+    bit for bit, wherever the read-back (post-streaming) populations are
+    copies of stored ones: every column but the inlet and outlet, whose NEE
+    rule recomputes them. Later steps differ from plain FP64 by about what
+    FP32 storage costs, and a handle created without the variable is
+    unaffected."""
+    w = oracle.feq(1.0, 0.08, 0.0)[:, None, None]  # the offset c (Compute_feq's expression order)
     runs = {}
     for mode in ("plain", "round", "plain_after"):
         if mode == "round":
