@@ -7,8 +7,9 @@ FP32 accumulation, guard >= 0.501) at every node, on 4096 x 2048 nodes: one
 cylinder of diameter 160 lu (503 markers) at (1024, 1024), Re = 1e4
 (tau0 = 0.5 + 3 U D / Re = 0.50384), inflow U = 0.08 with seeded +-1e-3
 velocity noise, non-equilibrium-extrapolation inlet/outlet and free-slip
-walls, populations stored in FP32 as f - w. The network's weights are the
-seeded Xavier draw of workloads.xavier_blob(7). A "step" is one full time
+walls, populations stored in FP32 as deviations f - feq(1, (0.08, 0)) from
+the free-stream equilibrium. The network's weights are the seeded Xavier
+draw of workloads.xavier_blob(7) (W1, W2 rounded to bf16). A "step" is one full time
 step: one fused stream-collide kernel, which also computes the immersed
 boundary and the network.
 
