@@ -37,8 +37,8 @@ class DragCase:
     def solver(self, tau, center):
         """A fresh solver in the case's initial state."""
         s = Solver(self.nx, self.ny, "bgk", self.precision, "periodic_bb", tau_mode="field",
-                   tau0=float(np.mean(tau)), device=self.device,
-                   force_history=max(self.nsteps, 1))
+                   tau0=float(np.mean(tau)), u_in=(self.u0[0], 0.0), device=self.device,
+                   force_history=max(self.nsteps, 1))  # u_in: the FP32 storage's reference velocity
         s.set_tau_field(tau)
         s.init_fields(self.seed, 1.0, self.u0, self.noise)
         s.add_body(float(center[0]), float(center[1]), self.diameter, self.markers)

@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ippi_lbm.h"
@@ -2134,38 +2135,45 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
     const int x = bxi * BX + threadIdx.x;
     if (x >= nx) return;
     const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
-    A l[9];
+    // arithmetic in FP64 whatever the adjoint's storage: with FP32 arithmetic
+    // the rounding of each reverse step accumulates into the centre gradient,
+    // a sum that cancels by about 1e4 (C4: 1.7e-4 / 3.4e-4 off the FP64
+    // adjoint, against 6e-6 / 7e-6 with FP64 arithmetic over the same FP32
+    // storage; the kernel stays bound by its memory traffic)
+    typedef double C;
+    A la[9];
     S f[9], drho, rho, ux, uy;
-    pull_adj(lin, x, y, nx, ny, N, l);
+    pull_adj(lin, x, y, nx, ny, N, la);
     const A c0 = lin[0];  // constant-mode removal
+    C l[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) l[k] -= c0;
+    for (int k = 0; k < 9; k++) l[k] = (C)la[k] - (C)c0;
     adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, a.ref, f, drho, rho, ux, uy);
-    A fx = A(0), fy = A(0), lix = A(0), liy = A(0);
+    C fx = C(0), fy = C(0), lix = C(0), liy = C(0);
     const int bx0 = bxi * BX;
     const int r = a.nbody > 0 ? adj_patch(a, x, y, bx0, bx0 + BX - 1) : -1;
     const bool forced = r >= 0;
     if (forced) {
-        fx = (A)a.patch_f[2 * r];
-        fy = (A)a.patch_f[2 * r + 1];
-        lix = (A)a.patch_l[2 * r];
-        liy = (A)a.patch_l[2 * r + 1];
+        fx = (C)a.patch_f[2 * r];
+        fy = (C)a.patch_f[2 * r + 1];
+        lix = (C)a.patch_l[2 * r];
+        liy = (C)a.patch_l[2 * r + 1];
     }
-    const adj::NodeFwd<A> s = adj::node_fwd<A>((A)rho, (A)ux, (A)uy, fx, fy, adj_tx<S, A>(a, n));
-    const adj::WMom<A> m = adj::wmoments(l);
-    const adj::UsAdj<A> ua = adj::us_adjoint(m, s, forced);
+    const adj::NodeFwd<C> s = adj::node_fwd<C>((C)rho, (C)ux, (C)uy, fx, fy, adj_tx<S, C>(a, n));
+    const adj::WMom<C> m = adj::wmoments(l);
+    const adj::UsAdj<C> ua = adj::us_adjoint(m, s, forced);
     // non-equilibrium part in the representation of the stored populations
     S q[9];
     adj_eq(drho, rho, ua.usx, ua.usy, a.ref, q);
-    A dd[9], nu[9], dt;
+    C dd[9], nu[9], dt;
 #pragma unroll
-    for (int k = 0; k < 9; k++) dd[k] = (A)(f[k] - q[k]);
+    for (int k = 0; k < 9; k++) dd[k] = (C)(f[k] - q[k]);
     adj::node_adjoint(l, dd, m, ua, s, forced, lix, liy, nu, dt);
     if (a.store) {
 #pragma unroll
-        for (int k = 0; k < 9; k++) lout[k * N + n] = nu[k];
+        for (int k = 0; k < 9; k++) lout[k * N + n] = (A)nu[k];
     }
-    gacc[n] += dt;
+    gacc[n] += (A)dt;
 }
 
 // total += part; part = 0 (per-segment tau-gradient partial sums).
@@ -2827,8 +2835,10 @@ int macro(ippi_handle* h, double* f_out) {
 // Scratch of one adjoint run, freed when it ends.
 struct AdjBufs {
     std::vector<void*> ptrs;
-    ~AdjBufs() {
+    ~AdjBufs() { release(); }
+    void release() {
         for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
     }
     template <typename P>
     hipError_t alloc(P** p, size_t bytes) {
@@ -2965,6 +2975,7 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     // second set of states fits in 8 GB; otherwise, or with one segment, the
     // two phases alternate on the handle's stream. The results are the same
     // either way: the same kernels read the same inputs.
+    AdjBufs mem2;  // the second set (declared before the stream, so freed after it is synchronised)
     struct Overlap {
         hipStream_t s = nullptr;
         hipEvent_t rec[2] = {nullptr, nullptr}, rev[2] = {nullptr, nullptr};
@@ -2979,14 +2990,28 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
                 if (e) (void)hipEventDestroy(e);
         }
     } ov;
-    const bool overlap = h->adj_overlap && nseg > 1 && sizeof(T) * SZ * (K - 1) <= ((size_t)8 << 30);
+    // the second set of segment buffers: at most 8 GB, only while it leaves
+    // 1 GB of the device free, and without it (one stream) if allocating fails
+    const size_t ov_bytes = sizeof(T) * SZ * (K - 1) + sizeof(double) * 2 * ((size_t)h->npn + (size_t)nm) * K;
+    bool overlap = h->adj_overlap && nseg > 1 && ov_bytes <= ((size_t)8 << 30);
+    if (overlap) {
+        size_t fr = 0, tot = 0;
+        overlap = hipMemGetInfo(&fr, &tot) == hipSuccess && fr >= ov_bytes + ((size_t)1 << 30);
+    }
     T* segs[2] = {seg, seg};
     double* pfs[2] = {pf, pf};
     double* Fss[2] = {Fs, Fs};
+    if (overlap && (mem2.alloc(&segs[1], sizeof(T) * SZ * (K - 1)) != hipSuccess ||
+                    mem2.alloc(&pfs[1], sizeof(double) * 2 * (size_t)h->npn * K) != hipSuccess ||
+                    mem2.alloc(&Fss[1], sizeof(double) * 2 * (size_t)nm * K) != hipSuccess)) {
+        (void)hipGetLastError();  // clear the allocation error; run on one stream
+        mem2.release();
+        segs[1] = seg;
+        pfs[1] = pf;
+        Fss[1] = Fs;
+        overlap = false;
+    }
     if (overlap) {
-        HIPC(h, mem.alloc(&segs[1], sizeof(T) * SZ * (K - 1)));
-        HIPC(h, mem.alloc(&pfs[1], sizeof(double) * 2 * (size_t)h->npn * K));
-        HIPC(h, mem.alloc(&Fss[1], sizeof(double) * 2 * (size_t)nm * K));
         HIPC(h, hipMemsetAsync(pfs[1], 0, sizeof(double) * 2 * (size_t)h->npn * K, h->s));  // inactive patch nodes
         HIPC(h, hipStreamCreateWithFlags(&ov.s, hipStreamNonBlocking));
         for (int b = 0; b < 2; b++) {
