@@ -46,7 +46,8 @@ Reported beside it:
                                    D ~ U[40, 120] (seeded placement), FP32 vs
                                    FP64, 5,000 steps, per-body drag error;
                     flatplate_2401x501  the reference's own run (flat plate,
-                                   BGK + LES, Re = 5e5);
+                                   BGK + LES, Re = 5e5), also with its
+                                   boundary-row / column networks set;
                     tausearch      FindAndWriteTau's candidate search.
 
 `python bench.py --copy-probe MB` runs only a plain device copy of MB
@@ -419,17 +420,28 @@ def leg_flatplate(ippi, steps=5000, marks=(1000, 5000)):
     nx, ny, u = 2401, 501, 0.1
     tau = 0.5 + 6.0 * (ny * u) / 5e5
     out = {"workload": f"flat plate 2401x501 (the reference's main.cpp run), BGK+LES(Cs=0.173), U=0.1, Re=5e5, "
-                       f"tau={tau:.7f}, no-slip from x=400", "steps": steps}
+                       f"tau={tau:.7f}, no-slip from x=400; with and without the boundary-row / column "
+                       "networks (types 1, 3, 5, 7; seeded weights)", "steps": steps}
 
     def make(prec, noise):
         s = ippi.Solver(nx, ny, "bgk", prec, "flatplate", les=True, tau0=tau, cs=0.173, u_in=(u, 0.0), plate_x=400)
         s.init_fields(1, 1.0, (u, 0.0), noise)
         return s
+    from ippi_amd import workloads
     for prec in ("fp32", "fp64"):
         s = make(prec, 0.0)
         log(f"flat plate {prec}: {steps} steps")
         dt = timed_run(s, steps, 20)
         out[f"mlups_{prec}"] = round(nx * ny * steps / dt / 1e6, 1)
+        s.close()
+        # the reference's boundary-row / column networks (types 1, 3, 5, 7) at its 5,800 edge nodes
+        s = make(prec, 0.0)
+        for t in (1, 3, 5, 7):
+            s.set_boundary_mlp(t, workloads.boundary_blob(t))
+        log(f"flat plate {prec} with the boundary networks: {steps // 5} steps")
+        dtb = timed_run(s, steps // 5, 20)
+        out[f"mlups_{prec}_boundary_networks"] = round(nx * ny * (steps // 5) / dtb / 1e6, 1)
+        out[f"boundary_networks_us_per_step_{prec}"] = round((dtb / (steps // 5) - dt / steps) * 1e6, 2)
         s.close()
     out["speedup_vs_fp64"] = round(out["mlups_fp32"] / out["mlups_fp64"], 3)
     runs = {"fp32": make("fp32", 0.0), "fp64": make("fp64", 0.0), "fp64_noise1e-9": make("fp64", 1e-9)}

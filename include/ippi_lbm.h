@@ -24,6 +24,7 @@
  *   ippi_add_body      Mesh::InitialIBParticle       mesh.h:57, mesh.cpp:173-191
  *   ippi_set_tau_field (writes of LatMesh[i][j].tau / TauAtBeginning, mesh.cpp:90-91)
  *   ippi_set_mlp       NNmodel_Type00..07 model load mesh.cpp:2767-3260
+ *   ippi_set_boundary_mlp  NNmodel_Type01/03/05/07    mesh.cpp:2767-2898, 3128-3260
  *   ippi_step          Mesh::OneLBMStep              mesh.h:60, mesh.cpp:357-513
  *                      (+ InterpolateVel / CalParticleForce / SpreadForce /
  *                       UpdateVelocity, mesh.cpp:514-832,1244-1281)
@@ -127,6 +128,23 @@ int ippi_set_tau_field(ippi_handle* h, const double* tau);
 /* Relaxation-time MLP 6 -> 32 -> 32 -> 1 (IPPI_TAU_NN); blob layout:
  * W1[32][6] b1[32] W2[32][32] b2[32] W3[32] b3 mu[6] isig[6] (n = 1325). */
 int ippi_set_mlp(ippi_handle* h, const double* blob, int n);
+
+/* Boundary-row / column relaxation-time networks, the reference's
+ * Mesh::NNmodel_Type01 / 03 / 05 / 07 (mesh.cpp:3128-3194, 2767-2832,
+ * 3195-3260, 2833-2898): type 1 at column 0 (1 <= y <= ny-2), 3 at row
+ * ny-2, 5 at column nx-1, 7 at row 1 (every x). Each step, before the
+ * collision, a node of a set type reads (u.x, u.y, rho) of five neighbours
+ * in the reference's order as float, evaluates Linear(15,128) -> 6 x
+ * Linear(128,128) -> Linear(128,1) in FP32 with ReLU between layers, and
+ * takes the output as its tau where it is >= 0.501 (mesh.cpp:2821-2827);
+ * otherwise the node keeps its LES / network / base tau. Where a row and a
+ * column meet the row's network decides (the reference calls it last).
+ * Neighbours beyond x = 0 and nx-1 are taken periodically (the reference's
+ * ghost columns). Blob: W0[128][15] b0[128], six times W[128][128] b[128],
+ * W7[128] b7 (n = 101249, PyTorch's nn.Linear layout). The reference loads
+ * TorchScript files per call (AutoT0x.pt); their activation is not stated
+ * in its source. */
+int ippi_set_boundary_mlp(ippi_handle* h, int type, const double* blob, int n);
 
 /* Advance nsteps time steps (asynchronous). */
 int ippi_step(ippi_handle* h, int nsteps);

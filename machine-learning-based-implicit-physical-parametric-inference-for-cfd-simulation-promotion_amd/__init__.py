@@ -91,6 +91,7 @@ _SIGS = {
     "ippi_pattern_probe": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D],
                            ctypes.c_int),
     "ippi_get_stencil": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _D, _D, _D], ctypes.c_int),
+    "ippi_set_boundary_mlp": ([ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int], ctypes.c_int),
     "ippi_tau_search": ([ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                          _D, _D], ctypes.c_int),
     "ippi_tau_search_noalloc": ([ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_int,
@@ -212,6 +213,13 @@ class Solver:
     def set_mlp(self, blob):
         b = _f64(blob, MLP_BLOB)
         self._check(library().ippi_set_mlp(self.h, _p(b), b.size))
+
+    def set_boundary_mlp(self, net_type, blob):
+        """The reference's boundary-row / column network of type 1, 3, 5 or
+        7 (NNmodel_Type01/03/05/07): 101,249 parameters, W0[128][15] b0[128],
+        6 x (W[128][128] b[128]), W7[128] b7 (nn.Linear layout)."""
+        b = _f64(blob, BNN_NPARAM)
+        self._check(library().ippi_set_boundary_mlp(self.h, int(net_type), _p(b), b.size))
 
     def step(self, n=1):
         self._check(library().ippi_step(self.h, int(n)))
@@ -408,6 +416,7 @@ def pattern_probe(nx, ny, precision="fp32", mode="pull", reps=200, device=0):
 
 
 TAU_SEARCH_NOALLOC_MAX = 32768
+BNN_NPARAM = 15 * 128 + 128 + 6 * (128 * 128 + 128) + 128 + 1  # 101249
 
 
 def tau_search(fnew, feq, tau, target, m_lo=-10000, m_hi=10000, step=1e-5, device=0, noalloc=False):

@@ -113,10 +113,29 @@ struct StepArgs {
     unsigned ib_seq;        // this step's sequence number
     unsigned ib_wait;       // the value the band waits for (ib_seq; ib_seq + 1 only to test the time-out)
     int band0, band1;
+    // boundary-row / column networks (the reference's NNmodel_Type01/03/05/07):
+    // accepted tau of this step (0: rejected) at row 1 [nx], row ny-2 [nx],
+    // column 0 [ny], column nx-1 [ny]; bnn_on: bit t set for type 2t+1
+    const double* bnn;
+    int bnn_on;
     int rot;                // lattice row of grid row 1 (the rows follow cyclically)
     int xcd;                // XCD-grouped block order (xcd_block)
     Patch patch[IPPI_MAXB];
 };
+
+// Accepted boundary-network tau of node (x, y) (0: none): the row networks
+// (T07 at row 1, T03 at row ny-2) decide where they meet a column network,
+// as their calls come after T01 and T05 in the reference's step.
+__device__ __forceinline__ double bnn_tau_at(const StepArgs& a, int x, int y) {
+    const int nx = a.nx, ny = a.ny;
+    if ((a.bnn_on & 8) && y == 1) return a.bnn[x];
+    if ((a.bnn_on & 2) && y == ny - 2) return a.bnn[nx + x];
+    if (y >= 1 && y <= ny - 2) {
+        if ((a.bnn_on & 1) && x == 0) return a.bnn[2 * nx + y];
+        if ((a.bnn_on & 4) && x == nx - 1) return a.bnn[2 * nx + ny + y];
+    }
+    return 0.0;
+}
 
 // ----------------------------------------------------------------- streaming
 // Loads use a wave-uniform base pointer (one population array, in SGPRs) and
@@ -643,6 +662,13 @@ __device__ __forceinline__ void node_f64(double* f, int x, int y, unsigned N, co
         nn_ok = t >= 0.501 && t <= a.nn_max;
         if (nn_ok) tau = t;
     }
+    if (a.bnn_on) {  // a boundary network's accepted value (NNflag, mesh.cpp:393)
+        const double tbn = bnn_tau_at(a, x, y);
+        if (tbn > 0.0) {
+            tau = tbn;
+            nn_ok = true;
+        }
+    }
     if (COLL == IPPI_COLL_MRT) {
         if (a.les && !nn_ok) tau = ref::les_tau_q(f, q, rho, tb, a.cs);
         ref::mrt(f, rho, ux, uy, tau, o);
@@ -783,6 +809,13 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         const float t = mlp_tau_nn<NN>(feat, a.mlp, lds, lf);  // tau - 1/2
         nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
         if (nn_ok) tx = t;
+    }
+    if (a.bnn_on) {  // a boundary network's accepted value (NNflag, mesh.cpp:393)
+        const double tbn = bnn_tau_at(a, x, y);
+        if (tbn > 0.0) {
+            tx = (float)(tbn - 0.5);
+            nn_ok = true;
+        }
     }
     if (a.les && !nn_ok) tx = sh::les_tx(nxx, nxy, nyy, rho, txb, a.csf);
     const float dl = sh::dl_of(tx);  // 2 - omega
@@ -1014,6 +1047,13 @@ __global__ void __launch_bounds__(BX) k_tau32(const float* __restrict__ src, dou
         nn_ok = t >= 0.001f && t <= a.nn_maxtxf;
         if (nn_ok) tx = t;
     }
+    if (a.bnn_on) {
+        const double tbn = bnn_tau_at(a, x, y);
+        if (tbn > 0.0) {
+            tx = (float)(tbn - 0.5);
+            nn_ok = true;
+        }
+    }
     if (a.les && !nn_ok) tx = sh::les_tx(nxx, nxy, nyy, o.rho, txb, a.csf);
     if (active) out[n] = 0.5 + (double)tx;
 }
@@ -1049,6 +1089,13 @@ __global__ void __launch_bounds__(BX) k_tau64(const double* __restrict__ src, do
         const double t = nn_tau_f64(f, q, rho, ux, uy, a.mlp.blob);
         nn_ok = t >= 0.501 && t <= a.nn_max;
         if (nn_ok) tau = t;
+    }
+    if (a.bnn_on) {
+        const double tbn = bnn_tau_at(a, x, y);
+        if (tbn > 0.0) {
+            tau = tbn;
+            nn_ok = true;
+        }
     }
     if (a.les && !nn_ok) tau = ref::les_tau_q(f, q, rho, tb, a.cs);
     out[n] = tau;
@@ -1253,27 +1300,25 @@ __global__ void k_corner(const T* __restrict__ src, double* __restrict__ corner,
     corner[t] = v;
 }
 
+// Populations (full f, FP64) and macros rho, u of node (x, y) of the
+// current post-streaming state, as the read-back reports them: FP32
+// deviations are accumulated in FP64. The flat-plate corners keep their
+// initial populations and the macros in mac.
 template <typename T, int BC>
-__global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, double* __restrict__ f_out, int nx,
-                        int ny, double uinx, double uiny, int plate_x, const double* __restrict__ corner,
-                        double* __restrict__ mean, double kk, RefState ref) {
+__device__ __forceinline__ void macro_at(const T* __restrict__ src, const double* __restrict__ mac,
+                                         const double* __restrict__ corner, int x, int y, int nx, int ny,
+                                         double uinx, double uiny, int plate_x, const RefState& ref, double* f,
+                                         double& rho, double& ux, double& uy) {
 #pragma clang fp contract(off)
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= nx) return;
     const unsigned N = (unsigned)nx * ny;
     const unsigned n = (unsigned)y * nx + x;
-    double f[9], rho, ux, uy;
     if constexpr (BC == IPPI_BC_FLATPLATE) {
         if (y == 0 && (x == 0 || x == nx - 1)) {  // corner: state of the initialisation, macros kept in mac
-            if (f_out) {
 #pragma unroll
-                for (int k = 0; k < 9; k++) f_out[(size_t)k * N + n] = corner[9 * (x != 0) + k];
-            }
-            if (mean) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) mean[c * N + n] = (mean[c * N + n] * kk + mac[c * N + n]) / (kk + 1.0);
-            }
+            for (int k = 0; k < 9; k++) f[k] = corner[9 * (x != 0) + k];
+            rho = mac[n];
+            ux = mac[N + n];
+            uy = mac[2 * N + n];
             return;
         }
         if constexpr (sizeof(T) == 8) {
@@ -1315,6 +1360,20 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
 #pragma unroll
         for (int k = 0; k < 9; k++) f[k] = gd[k] + ref.c[k];
     }
+}
+
+template <typename T, int BC>
+__global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, double* __restrict__ f_out, int nx,
+                        int ny, double uinx, double uiny, int plate_x, const double* __restrict__ corner,
+                        double* __restrict__ mean, double kk, RefState ref) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= nx) return;
+    const unsigned N = (unsigned)nx * ny;
+    const unsigned n = (unsigned)y * nx + x;
+    double f[9], rho, ux, uy;
+    macro_at<T, BC>(src, mac, corner, x, y, nx, ny, uinx, uiny, plate_x, ref, f, rho, ux, uy);
     mac[n] = rho;
     mac[N + n] = ux;
     mac[2 * N + n] = uy;
@@ -1326,6 +1385,137 @@ __global__ void k_macro(const T* __restrict__ src, double* __restrict__ mac, dou
     if (f_out) {
 #pragma unroll
         for (int k = 0; k < 9; k++) f_out[(size_t)k * N + n] = f[k];
+    }
+}
+
+// ------------------------------------------- boundary-row / column networks
+// The reference's per-node networks at the domain's edge rows and columns
+// (Mesh::NNmodel_Type01/03/05/07, mesh.cpp:2767-2898, 3128-3260): inputs
+// (u.x, u.y, rho) of five neighbours as float, Linear(15,128) -> 6 x
+// Linear(128,128) -> Linear(128,1) in FP32, ReLU between layers; the node
+// takes the output as its tau where it is >= 0.501 (mesh.cpp:2821-2827).
+// One block of BNN_T threads serves BNN_NODES nodes: thread t computes
+// neuron t % 128 for the nodes of half t / 128. Each layer's weights are
+// staged through LDS in chunks of BNN_KC inputs with 16-byte loads, so a
+// block reads them once and many loads are in flight together; every
+// output is a chain of fused multiply-adds over the inputs in increasing
+// order, so the result is that of a sequential FP32 loop bit for bit.
+// Weights are stored input-major (Wt[k][o]).
+constexpr int BNN_NIN = 15, BNN_NH = 128, BNN_NHID = 6;
+constexpr int BNN_NPARAM = BNN_NIN * BNN_NH + BNN_NH + BNN_NHID * (BNN_NH * BNN_NH + BNN_NH) + BNN_NH + 1;  // 101249
+constexpr int BNN_T = 2 * BNN_NH, BNN_NODES = 16, BNN_HALF = BNN_NODES / 2, BNN_KC = 64;
+
+struct BnnNode {
+    int x, y, slot, t;  // node, index into the tau arrays, type index (0: T01, 1: T03, 2: T05, 3: T07)
+};
+struct BnnDev {
+    const BnnNode* nodes;
+    int nnodes;
+    const float* w[4];  // per type, device layout (input-major), nullptr when not set
+    double* tau;        // [2 nx + 2 ny]
+};
+
+// neighbour offsets in the reference's input order (mesh.cpp:3149-3165,
+// 2788-2804, 3216-3232, 2854-2870)
+__constant__ int BNN_DX[4][5] = {{0, 1, 1, 1, 0}, {1, 1, 0, -1, -1}, {0, -1, -1, -1, 0}, {-1, -1, 0, 1, 1}};
+__constant__ int BNN_DY[4][5] = {{1, 1, 0, -1, -1}, {0, -1, -1, -1, 0}, {1, 1, 0, -1, -1}, {0, 1, 1, 1, 0}};
+
+// LDS copy of rows [k0, k0 + nk) of an input-major weight matrix (nk x 128)
+__device__ __forceinline__ void bnn_stage(const float* __restrict__ Wt, int k0, int nk, float* lw) {
+    const float4* src = reinterpret_cast<const float4*>(Wt + (size_t)k0 * BNN_NH);
+    float4* dst = reinterpret_cast<float4*>(lw);
+    const int n4 = nk * BNN_NH / 4;
+    for (int i = threadIdx.x; i < n4; i += BNN_T) dst[i] = src[i];
+}
+
+template <typename T, int BC>
+__global__ void __launch_bounds__(BNN_T) k_bnn(const T* __restrict__ src, const double* __restrict__ mac,
+                                              const double* __restrict__ corner, int pull, StepArgs a, BnnDev d) {
+    __shared__ __attribute__((aligned(16))) float lw[BNN_KC * BNN_NH];  // staged weights
+    __shared__ __attribute__((aligned(16))) float act[2][BNN_NH][BNN_NODES];  // [k][node]
+    __shared__ __attribute__((aligned(16))) float xin[BNN_NIN][BNN_NODES];
+    __shared__ int typ[BNN_NODES];
+    const int t = threadIdx.x, o = t % BNN_NH, h0 = (t / BNN_NH) * BNN_HALF;
+    const int i0 = blockIdx.x * BNN_NODES;
+    const int nx = a.nx, ny = a.ny;
+    if (t < BNN_NODES) typ[t] = d.nodes[min(i0 + t, d.nnodes - 1)].t;
+    // inputs: (u.x, u.y, rho) of the 5 neighbours of each node, as float
+    if (t < 5 * BNN_NODES) {
+        const int j = t / BNN_NODES, q = t % BNN_NODES;
+        const BnnNode b = d.nodes[min(i0 + q, d.nnodes - 1)];
+        int xn = b.x + BNN_DX[b.t][j], yn = b.y + BNN_DY[b.t][j];
+        xn = xn < 0 ? xn + nx : (xn >= nx ? xn - nx : xn);  // the ring's ghost columns
+        double f[9], rho, ux, uy;
+        if (pull) {
+            macro_at<T, BC>(src, mac, corner, xn, yn, nx, ny, a.uinx, a.uiny, a.plate_x, a.ref, f, rho, ux, uy);
+        } else {  // first step: the macros of the initial state
+            const unsigned N = (unsigned)nx * ny, n = (unsigned)yn * nx + xn;
+            rho = mac[n];
+            ux = mac[N + n];
+            uy = mac[2 * N + n];
+        }
+        xin[3 * j][q] = (float)ux;
+        xin[3 * j + 1][q] = (float)uy;
+        xin[3 * j + 2][q] = (float)rho;
+    }
+    __syncthreads();
+    // a block's nodes may be of two types: each type's weights are applied
+    // to all of them and kept for that type's nodes
+    for (int tp = 0; tp < 4; tp++) {
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < BNN_NODES; q++) any = any || typ[q] == tp;
+        if (!any || !d.w[tp]) continue;  // block-uniform
+        const float* W = d.w[tp];
+        float v[BNN_HALF];
+        // layer 0: 15 inputs
+        __syncthreads();  // the previous pass's readers of lw and act are done
+        bnn_stage(W, 0, BNN_NIN, lw);
+        __syncthreads();
+        {
+            const float bo = W[BNN_NIN * BNN_NH + o];
+#pragma unroll
+            for (int q = 0; q < BNN_HALF; q++) v[q] = bo;
+#pragma unroll
+            for (int k = 0; k < BNN_NIN; k++) {
+                const float w = lw[k * BNN_NH + o];
+#pragma unroll
+                for (int q = 0; q < BNN_HALF; q++) v[q] = __builtin_fmaf(w, xin[k][h0 + q], v[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BNN_HALF; q++) act[0][o][h0 + q] = v[q] > 0.0f ? v[q] : 0.0f;
+        size_t off = BNN_NIN * BNN_NH + BNN_NH;
+        int cur = 0;
+        for (int l = 0; l < BNN_NHID; l++) {
+            const float* Wl = W + off;
+            const float bo = Wl[BNN_NH * BNN_NH + o];
+#pragma unroll
+            for (int q = 0; q < BNN_HALF; q++) v[q] = bo;
+            for (int k0 = 0; k0 < BNN_NH; k0 += BNN_KC) {
+                __syncthreads();  // lw free, and act[cur] written
+                bnn_stage(Wl, k0, BNN_KC, lw);
+                __syncthreads();
+#pragma unroll 8
+                for (int k = 0; k < BNN_KC; k++) {
+                    const float w = lw[k * BNN_NH + o];
+#pragma unroll
+                    for (int q = 0; q < BNN_HALF; q++) v[q] = __builtin_fmaf(w, act[cur][k0 + k][h0 + q], v[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < BNN_HALF; q++) act[cur ^ 1][o][h0 + q] = v[q] > 0.0f ? v[q] : 0.0f;
+            cur ^= 1;
+            off += BNN_NH * BNN_NH + BNN_NH;
+        }
+        __syncthreads();
+        if (t < BNN_NODES && typ[t] == tp && i0 + t < d.nnodes) {
+            const float* Wl = W + off;
+            float r = Wl[BNN_NH];
+            for (int k = 0; k < BNN_NH; k++) r = __builtin_fmaf(Wl[k], act[cur][k][t], r);
+            const double tau = (double)r;
+            d.tau[d.nodes[i0 + t].slot] = tau >= 0.501 ? tau : 0.0;
+        }
     }
 }
 
@@ -2251,6 +2441,12 @@ struct ippi_handle {
     double* d_blob = nullptr;
     float mlp_mu[6] = {0}, mlp_isig[6] = {0}, mlp_isig_e[6] = {0}, mlp_b3 = 0.f;
     bool mlp_exact = false;  // W1 and W2 hold bf16 values only (the FP32 path's exact-weight network)
+    // boundary-row / column networks (types 1, 3, 5, 7 -> index 0..3)
+    float* d_bnn_w[4] = {nullptr, nullptr, nullptr, nullptr};
+    BnnNode* d_bnn_nodes = nullptr;
+    int bnn_nnodes = 0;
+    double* d_bnn_tau = nullptr;  // [2 nx + 2 ny]
+    int bnn_on = 0;
     // optional per-launch timing (HIP events on this handle's stream)
     int timing = 0;                      // 1: events around each launch; 2: around each ippi_step call
     int64_t batch_steps = 0, batch_ib = 0;  // kernels launched inside the timed calls (mode 2)
@@ -2405,6 +2601,8 @@ StepArgs make_args(ippi_handle* h) {
     a.mlp.b3 = h->mlp_b3;
     a.plate_x = h->c.plate_x;
     a.corner = h->corner;
+    a.bnn = h->d_bnn_tau;
+    a.bnn_on = h->bnn_on;
     a.mac = h->mac;
     for (size_t b = 0; b < h->bodies.size(); b++) a.patch[b] = h->bodies[b].p;
     return a;
@@ -2665,6 +2863,25 @@ struct CallTimer {
     }
 };
 
+// Boundary-network tau of the state buf[cur] holds (before the step kernel
+// that reads it, or before a tau read-back).
+template <typename T>
+int launch_bnn(ippi_handle* h, const StepArgs& a, bool pull) {
+    BnnDev d{h->d_bnn_nodes, h->bnn_nnodes, {h->d_bnn_w[0], h->d_bnn_w[1], h->d_bnn_w[2], h->d_bnn_w[3]},
+             h->d_bnn_tau};
+    const dim3 g((unsigned)((h->bnn_nnodes + BNN_NODES - 1) / BNN_NODES));
+    const T* src = (const T*)h->buf[h->cur];
+    const int p = pull ? 1 : 0;
+    if (h->c.bc == IPPI_BC_PERIODIC_BB)
+        hipLaunchKernelGGL((k_bnn<T, IPPI_BC_PERIODIC_BB>), g, dim3(BNN_T), 0, h->s, src, h->mac, h->corner, p, a, d);
+    else if (h->c.bc == IPPI_BC_INFLOW)
+        hipLaunchKernelGGL((k_bnn<T, IPPI_BC_INFLOW>), g, dim3(BNN_T), 0, h->s, src, h->mac, h->corner, p, a, d);
+    else
+        hipLaunchKernelGGL((k_bnn<T, IPPI_BC_FLATPLATE>), g, dim3(BNN_T), 0, h->s, src, h->mac, h->corner, p, a, d);
+    HIPC(h, hipGetLastError());
+    return IPPI_OK;
+}
+
 template <typename T>
 int run_steps(ippi_handle* h, int nsteps) {
     CallTimer call_timer(h);
@@ -2703,6 +2920,10 @@ int run_steps(ippi_handle* h, int nsteps) {
         }
         const int nk = nn_kind(h);
         const bool nn = nk != 0;
+        if (h->bnn_on) {
+            int r = launch_bnn<T>(h, a, pull);
+            if (r) return r;
+        }
         // IB inside the step kernel (pull steps; FP64 without the network)
         if (ib && pull && h->ink_ok && h->ink_enabled && (sizeof(T) == 4 || !nn)) {
             const int nb = (int)h->bodies.size();
@@ -3159,6 +3380,9 @@ int ippi_destroy(ippi_handle* h) {
     free_ib(h);
     dfree(h->d_frag);
     dfree(h->d_frage);
+    for (auto& w : h->d_bnn_w) dfree(w);
+    dfree(h->d_bnn_nodes);
+    dfree(h->d_bnn_tau);
     dfree(h->d_blob);
     if (h->s) (void)hipStreamDestroy(h->s);
     delete h;
@@ -3473,6 +3697,50 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
     return IPPI_OK;
 }
 
+int ippi_set_boundary_mlp(ippi_handle* h, int type, const double* blob, int n) {
+    if (!h || !blob) return IPPI_ERR_ARG;
+    const int t = type == 1 ? 0 : type == 3 ? 1 : type == 5 ? 2 : type == 7 ? 3 : -1;
+    if (t < 0) return fail(h, IPPI_ERR_ARG, "boundary network type must be 1, 3, 5 or 7");
+    if (n != BNN_NPARAM) return fail(h, IPPI_ERR_ARG, "boundary network blob must hold 101249 values");
+    if (h->c.nx < 3 || h->c.ny < 4) return fail(h, IPPI_ERR_ARG, "lattice too small for the boundary networks");
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(blob[i])) return fail(h, IPPI_ERR_ARG, "boundary network blob holds a non-finite value");
+    // device layout: each weight matrix input-major (W[o][k] -> Wt[k][o]), biases as given
+    std::vector<float> w(BNN_NPARAM);
+    size_t off = 0;
+    auto layer = [&](int nin, int nout) {
+        for (int o = 0; o < nout; o++)
+            for (int k = 0; k < nin; k++) w[off + (size_t)k * nout + o] = (float)blob[off + (size_t)o * nin + k];
+        off += (size_t)nin * nout;
+        for (int o = 0; o < nout; o++) w[off + o] = (float)blob[off + o];
+        off += nout;
+    };
+    layer(BNN_NIN, BNN_NH);
+    for (int l = 0; l < BNN_NHID; l++) layer(BNN_NH, BNN_NH);
+    for (int k = 0; k <= BNN_NH; k++) w[off + k] = (float)blob[off + k];  // output row and bias
+    HIPC(h, hipStreamSynchronize(h->s));
+    if (!h->d_bnn_w[t]) HIPC(h, hipMalloc(&h->d_bnn_w[t], sizeof(float) * BNN_NPARAM));
+    HIPC(h, hipMemcpy(h->d_bnn_w[t], w.data(), sizeof(float) * BNN_NPARAM, hipMemcpyHostToDevice));
+    h->bnn_on |= 1 << t;
+    // node list of every type set, grouped by type
+    const int nx = h->c.nx, ny = h->c.ny;
+    std::vector<BnnNode> nodes;
+    for (int tt = 0; tt < 4; tt++) {
+        if (!(h->bnn_on & (1 << tt))) continue;
+        if (tt == 0 || tt == 2)
+            for (int y = 1; y <= ny - 2; y++) nodes.push_back(BnnNode{tt == 0 ? 0 : nx - 1, y, 2 * nx + (tt == 2 ? ny : 0) + y, tt});
+        else
+            for (int x = 0; x < nx; x++) nodes.push_back(BnnNode{x, tt == 3 ? 1 : ny - 2, (tt == 1 ? nx : 0) + x, tt});
+    }
+    dfree(h->d_bnn_nodes);
+    HIPC(h, hipMalloc(&h->d_bnn_nodes, sizeof(BnnNode) * nodes.size()));
+    HIPC(h, hipMemcpy(h->d_bnn_nodes, nodes.data(), sizeof(BnnNode) * nodes.size(), hipMemcpyHostToDevice));
+    h->bnn_nnodes = (int)nodes.size();
+    if (!h->d_bnn_tau) HIPC(h, hipMalloc(&h->d_bnn_tau, sizeof(double) * (2 * (size_t)nx + 2 * (size_t)ny)));
+    HIPC(h, hipMemset(h->d_bnn_tau, 0, sizeof(double) * (2 * (size_t)nx + 2 * (size_t)ny)));
+    return IPPI_OK;
+}
+
 int ippi_step(ippi_handle* h, int nsteps) {
     if (!h || nsteps < 0) return IPPI_ERR_ARG;
     if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
@@ -3585,7 +3853,7 @@ int ippi_adjoint_drag(ippi_handle* h, int nsteps, int avg_from, int ckpt, int fl
                       double* J, double* g_tau, double* g_center) {
     if (!h || nsteps < 1 || avg_from < 1 || avg_from > nsteps || ckpt < 1 || (flags & ~IPPI_ADJ_FP64))
         return IPPI_ERR_ARG;
-    if (h->c.collision != IPPI_COLL_BGK || h->c.bc != IPPI_BC_PERIODIC_BB || h->c.les || nn_kind(h))
+    if (h->c.collision != IPPI_COLL_BGK || h->c.bc != IPPI_BC_PERIODIC_BB || h->c.les || nn_kind(h) || h->bnn_on)
         return fail(h, IPPI_ERR_ARG, "the adjoint covers BGK, periodic x / bounce-back walls, no LES, "
                                      "constant or per-node tau");
     if (h->bodies.empty()) return fail(h, IPPI_ERR_STATE, "the drag objective needs an immersed body");
@@ -3692,6 +3960,10 @@ int ippi_get_tau(ippi_handle* h, double* tau) {
     if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
     if (nn_kind(h) && !h->has_mlp) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
     StepArgs a = make_args(h);
+    if (h->bnn_on) {
+        int r = h->esz == 8 ? launch_bnn<double>(h, a, true) : launch_bnn<float>(h, a, true);
+        if (r) return r;
+    }
     double* d_out = nullptr;
     HIPC(h, hipMalloc(&d_out, sizeof(double) * h->N));
     dim3 grid((h->c.nx + BX - 1) / BX, h->c.ny);

@@ -55,6 +55,28 @@ def xavier_blob(seed, b3=-5.0, mu=(1.0, 0.05, 0.0, 2e-3, 0.0, 0.0), sig=(2e-3, 0
     return p
 
 
+BNN_PARAMS = 15 * 128 + 128 + 6 * (128 * 128 + 128) + 128 + 1
+
+
+def boundary_blob(seed, out_bias=0.55, out_scale=0.02):
+    """Seeded boundary-row / column network (the reference's AutoT0x shape:
+    Linear(15,128), 6 x Linear(128,128), Linear(128,1)), Xavier-uniform
+    hidden weights; the output sits near out_bias, above the 0.501 guard."""
+    rng = np.random.default_rng(seed)
+    p = []
+
+    def lin(nin, nout):
+        a = math.sqrt(6.0 / (nin + nout))
+        p.append(rng.uniform(-a, a, (nout, nin)).ravel())
+        p.append(rng.uniform(-0.1, 0.1, nout))
+    lin(15, 128)
+    for _ in range(6):
+        lin(128, 128)
+    p.append(rng.uniform(-1, 1, 128) * out_scale)
+    p.append([out_bias])
+    return np.concatenate(p)
+
+
 def markers_for(diameter):
     return int(round(math.pi * diameter))
 

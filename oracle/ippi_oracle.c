@@ -386,6 +386,8 @@ struct or_sim {
     int nrec, cap_rec;
     double mlp[OR_MLP_NPARAM + 2 * OR_MLP_NIN];
     int has_mlp;
+    float* bnn[4];     /* boundary networks T01, T03, T05, T07 (FP32 parameters), NULL = not set */
+    double* bnn_tau;   /* per node: accepted boundary-network tau of this step, 0 = none */
 };
 
 static int g_threads = 0;
@@ -414,6 +416,8 @@ void or_destroy(or_sim* s) {
     free(s->fnew); free(s->fcol); free(s->rho); free(s->ux); free(s->uy);
     free(s->tau); free(s->tau_base); free(s->Fx); free(s->Fy);
     free(s->mk); free(s->forces);
+    for (int t = 0; t < 4; t++) free(s->bnn[t]);
+    free(s->bnn_tau);
     free(s);
 }
 
@@ -474,6 +478,94 @@ void or_set_tau_field(or_sim* s, const double* tau) {
 void or_set_mlp(or_sim* s, const double* params) {
     memcpy(s->mlp, params, sizeof(s->mlp));
     s->has_mlp = 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Boundary-row and -column networks: Mesh::NNmodel_Type01/03/05/07     */
+/* (mesh.cpp:3128-3194, 2767-2832, 3195-3260, 2833-2898). Each reads     */
+/* (u.x, u.y, rho) of five neighbours as float (mesh.cpp:2806-2808),     */
+/* runs Linear(15,128) -> 6 x Linear(128,128) -> Linear(128,1) in FP32     */
+/* and sets the node's tau where the output is >= 0.501 (mesh.cpp:       */
+/* 2821-2827), the LES value otherwise. The activation between layers is  */
+/* ReLU (the source does not state it). Products are fused multiply-adds  */
+/* in increasing input order, so the GPU evaluation matches bit for bit.  */
+/* ------------------------------------------------------------------ */
+float or_bnn_forward(const float* p, const float in[OR_BNN_NIN]) {
+    float a[OR_BNN_NH], b[OR_BNN_NH];
+    const float* W = p;
+    const float* B = p + OR_BNN_NH * OR_BNN_NIN;
+    for (int o = 0; o < OR_BNN_NH; o++) {
+        float v = B[o];
+        for (int k = 0; k < OR_BNN_NIN; k++) v = fmaf(W[o * OR_BNN_NIN + k], in[k], v);
+        a[o] = v > 0.0f ? v : 0.0f;
+    }
+    size_t off = OR_BNN_NH * OR_BNN_NIN + OR_BNN_NH;
+    for (int l = 0; l < OR_BNN_NHID; l++) {
+        W = p + off;
+        B = W + OR_BNN_NH * OR_BNN_NH;
+        for (int o = 0; o < OR_BNN_NH; o++) {
+            float v = B[o];
+            for (int k = 0; k < OR_BNN_NH; k++) v = fmaf(W[o * OR_BNN_NH + k], a[k], v);
+            b[o] = v > 0.0f ? v : 0.0f;
+        }
+        memcpy(a, b, sizeof(a));
+        off += OR_BNN_NH * OR_BNN_NH + OR_BNN_NH;
+    }
+    W = p + off;
+    float v = W[OR_BNN_NH];
+    for (int k = 0; k < OR_BNN_NH; k++) v = fmaf(W[k], a[k], v);
+    return v;
+}
+
+/* node (x, y) of type index t (0: T01, 1: T03, 2: T05, 3: T07) and input j's
+ * neighbour offset; the neighbours' order is the reference's input order */
+static const int BNN_OFF[4][5][2] = {
+    {{0, 1}, {1, 1}, {1, 0}, {1, -1}, {0, -1}},      /* T01, column 0, mesh.cpp:3149-3165 */
+    {{1, 0}, {1, -1}, {0, -1}, {-1, -1}, {-1, 0}},   /* T03, row ny-2, mesh.cpp:2788-2804 */
+    {{0, 1}, {-1, 1}, {-1, 0}, {-1, -1}, {0, -1}},   /* T05, column nx-1, mesh.cpp:3216-3232 */
+    {{-1, 0}, {-1, 1}, {0, 1}, {1, 1}, {1, 0}}};     /* T07, row 1, mesh.cpp:2854-2870 */
+
+int or_set_bnn(or_sim* s, int type, const double* params) {
+    const int t = type == 1 ? 0 : type == 3 ? 1 : type == 5 ? 2 : type == 7 ? 3 : -1;
+    if (t < 0) return -1;
+    if (!s->bnn[t]) s->bnn[t] = (float*)malloc(sizeof(float) * OR_BNN_NPARAM);
+    for (int i = 0; i < OR_BNN_NPARAM; i++) s->bnn[t][i] = (float)params[i];
+    if (!s->bnn_tau) s->bnn_tau = (double*)calloc((size_t)s->N, sizeof(double));
+    return 0;
+}
+
+/* The reference's call order in one step (mesh.cpp:363-378, one rank):
+ * T01, T05, then T03, T07, each writing the tau or the rejection of its
+ * nodes, so the row networks decide at the four nodes a column shares
+ * with them. Neighbours beyond x = 0 or nx-1 are the periodic ring's
+ * (the reference's ghost columns, mesh.cpp:440-501). */
+static void bnn_phase(or_sim* s) {
+    const int nx = s->c.nx, ny = s->c.ny;
+    static const int order[4] = {0, 2, 1, 3};
+    for (int q = 0; q < 4; q++) {
+        const int t = order[q];
+        if (!s->bnn[t]) continue;
+        const int cnt = (t == 1 || t == 3) ? nx : ny - 2;
+        for (int i = 0; i < cnt; i++) {
+            int x, y;
+            if (t == 0) { x = 0; y = 1 + i; }
+            else if (t == 2) { x = nx - 1; y = 1 + i; }
+            else if (t == 1) { x = i; y = ny - 2; }
+            else { x = i; y = 1; }
+            float in[OR_BNN_NIN];
+            for (int j = 0; j < 5; j++) {
+                int xn = x + BNN_OFF[t][j][0], yn = y + BNN_OFF[t][j][1];
+                if (xn < 0) xn += nx;
+                if (xn >= nx) xn -= nx;
+                const int n = yn * nx + xn;
+                in[3 * j] = (float)s->ux[n];
+                in[3 * j + 1] = (float)s->uy[n];
+                in[3 * j + 2] = (float)s->rho[n];
+            }
+            const double tau = (double)or_bnn_forward(s->bnn[t], in);
+            s->bnn_tau[y * nx + x] = tau >= 0.501 ? tau : 0.0;
+        }
+    }
 }
 
 static void record_forces(or_sim* s) {
@@ -555,7 +647,10 @@ static void collide_phase(or_sim* s) {
         double tb = s->tau_base[n];
         int nn_ok = 0;
         double tau = tb;
-        if (c->tau_mode == OR_TAU_NN && s->has_mlp) {
+        if (s->bnn_tau && s->bnn_tau[n] > 0.0) { /* a boundary network's accepted value */
+            tau = s->bnn_tau[n];
+            nn_ok = 1;
+        } else if (c->tau_mode == OR_TAU_NN && s->has_mlp) {
             double feat[6];
             or_mlp_features(f, rho, ux, uy, feat);
             double t = or_mlp_forward(s->mlp, feat);
@@ -767,6 +862,7 @@ static void stream_phase(or_sim* s) {
  * with the immersed boundary wired before the collision. */
 int or_step(or_sim* s, int nsteps) {
     for (int t = 0; t < nsteps; t++) {
+        if (s->bnn_tau) bnn_phase(s);
         if (s->nbody > 0) ib_phase(s);
         collide_phase(s);
         stream_phase(s);

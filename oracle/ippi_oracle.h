@@ -63,6 +63,15 @@ double or_mlp_forward(const double* params, const double in[6]);
 /* loss = mean_i 0.5*(1/tau(x_i) - y_i)^2 (y: relaxation rates); returns loss, writes dL/dparams */
 double or_mlp_loss_grad(const double* params, const double* x, const double* y, int n, double* grad);
 
+/* ---- boundary-row / column networks (Mesh::NNmodel_Type01/03/05/07) ---- */
+/* params (FP64 values, used as FP32): W0[128][15] b0[128], 6 x (W[128][128]
+ * b[128]), W7[128] b7; ReLU between layers. */
+#define OR_BNN_NIN 15
+#define OR_BNN_NH 128
+#define OR_BNN_NHID 6
+#define OR_BNN_NPARAM (OR_BNN_NIN * OR_BNN_NH + OR_BNN_NH + OR_BNN_NHID * (OR_BNN_NH * OR_BNN_NH + OR_BNN_NH) + OR_BNN_NH + 1)
+float  or_bnn_forward(const float* params, const float in[OR_BNN_NIN]);
+
 /* ---- whole-domain solver ---- */
 typedef struct or_sim or_sim;
 or_sim* or_create(const or_config* cfg);
@@ -73,6 +82,8 @@ void    or_set_state(or_sim* s, const double* fnew, const double* rho, const dou
 int     or_add_body(or_sim* s, double cx, double cy, double diameter, int nmarkers, double udx, double udy);
 void    or_set_tau_field(or_sim* s, const double* tau);
 void    or_set_mlp(or_sim* s, const double* params);
+/* type 1, 3, 5 or 7: the reference's network type; returns 0, or -1 for another type */
+int     or_set_bnn(or_sim* s, int type, const double* params);
 int     or_step(or_sim* s, int nsteps);
 void    or_get_fields(const or_sim* s, double* rho, double* ux, double* uy);
 void    or_get_fnew(const or_sim* s, double* fnew);

@@ -58,6 +58,10 @@ def lib():
         L.or_noise.argtypes = [ctypes.c_uint64, ctypes.c_int64, i]
         L.or_noise.restype = d
         L.or_mlp_features.argtypes = [_D, d, d, d, _D]
+        L.or_bnn_forward.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.or_bnn_forward.restype = ctypes.c_float
+        L.or_set_bnn.argtypes = [vp, i, _D]
+        L.or_set_bnn.restype = i
         L.or_mlp_forward.argtypes = [_D, _D]
         L.or_mlp_forward.restype = d
         L.or_mlp_loss_grad.argtypes = [_D, _D, _D, i, _D]
@@ -164,6 +168,18 @@ def mlp_loss_grad(params, xs, ys):
     return loss, g
 
 
+BNN_NPARAM = 15 * 128 + 128 + 6 * (128 * 128 + 128) + 128 + 1
+
+
+def bnn_forward(params, x):
+    """The boundary network of Mesh::NNmodel_Type0x in FP32: params
+    W0[128][15] b0[128], 6 x (W[128][128] b[128]), W7[128] b7; x [15]."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    v = np.ascontiguousarray(x, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    return float(lib().or_bnn_forward(p.ctypes.data_as(fp), v.ctypes.data_as(fp)))
+
+
 # ---- whole-domain solver ------------------------------------------------
 class Sim:
     """FP64 CPU solver; same configuration vocabulary as the HIP library."""
@@ -210,6 +226,14 @@ class Sim:
         b = np.ascontiguousarray(blob, dtype=np.float64)
         assert b.size == MLP_BLOB
         lib().or_set_mlp(self.h, _p(b))
+
+    def set_bnn(self, net_type, params):
+        """Boundary-row / column network of type 1, 3, 5 or 7 (FP64 values,
+        evaluated in FP32), layout of BNN_NPARAM parameters as bnn_forward."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        assert p.size == BNN_NPARAM
+        if lib().or_set_bnn(self.h, int(net_type), _p(p)) != 0:
+            raise ValueError("boundary network type must be 1, 3, 5 or 7")
 
     def step(self, n):
         return lib().or_step(self.h, n)
