@@ -15,9 +15,14 @@
 //       (free slip ahead of x = 400, no slip beyond); Tecplot fields every
 //       SaveStep (WriteDatFluid + MergeDatFluid), running means from
 //       startAvg = 30000 (ComputeAvgVelocity).
+//   ippi_main plate NumStep SaveStep NNdir
+//       the same with the boundary-row / column networks whose weight files
+//       NNdir/AutoT0{1,3,5,7}.bin exist (101,249 FP64 values each).
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <string>
 
 #include "mesh.h"
 
@@ -68,7 +73,7 @@ static int run_cylinder(int NumStep, int SaveStep) {
     return 0;
 }
 
-static int run_plate(int NumStep, int SaveStep) {
+static int run_plate(int NumStep, int SaveStep, const char* NNdir) {
     int RankID = 0, CoreNum = 1;
     REAL Cs = 0.173;
     REAL Pr = 1;
@@ -99,6 +104,17 @@ static int run_plate(int NumStep, int SaveStep) {
     mesh.Precision = IPPI_FP64;
     mesh.Boundary = IPPI_BC_FLATPLATE;
     mesh.PlateX = 400;
+    // optional boundary networks: <NNdir>/AutoT0{1,3,5,7}.bin where present
+    // (the reference's AutoT0x.pt models, mesh.cpp:2778, as raw FP64 weights)
+    if (NNdir) {
+        for (int type = 1; type <= 7; type += 2) {
+            const std::string fn = std::string(NNdir) + "/AutoT0" + std::to_string(type) + ".bin";
+            if (std::FILE* fp = std::fopen(fn.c_str(), "rb")) {
+                std::fclose(fp);
+                mesh.NNmodelPath[type] = fn;
+            }
+        }
+    }
     mesh.InitialFile();
     mesh.InitialLatMesh(Nx, Ny, tau, StartVel, Rho, RankID, CoreNum, 0);
     mesh.JudgeStyle(CylinderCenter, Radius, RankID, CoreNum);
@@ -198,6 +214,7 @@ int main(int argc, char* argv[]) {
     if (argc > 1 && std::strcmp(argv[1], "check") == 0) return run_check();
     if (argc > 2 && std::strcmp(argv[1], "xflow") == 0) return run_xflow(argv[2]);
     if (argc > 1 && std::strcmp(argv[1], "plate") == 0)
-        return run_plate(argc > 2 ? std::atoi(argv[2]) : 200000, argc > 3 ? std::atoi(argv[3]) : 20000);
+        return run_plate(argc > 2 ? std::atoi(argv[2]) : 200000, argc > 3 ? std::atoi(argv[3]) : 20000,
+                         argc > 4 ? argv[4] : nullptr);
     return run_cylinder(argc > 1 ? std::atoi(argv[1]) : 2000, argc > 2 ? std::atoi(argv[2]) : 500);
 }

@@ -77,6 +77,20 @@ void Mesh::EnsureHandle(REAL Cs) {
         tau_.assign((size_t)nx_ * ny_, startTau);
         UploadTau();
     }
+    // boundary networks: loaded once, evaluated by every OneLBMStep (the
+    // reference reloads its model file on every call, mesh.cpp:2780)
+    for (int type = 1; type <= 7; type += 2) {
+        if (NNmodelPath[type].empty()) continue;
+        std::vector<double> w(101249);
+        std::FILE* fp = std::fopen(NNmodelPath[type].c_str(), "rb");
+        const size_t got = fp ? std::fread(w.data(), sizeof(double), w.size(), fp) : 0;
+        if (fp) std::fclose(fp);
+        if (got != w.size()) {
+            std::cerr << "Mesh facade: cannot read 101249 values from " << NNmodelPath[type] << std::endl;
+            std::exit(2);
+        }
+        Check(ippi_set_boundary_mlp(h_, type, w.data(), (int)w.size()), "ippi_set_boundary_mlp");
+    }
 }
 
 void Mesh::UploadTau() { Check(ippi_set_tau_field(h_, tau_.data()), "ippi_set_tau_field"); }

@@ -60,6 +60,11 @@ public:
     // per-node relaxation times held by the facade (the reference's
     // LatMesh[i][j].tau), needed by FindAndWriteTau and ChangeTauFromXflow
     int TauField = 0;
+    // boundary-row / column networks (NNmodel_Type01/03/05/07): weight file
+    // per type, 101,249 little-endian FP64 values in nn.Linear order
+    // (W0[128][15] b0, 6 x (W[128][128] b), W7[128] b7); empty = not used.
+    // The reference loads TorchScript archives (AutoT0x.pt, mesh.cpp:2778).
+    std::string NNmodelPath[8];
     // high-fidelity data files: XflowPrefix + (t + XflowStepOffset) + ".csv"
     std::string XflowPrefix = "Xflowdata_pingban_every_1_step/pb_";
     int XflowStepOffset = 10014;

@@ -39,3 +39,37 @@ def test_facade_flatplate_tecplot(tmp_path):
     np.testing.assert_allclose(d[:, 2], rho[Y, X], rtol=1e-5)
     np.testing.assert_allclose(d[:, 3], ux[Y, X], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(d[:, 4], uy[Y, X], rtol=1e-5, atol=1e-9)
+
+
+def test_facade_flatplate_boundary_networks(tmp_path):
+    """ippi_main plate with boundary-network weight files (AutoT03.bin and
+    AutoT07.bin, the top and bottom fluid rows) against the binding's solver
+    with the same networks."""
+    assert os.path.exists(BIN), "facade driver not built (run __graft_entry__.build())"
+    ippi = _pkg.load()
+    from ippi_amd import workloads as wl  # noqa: E402  (package alias set by tests._pkg)
+    nets = {t: wl.boundary_blob(40 + t) for t in (3, 7)}
+    for t, p in nets.items():
+        p.astype("<f8").tofile(tmp_path / f"AutoT0{t}.bin")
+    print("[facade] ippi_main plate 100 100 <nets 3, 7>", flush=True)
+    r = subprocess.run([BIN, "plate", "100", "100", str(tmp_path)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = np.loadtxt(tmp_path / "fluid" / "fluid_Merged_100.dat", skiprows=3)
+    tau = 0.5 + 6.0 * (501 * 0.1) / 5e5
+    with ippi.Solver(2401, 501, "bgk", "fp64", "flatplate", les=True, tau0=tau, cs=0.173, u_in=(0.1, 0.0),
+                     plate_x=400) as s:
+        s.init_fields(0, 1.0, (0.1, 0.0), 0.0)
+        for t, p in nets.items():
+            s.set_boundary_mlp(t, p)
+        s.step(100)
+        rho, ux, uy = s.fields()
+    X, Y = d[:, 0].astype(int), d[:, 1].astype(int)
+    np.testing.assert_allclose(d[:, 2], rho[Y, X], rtol=1e-5)
+    np.testing.assert_allclose(d[:, 3], ux[Y, X], rtol=1e-5, atol=1e-9)
+    with ippi.Solver(2401, 501, "bgk", "fp64", "flatplate", les=True, tau0=tau, cs=0.173, u_in=(0.1, 0.0),
+                     plate_x=400) as s:
+        s.init_fields(0, 1.0, (0.1, 0.0), 0.0)
+        s.step(100)
+        ux0 = s.fields()[1]
+    assert not np.allclose(ux0[Y, X], d[:, 3], rtol=1e-5, atol=1e-9), "networks had no effect"
