@@ -183,6 +183,23 @@ int ippi_mass(ippi_handle* h, double* mass);
 int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, float* feat, float* label,
                       float* weight);
 
+/* Effective-relaxation-time pairs at half this solver's resolution, from the
+ * current state (the solver does not advance). No counterpart in the
+ * reference, whose labels come from the one-step search above
+ * (FindAndWriteTau, mesh.cpp:2625-2715); this criterion labels each coarse
+ * node with the relaxation time whose viscous stress on the coarse grid,
+ * 2 rho nu C with nu = (tau - 1/2)/3, best matches (least squares over the
+ * deviatoric components) the restricted fine momentum flux: the block mean
+ * of the fine viscous stress -(1 - 1/(2 tau_l)) Pi^neq (tau_l the fine
+ * node's relaxation time, LES included, as in mesh.cpp:877-910) minus the
+ * subgrid stress <rho u u> - rho_c u_c u_c, against the coarse strain
+ * C = 2 <S> (S = -Pi^neq / (2 rho c_s^2 tau_l) per fine node). Outputs per
+ * coarse node: feat[6] as above (restricted state, scale
+ * s = 2 tau_coarse / tau0), label omega = 1/clamp(tau, 0.5005, 2), weight
+ * dev C : dev C (0 where the coarse strain vanishes or within two coarse
+ * cells of a body's marker support). */
+int ippi_coarse_stress_pairs(ippi_handle* h, double tau_coarse, float* feat, float* label, float* weight);
+
 /* GPU trainer of the relaxation-time network (no counterpart in the
  * reference, whose networks are trained offline). Targets are relaxation
  * rates omega; loss sum w (1/tau(x) - y)^2 / (2 sum w) in FP32;

@@ -76,6 +76,7 @@ _SIGS = {
                         ctypes.c_int),
     "ippi_set_populations": ([ctypes.c_void_p, _D], ctypes.c_int),
     "ippi_coarse_pairs": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _F, _F, _F], ctypes.c_int),
+    "ippi_coarse_stress_pairs": ([ctypes.c_void_p, ctypes.c_double, _F, _F, _F], ctypes.c_int),
     "ippi_trainer_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "ippi_trainer_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "ippi_trainer_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
@@ -195,15 +196,23 @@ class Solver:
         a = _f64(f, 9 * self.N)
         self._check(library().ippi_set_populations(self.h, _p(a)))
 
-    def coarse_pairs(self, tau_coarse, coarse_collision="rr"):
-        """Training pairs at half resolution; advances this solver two steps.
-        Returns (features [n,6], labels [n], weights [n]) over the coarse grid."""
+    def coarse_pairs(self, tau_coarse, coarse_collision="rr", criterion="step"):
+        """Training pairs at half resolution: (features [n,6], labels omega
+        [n], weights [n]) over the coarse grid. criterion "step": the
+        reference's one-step search (advances this solver two steps);
+        "stress": effective relaxation times by stress matching (no advance,
+        coarse_collision unused)."""
         nc = (self.nx // 2) * (self.ny // 2)
         feat = np.empty(6 * nc, dtype=np.float32)
         lab = np.empty(nc, dtype=np.float32)
         w = np.empty(nc, dtype=np.float32)
-        self._check(library().ippi_coarse_pairs(self.h, float(tau_coarse), COLLISION[coarse_collision],
-                                                _pf(feat), _pf(lab), _pf(w)))
+        if criterion == "step":
+            self._check(library().ippi_coarse_pairs(self.h, float(tau_coarse), COLLISION[coarse_collision],
+                                                    _pf(feat), _pf(lab), _pf(w)))
+        elif criterion == "stress":
+            self._check(library().ippi_coarse_stress_pairs(self.h, float(tau_coarse), _pf(feat), _pf(lab), _pf(w)))
+        else:
+            raise ValueError(f"unknown criterion {criterion!r}")
         return feat.reshape(nc, 6), lab, w
 
     def set_tau_field(self, tau):

@@ -1155,6 +1155,30 @@ __global__ void __launch_bounds__(BX) k_restrict(const T* __restrict__ src, doub
     }
 }
 
+// The network's inputs at coarse node p, from p's restricted state: rho,
+// ux, uy, |S| (the strain estimate of the LES, from the non-equilibrium
+// stress), Pi_xx - Pi_yy and Pi_xy of f - feq.
+__device__ __forceinline__ void pair_features(const double* __restrict__ fc, const double* __restrict__ mc,
+                                              unsigned Nc, unsigned p, float* __restrict__ feat) {
+#pragma clang fp contract(off)
+    double d[9], q[9];
+    const double rho = mc[p], ux = mc[Nc + p], uy = mc[2 * Nc + p];
+    ref::feq(rho, ux, uy, q);
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = fc[(size_t)k * Nc + p] - q[k];
+    const double pxx = ((((d[1] + d[3]) + d[5]) + d[6]) + d[7]) + d[8];
+    const double pxy = ((d[5] - d[6]) + d[7]) - d[8];
+    const double pyy = ((((d[2] + d[4]) + d[5]) + d[6]) + d[7]) + d[8];
+    const double c = -3.0 / (2 * rho);
+    const double sxx = pxx * c, sxy = pxy * c, syy = pyy * c;
+    feat[6 * p + 0] = (float)rho;
+    feat[6 * p + 1] = (float)ux;
+    feat[6 * p + 2] = (float)uy;
+    feat[6 * p + 3] = (float)sqrt(2.0 * (sxx * sxx + 2.0 * sxy * sxy + syy * syy));
+    feat[6 * p + 4] = (float)(pxx - pyy);
+    feat[6 * p + 5] = (float)pxy;
+}
+
 // Labels: for coarse node p, the relaxation rate omega whose collision at
 // the nine upstream nodes q_k = p - e_k, once streamed into p, best matches
 // the target mass and momentum (least squares, closed form). This is the
@@ -1171,28 +1195,7 @@ __global__ void __launch_bounds__(BX) k_labels(const double* __restrict__ fc, co
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y;
     if (I >= nxc) return;
     const unsigned Nc = (unsigned)nxc * nyc, p = (unsigned)J * nxc + I;
-    // features of p (the network's inputs, from p's own state)
-    double f[9], q[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) f[k] = fc[(size_t)k * Nc + p];
-    const double rho = mc[p], ux = mc[Nc + p], uy = mc[2 * Nc + p];
-    ref::feq(rho, ux, uy, q);
-    {
-        double d[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) d[k] = f[k] - q[k];
-        const double pxx = ((((d[1] + d[3]) + d[5]) + d[6]) + d[7]) + d[8];
-        const double pxy = ((d[5] - d[6]) + d[7]) - d[8];
-        const double pyy = ((((d[2] + d[4]) + d[5]) + d[6]) + d[7]) + d[8];
-        const double c = -3.0 / (2 * rho);
-        const double sxx = pxx * c, sxy = pxy * c, syy = pyy * c;
-        feat[6 * p + 0] = (float)rho;
-        feat[6 * p + 1] = (float)ux;
-        feat[6 * p + 2] = (float)uy;
-        feat[6 * p + 3] = (float)sqrt(2.0 * (sxx * sxx + 2.0 * sxy * sxy + syy * syy));
-        feat[6 * p + 4] = (float)(pxx - pyy);
-        feat[6 * p + 5] = (float)pxy;
-    }
+    pair_features(fc, mc, Nc, p, feat);
     bool ok = I >= 2 && I < nxc - 2 && J >= 2 && J < nyc - 2;
     for (int b = 0; b < nexcl; b++) {
         const int4 e = excl[b];
@@ -1241,6 +1244,92 @@ __global__ void __launch_bounds__(BX) k_labels(const double* __restrict__ fc, co
     ok = ok && aa > 1e-30 && om > 0.0 && om < 4.0;
     label[p] = ok ? (float)om : 0.0f;
     weight[p] = ok ? (float)aa : 0.0f;
+}
+
+// Effective-relaxation-time labels (stress matching). For coarse node p and
+// its 2x2 fine block B, with each fine node's post-streaming state, its
+// non-equilibrium stress Pi = sum e e (f - feq) and the relaxation time tau_l
+// of its next collision (LES included):
+//   fine viscous stress  sigma = -(1 - 1/(2 tau_l)) Pi,
+//   fine strain          S     = -Pi / (2 rho c_s^2 tau_l),
+//   subgrid stress       R     = <rho u u>_B - rho_c u_c u_c,
+//   target stress        T     = <sigma>_B - R,
+//   coarse strain        C     = 2 <S>_B (coarse time unit = 2 fine),
+// the coarse viscosity nu minimising |dev T - 2 rho_c nu dev C|^2 is
+// nu = dev T : dev C / (2 rho_c dev C : dev C), and tau = 1/2 + 3 nu: the
+// relaxation time whose viscous stress on the coarse grid carries the
+// momentum flux of the resolved fine solution (the a-priori dynamic
+// procedure of LES). Label omega = 1 / clamp(tau, 0.5005, 2); weight
+// dev C : dev C (0 within two coarse cells of a body's support).
+template <typename T, int BC>
+__global__ void __launch_bounds__(BX) k_stress_labels(const T* __restrict__ src, const double* __restrict__ tauf,
+                                                      const double* __restrict__ fc, const double* __restrict__ mc,
+                                                      int nx, int ny, double uinx, double uiny, RefState ref,
+                                                      const int4* __restrict__ excl, int nexcl,
+                                                      float* __restrict__ feat, float* __restrict__ label,
+                                                      float* __restrict__ weight) {
+#pragma clang fp contract(off)
+    const int nxc = nx / 2, nyc = ny / 2;
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y;
+    if (I >= nxc) return;
+    const unsigned N = (unsigned)nx * ny, Nc = (unsigned)nxc * nyc, p = (unsigned)J * nxc + I;
+    pair_features(fc, mc, Nc, p, feat);
+    // block sums: rho, rho u, rho u u, sigma, S
+    double sr = 0, smx = 0, smy = 0, rxx = 0, rxy = 0, ryy = 0, gxx = 0, gxy = 0, gyy = 0, cxx = 0, cxy = 0, cyy = 0;
+    for (int dj = 0; dj < 2; dj++)
+        for (int di = 0; di < 2; di++) {
+            const int x = 2 * I + di, y = 2 * J + dj;
+            double f[9], r, vx, vy;
+            if constexpr (sizeof(T) == 8) {
+                node_state(src, x, y, nx, ny, N, BC, uinx, uiny, f, r, vx, vy);
+            } else {
+                float g[9];
+                Mo o;
+                node_state32<BC>(src, x, y, nx, ny, N, (float)(uinx - ref.U), (float)uiny, ref, g, o);
+#pragma unroll
+                for (int k = 0; k < 9; k++) f[k] = (double)g[k] + ref.c[k];
+                ref::macro(f, r, vx, vy);
+            }
+            double q[9], d[9];
+            ref::feq(r, vx, vy, q);
+#pragma unroll
+            for (int k = 0; k < 9; k++) d[k] = f[k] - q[k];
+            const double pxx = ((((d[1] + d[3]) + d[5]) + d[6]) + d[7]) + d[8];
+            const double pxy = ((d[5] - d[6]) + d[7]) - d[8];
+            const double pyy = ((((d[2] + d[4]) + d[5]) + d[6]) + d[7]) + d[8];
+            const double tl = tauf[(unsigned)y * nx + x];
+            const double gs = -(1.0 - 0.5 / tl), cs = -3.0 / (2.0 * r * tl);
+            sr += r;
+            smx += r * vx;
+            smy += r * vy;
+            rxx += r * vx * vx;
+            rxy += r * vx * vy;
+            ryy += r * vy * vy;
+            gxx += gs * pxx;
+            gxy += gs * pxy;
+            gyy += gs * pyy;
+            cxx += cs * pxx;
+            cxy += cs * pxy;
+            cyy += cs * pyy;
+        }
+    const double rc = 0.25 * sr, ux = smx / sr, uy = smy / sr;
+    // T = <sigma> - (<rho u u> - rho_c u_c u_c); C = 2 <S>
+    const double txx = 0.25 * gxx - (0.25 * rxx - rc * ux * ux);
+    const double txy = 0.25 * gxy - (0.25 * rxy - rc * ux * uy);
+    const double tyy = 0.25 * gyy - (0.25 * ryy - rc * uy * uy);
+    const double Cxx = 0.5 * cxx, Cxy = 0.5 * cxy, Cyy = 0.5 * cyy;
+    const double tdx = 0.5 * (txx - tyy), cdx = 0.5 * (Cxx - Cyy);  // deviatoric xx (= -yy)
+    const double num = 2.0 * tdx * cdx + 2.0 * txy * Cxy;
+    const double den = 2.0 * cdx * cdx + 2.0 * Cxy * Cxy;
+    bool ok = den > 0.0;
+    for (int b = 0; b < nexcl; b++) {
+        const int4 e = excl[b];
+        if (I >= e.x && I <= e.z && J >= e.y && J <= e.w) ok = false;
+    }
+    const double tau = ok ? 0.5 + 3.0 * (num / (2.0 * rc * den)) : 1.0;
+    const double tcl = fmin(fmax(tau, 0.5005), 2.0);
+    label[p] = ok ? (float)(1.0 / tcl) : 0.0f;
+    weight[p] = ok ? (float)den : 0.0f;
 }
 
 // ----------------------------------------------------------- init kernels
@@ -3425,6 +3514,40 @@ int ippi_set_fields(ippi_handle* h, const double* rho, const double* ux, const d
 }
 
 
+// Restriction of the current post-streaming state to the grid of half the
+// resolution (k_restrict): fc (9 Nc, skipped when null) and mc (3 Nc).
+static hipError_t launch_restrict(ippi_handle* h, double sc, double* f_out, double* m_out) {
+    const int nxc = h->c.nx / 2, nyc = h->c.ny / 2;
+    dim3 grid((nxc + BX - 1) / BX, nyc);
+    const bool pb = h->c.bc == IPPI_BC_PERIODIC_BB;
+    if (h->esz == 8) {
+        const double* src = (const double*)h->buf[h->cur];
+        if (pb)
+            hipLaunchKernelGGL((k_restrict<double, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
+                               h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
+        else
+            hipLaunchKernelGGL((k_restrict<double, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
+                               h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
+    } else {
+        const float* src = (const float*)h->buf[h->cur];
+        if (pb)
+            hipLaunchKernelGGL((k_restrict<float, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
+                               h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
+        else
+            hipLaunchKernelGGL((k_restrict<float, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
+                               h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
+    }
+    return hipGetLastError();
+}
+
+// Coarse cells covered by a body's marker support, plus a margin of 2.
+static std::vector<int4> pair_exclusions(const ippi_handle* h) {
+    std::vector<int4> ex;
+    for (const auto& b : h->bodies)
+        ex.push_back(make_int4(b.p.x0 / 2 - 2, b.p.y0 / 2 - 2, (b.p.x0 + b.p.w) / 2 + 2, (b.p.y0 + b.p.h) / 2 + 2));
+    return ex;
+}
+
 // Training pairs at half this solver's resolution (see k_restrict / k_labels).
 int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, float* feat, float* label,
                       float* weight) {
@@ -3444,9 +3567,7 @@ int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, f
     double *fc = nullptr, *mc = nullptr, *tg = nullptr;
     float *df = nullptr, *dl = nullptr, *dw = nullptr;
     int4* dex = nullptr;
-    std::vector<int4> ex;
-    for (const auto& b : h->bodies)  // coarse cells covered by a body's support, plus a margin of 2
-        ex.push_back(make_int4(b.p.x0 / 2 - 2, b.p.y0 / 2 - 2, (b.p.x0 + b.p.w) / 2 + 2, (b.p.y0 + b.p.h) / 2 + 2));
+    const std::vector<int4> ex = pair_exclusions(h);
     int rc = IPPI_OK;
     auto cleanup = [&]() {
         for (void* q : {(void*)fc, (void*)mc, (void*)tg, (void*)df, (void*)dl, (void*)dw, (void*)dex})
@@ -3470,27 +3591,7 @@ int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, f
     if (!ex.empty()) PCK(hipMemcpy(dex, ex.data(), sizeof(int4) * ex.size(), hipMemcpyHostToDevice));
     const double sc = 2.0 * tau_coarse / h->c.tau0;
     dim3 grid((nxc + BX - 1) / BX, nyc);
-    auto restrict_ = [&](double* f_out, double* m_out) {
-        const bool pb = h->c.bc == IPPI_BC_PERIODIC_BB;
-        if (h->esz == 8) {
-            const double* src = (const double*)h->buf[h->cur];
-            if (pb)
-                hipLaunchKernelGGL((k_restrict<double, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
-                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
-            else
-                hipLaunchKernelGGL((k_restrict<double, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
-                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
-        } else {
-            const float* src = (const float*)h->buf[h->cur];
-            if (pb)
-                hipLaunchKernelGGL((k_restrict<float, IPPI_BC_PERIODIC_BB>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
-                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
-            else
-                hipLaunchKernelGGL((k_restrict<float, IPPI_BC_INFLOW>), grid, dim3(BX), 0, h->s, src, f_out, m_out,
-                                   h->c.nx, h->c.ny, h->c.u_in_x, h->c.u_in_y, sc, f_out ? 1 : 0, h->ref);
-        }
-        return hipGetLastError();
-    };
+    auto restrict_ = [&](double* f_out, double* m_out) { return launch_restrict(h, sc, f_out, m_out); };
     PCK(restrict_(fc, mc));
     rc = ippi_step(h, 2);  // one coarse step = two fine steps
     if (rc) {
@@ -3954,8 +4055,9 @@ int ippi_get_stencil(ippi_handle* h, int n, const int* xy, double* fnew, double*
     return IPPI_OK;
 }
 
-int ippi_get_tau(ippi_handle* h, double* tau) {
-    if (!h || !tau) return IPPI_ERR_ARG;
+// The relaxation time of the next collision at every node into d_out (device,
+// N doubles), queued on the solver's stream.
+static int launch_tau(ippi_handle* h, double* d_out) {
     if (h->phase == 0) return fail(h, IPPI_ERR_STATE, "no step taken yet");
     if (h->c.tau_mode == IPPI_TAU_FIELD && !h->tau) return fail(h, IPPI_ERR_STATE, "tau field not set");
     if (nn_kind(h) && !h->has_mlp) return fail(h, IPPI_ERR_STATE, "relaxation-time MLP not set");
@@ -3964,8 +4066,6 @@ int ippi_get_tau(ippi_handle* h, double* tau) {
         int r = h->esz == 8 ? launch_bnn<double>(h, a, true) : launch_bnn<float>(h, a, true);
         if (r) return r;
     }
-    double* d_out = nullptr;
-    HIPC(h, hipMalloc(&d_out, sizeof(double) * h->N));
     dim3 grid((h->c.nx + BX - 1) / BX, h->c.ny);
     const int nk = nn_kind(h);
     const bool nn = nk != 0;
@@ -4000,10 +4100,95 @@ int ippi_get_tau(ippi_handle* h, double* tau) {
     }
 #undef IPPI_TAU_LAUNCH
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(tau, d_out, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s);
+    if (e != hipSuccess) return fail(h, IPPI_ERR_HIP, std::string("relaxation-time kernel: ") + hipGetErrorString(e));
+    return IPPI_OK;
+}
+
+int ippi_get_tau(ippi_handle* h, double* tau) {
+    if (!h || !tau) return IPPI_ERR_ARG;
+    double* d_out = nullptr;
+    HIPC(h, hipMalloc(&d_out, sizeof(double) * h->N));
+    int r = launch_tau(h, d_out);
+    if (r) {
+        (void)hipFree(d_out);
+        return r;
+    }
+    hipError_t e = hipMemcpyAsync(tau, d_out, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->s);
     if (e == hipSuccess) e = hipStreamSynchronize(h->s);
     (void)hipFree(d_out);
     if (e != hipSuccess) return fail(h, IPPI_ERR_HIP, std::string("ippi_get_tau: ") + hipGetErrorString(e));
+    return check_flag(h);
+}
+
+// Effective-relaxation-time pairs at half this solver's resolution (see
+// k_stress_labels); the solver does not advance.
+int ippi_coarse_stress_pairs(ippi_handle* h, double tau_coarse, float* feat, float* label, float* weight) {
+    if (!h || !feat || !label || !weight) return IPPI_ERR_ARG;
+    if (h->c.nx % 2 || h->c.ny % 2 || h->c.nx < 16 || h->c.ny < 16) return fail(h, IPPI_ERR_ARG, "grid must be even and >= 16");
+    if (h->c.bc == IPPI_BC_FLATPLATE)
+        return fail(h, IPPI_ERR_ARG, "training pairs are built for the periodic and inflow boundary kinds");
+    if (!(tau_coarse > 0.5)) return IPPI_ERR_ARG;
+    if (h->phase == 0) {
+        int r = ippi_step(h, 1);
+        if (r) return r;
+    }
+    const int nxc = h->c.nx / 2, nyc = h->c.ny / 2;
+    const size_t Nc = (size_t)nxc * nyc;
+    double *fc = nullptr, *mc = nullptr, *tf = nullptr;
+    float *df = nullptr, *dl = nullptr, *dw = nullptr;
+    int4* dex = nullptr;
+    const std::vector<int4> ex = pair_exclusions(h);
+    auto cleanup = [&]() {
+        for (void* q : {(void*)fc, (void*)mc, (void*)tf, (void*)df, (void*)dl, (void*)dw, (void*)dex})
+            if (q) (void)hipFree(q);
+    };
+#define PCK(expr)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            cleanup();                                                                         \
+            return fail(h, IPPI_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+        }                                                                                      \
+    } while (0)
+    PCK(hipMalloc(&fc, sizeof(double) * 9 * Nc));
+    PCK(hipMalloc(&mc, sizeof(double) * 3 * Nc));
+    PCK(hipMalloc(&tf, sizeof(double) * h->N));
+    PCK(hipMalloc(&df, sizeof(float) * 6 * Nc));
+    PCK(hipMalloc(&dl, sizeof(float) * Nc));
+    PCK(hipMalloc(&dw, sizeof(float) * Nc));
+    PCK(hipMalloc(&dex, sizeof(int4) * std::max<size_t>(1, ex.size())));
+    if (!ex.empty()) PCK(hipMemcpy(dex, ex.data(), sizeof(int4) * ex.size(), hipMemcpyHostToDevice));
+    PCK(launch_restrict(h, 2.0 * tau_coarse / h->c.tau0, fc, mc));
+    int rc = launch_tau(h, tf);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    dim3 grid((nxc + BX - 1) / BX, nyc);
+    const bool pb = h->c.bc == IPPI_BC_PERIODIC_BB;
+#define IPPI_SL_LAUNCH(T, B)                                                                                   \
+    hipLaunchKernelGGL((k_stress_labels<T, B>), grid, dim3(BX), 0, h->s, (const T*)h->buf[h->cur],             \
+                       (const double*)tf, (const double*)fc, (const double*)mc, h->c.nx, h->c.ny, h->c.u_in_x, \
+                       h->c.u_in_y, h->ref, (const int4*)dex, (int)ex.size(), df, dl, dw)
+    if (h->esz == 8) {
+        if (pb)
+            IPPI_SL_LAUNCH(double, IPPI_BC_PERIODIC_BB);
+        else
+            IPPI_SL_LAUNCH(double, IPPI_BC_INFLOW);
+    } else {
+        if (pb)
+            IPPI_SL_LAUNCH(float, IPPI_BC_PERIODIC_BB);
+        else
+            IPPI_SL_LAUNCH(float, IPPI_BC_INFLOW);
+    }
+#undef IPPI_SL_LAUNCH
+    PCK(hipGetLastError());
+    PCK(hipMemcpyAsync(feat, df, sizeof(float) * 6 * Nc, hipMemcpyDeviceToHost, h->s));
+    PCK(hipMemcpyAsync(label, dl, sizeof(float) * Nc, hipMemcpyDeviceToHost, h->s));
+    PCK(hipMemcpyAsync(weight, dw, sizeof(float) * Nc, hipMemcpyDeviceToHost, h->s));
+    PCK(hipStreamSynchronize(h->s));
+#undef PCK
+    cleanup();
     return check_flag(h);
 }
 

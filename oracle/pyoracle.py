@@ -313,3 +313,82 @@ def tau_search(fnew, feq, tau, target, m_lo=-10000, m_hi=10000, step=1e-5):
     bd, be = np.zeros(n), np.zeros(n)
     lib().or_tau_search(n, _p(fnew), _p(feq), _p(tau), _p(target), int(m_lo), int(m_hi), float(step), _p(bd), _p(be))
     return bd, be
+
+
+# ---- effective-relaxation-time pairs (stress matching) -------------------
+_EX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], dtype=np.float64)
+_EY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], dtype=np.float64)
+_W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def _feq_grid(rho, ux, uy):
+    """feq of NodeAndLattice.cpp:124-149 on whole fields, [9, ...]."""
+    eu = _EX.reshape((9,) + (1,) * rho.ndim) * ux + _EY.reshape((9,) + (1,) * rho.ndim) * uy
+    uu = ux * ux + uy * uy
+    return _W.reshape((9,) + (1,) * rho.ndim) * rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uu)
+
+
+def _stress(d):
+    """Pi_xx, Pi_xy, Pi_yy of populations d [9, ...]."""
+    sh = (9,) + (1,) * (d.ndim - 1)
+    ex, ey = _EX.reshape(sh), _EY.reshape(sh)
+    return (d * ex * ex).sum(0), (d * ex * ey).sum(0), (d * ey * ey).sum(0)
+
+
+def _blk(a):
+    return 0.25 * (a[..., 0::2, 0::2] + a[..., 0::2, 1::2] + a[..., 1::2, 0::2] + a[..., 1::2, 1::2])
+
+
+def stress_pairs(fp, tau_l, tau_coarse, tau0, excl=()):
+    """Effective-relaxation-time training pairs at half resolution (the
+    restatement ippi_coarse_stress_pairs is checked against).
+
+    fp [9, ny, nx]: post-streaming fine populations; tau_l [ny, nx]: the
+    relaxation time of each fine node's next collision (hybrid Smagorinsky
+    included, mesh.cpp:877-910); tau0: the fine base relaxation time;
+    excl: coarse boxes (x0, y0, x1, y1) inclusive that get weight 0.
+
+    Per fine node: sigma = -(1 - 1/(2 tau_l)) Pi^neq (the viscous stress of
+    its collision) and S = -Pi^neq / (2 rho c_s^2 tau_l) (its strain). Per
+    2x2 block: T = <sigma> - (<rho u u> - rho_c u_c u_c) (resolved minus
+    subgrid stress), C = 2 <S> (strain in coarse time units), label tau =
+    1/2 + 3 (dev T : dev C) / (2 rho_c dev C : dev C), returned as omega =
+    1/clamp(tau, 0.5005, 2), weight dev C : dev C. Features: rho, ux, uy,
+    |S|, Pi_xx - Pi_yy, Pi_xy of the restricted state feq(rho_c, u_c) +
+    s <f - feq> with s = 2 tau_coarse / tau0 (the network's inputs, as the
+    oracle's or_mlp_features computes them from a node's populations).
+    No counterpart in the reference (its labels come from FindAndWriteTau's
+    one-step search, mesh.cpp:2625-2715)."""
+    fp = np.asarray(fp, dtype=np.float64)
+    rho = fp.sum(0)
+    ux = (fp * _EX[:, None, None]).sum(0) / rho
+    uy = (fp * _EY[:, None, None]).sum(0) / rho
+    neq = fp - _feq_grid(rho, ux, uy)
+    pxx, pxy, pyy = _stress(neq)
+    g = -(1.0 - 0.5 / tau_l)
+    c = -3.0 / (2.0 * rho * tau_l)
+    rc = _blk(rho)
+    ucx, ucy = _blk(rho * ux) / rc, _blk(rho * uy) / rc
+    txx = _blk(g * pxx) - (_blk(rho * ux * ux) - rc * ucx * ucx)
+    txy = _blk(g * pxy) - (_blk(rho * ux * uy) - rc * ucx * ucy)
+    tyy = _blk(g * pyy) - (_blk(rho * uy * uy) - rc * ucy * ucy)
+    cxx, cxy, cyy = 2 * _blk(c * pxx), 2 * _blk(c * pxy), 2 * _blk(c * pyy)
+    a_t, a_c = 0.5 * (txx - tyy), 0.5 * (cxx - cyy)
+    num = 2 * a_t * a_c + 2 * txy * cxy
+    den = 2 * a_c * a_c + 2 * cxy * cxy
+    ok = den > 0
+    for (x0, y0, x1, y1) in excl:
+        ok[max(y0, 0):y1 + 1, max(x0, 0):x1 + 1] = False
+    tau = np.where(ok, 0.5 + 3 * num / (2 * rc * np.where(ok, den, 1.0)), 1.0)
+    label = np.where(ok, 1.0 / np.clip(tau, 0.5005, 2.0), 0.0)
+    weight = np.where(ok, den, 0.0)
+    # features of the restricted state
+    fa, na = _blk(fp), _blk(neq)
+    r2 = fa.sum(0)
+    vx, vy = (fa * _EX[:, None, None]).sum(0) / r2, (fa * _EY[:, None, None]).sum(0) / r2
+    fc = _feq_grid(r2, vx, vy) + (2.0 * tau_coarse / tau0) * na
+    qx, qxy, qy = _stress(fc - _feq_grid(r2, vx, vy))
+    k = -3.0 / (2.0 * r2)
+    sxx, sxy, syy = qx * k, qxy * k, qy * k
+    feat = np.stack([r2, vx, vy, np.sqrt(2.0 * (sxx * sxx + 2.0 * sxy * sxy + syy * syy)), qx - qy, qxy], axis=-1)
+    return feat.reshape(-1, 6), label.ravel(), weight.ravel(), tau.ravel()
