@@ -200,6 +200,15 @@ int ippi_coarse_pairs(ippi_handle* h, double tau_coarse, int coarse_collision, f
  * cells of a body's marker support). */
 int ippi_coarse_stress_pairs(ippi_handle* h, double tau_coarse, float* feat, float* label, float* weight);
 
+/* The current state restricted to the grid of half the resolution, as the
+ * pairs above use it: coarse node (I, J) takes the mass and momentum of fine
+ * nodes (2I..2I+1, 2J..2J+1) and s = 2 tau_coarse / tau0 times their mean
+ * non-equilibrium part. fc [9][ny/2][nx/2] (may be NULL), mc = rho, ux, uy
+ * [3][ny/2][nx/2]. The starting state of a coarse run and its target.
+ * No counterpart in the reference (its high-fidelity data come from files,
+ * mesh.cpp:2414-2470). */
+int ippi_restrict(ippi_handle* h, double tau_coarse, double* fc, double* mc);
+
 /* GPU trainer of the relaxation-time network (no counterpart in the
  * reference, whose networks are trained offline). Targets are relaxation
  * rates omega; loss sum w (1/tau(x) - y)^2 / (2 sum w) in FP32;

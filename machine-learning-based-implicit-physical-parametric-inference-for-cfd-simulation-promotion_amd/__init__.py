@@ -77,6 +77,7 @@ _SIGS = {
     "ippi_set_populations": ([ctypes.c_void_p, _D], ctypes.c_int),
     "ippi_coarse_pairs": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _F, _F, _F], ctypes.c_int),
     "ippi_coarse_stress_pairs": ([ctypes.c_void_p, ctypes.c_double, _F, _F, _F], ctypes.c_int),
+    "ippi_restrict": ([ctypes.c_void_p, ctypes.c_double, _D, _D], ctypes.c_int),
     "ippi_trainer_create": ([ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "ippi_trainer_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "ippi_trainer_last_error": ([ctypes.c_void_p], ctypes.c_char_p),
@@ -214,6 +215,16 @@ class Solver:
         else:
             raise ValueError(f"unknown criterion {criterion!r}")
         return feat.reshape(nc, 6), lab, w
+
+    def restrict(self, tau_coarse, populations=True):
+        """This state at half resolution: (coarse populations [9, ny/2, nx/2]
+        or None, (rho, ux, uy) each [ny/2, nx/2])."""
+        nxc, nyc = self.nx // 2, self.ny // 2
+        mc = np.empty(3 * nxc * nyc)
+        fc = np.empty(9 * nxc * nyc) if populations else None
+        self._check(library().ippi_restrict(self.h, float(tau_coarse), None if fc is None else _p(fc), _p(mc)))
+        m = mc.reshape(3, nyc, nxc)
+        return (None if fc is None else fc.reshape(9, nyc, nxc)), (m[0], m[1], m[2])
 
     def set_tau_field(self, tau):
         t = _f64(tau, self.N)

@@ -1628,7 +1628,7 @@ __global__ void __launch_bounds__(BNN_T) k_bnn(const T* __restrict__ src, const 
 // record (minus the sum of the marker forces, fixed-order tree in LDS).
 constexpr int IB_T = 512;        // threads per IB block = active nodes per block
 constexpr int IB_MAXM = 1024;    // markers per body
-constexpr int IB_MAXACT = 3072;  // active nodes per body
+constexpr int IB_MAXACT = 6144;  // active nodes per body (96 KB of LDS velocities: gfx950 gives a block up to 160 KB)
 
 // Everything a block needs, fetched with one scalar load.
 struct alignas(16) IbBlk {
@@ -2761,7 +2761,7 @@ int upload_ib(ippi_handle* h) {
             }
         }
         const int a1 = (int)act.size();
-        if (a1 - a0 > IB_MAXACT) return fail(h, IPPI_ERR_ARG, "body support too large (active nodes > 3072)");
+        if (a1 - a0 > IB_MAXACT) return fail(h, IPPI_ERR_ARG, "body support too large (active nodes > 6144)");
         if (bi == nb - 1) act0s.push_back(a1);
         for (int m = b.first; m < b.first + b.count; m++)
             for (int i = 0; i < 4; i++)
@@ -3546,6 +3546,33 @@ static std::vector<int4> pair_exclusions(const ippi_handle* h) {
     for (const auto& b : h->bodies)
         ex.push_back(make_int4(b.p.x0 / 2 - 2, b.p.y0 / 2 - 2, (b.p.x0 + b.p.w) / 2 + 2, (b.p.y0 + b.p.h) / 2 + 2));
     return ex;
+}
+
+// The current state restricted to the grid of half the resolution
+// (k_restrict): coarse populations fc [9][ny/2][nx/2] (skipped when null)
+// and rho, ux, uy [3][ny/2][nx/2].
+int ippi_restrict(ippi_handle* h, double tau_coarse, double* fc, double* mc) {
+    if (!h || !mc) return IPPI_ERR_ARG;
+    if (h->c.nx % 2 || h->c.ny % 2 || h->c.nx < 16 || h->c.ny < 16) return fail(h, IPPI_ERR_ARG, "grid must be even and >= 16");
+    if (h->c.bc == IPPI_BC_FLATPLATE)
+        return fail(h, IPPI_ERR_ARG, "the restriction is built for the periodic and inflow boundary kinds");
+    if (!(tau_coarse > 0.5)) return IPPI_ERR_ARG;
+    if (h->phase == 0) {
+        int r = ippi_step(h, 1);
+        if (r) return r;
+    }
+    const size_t Nc = (size_t)(h->c.nx / 2) * (h->c.ny / 2);
+    double *dfc = nullptr, *dmc = nullptr;
+    hipError_t e = hipMalloc(&dmc, sizeof(double) * 3 * Nc);
+    if (e == hipSuccess && fc) e = hipMalloc(&dfc, sizeof(double) * 9 * Nc);
+    if (e == hipSuccess) e = launch_restrict(h, 2.0 * tau_coarse / h->c.tau0, dfc, dmc);
+    if (e == hipSuccess && fc) e = hipMemcpyAsync(fc, dfc, sizeof(double) * 9 * Nc, hipMemcpyDeviceToHost, h->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(mc, dmc, sizeof(double) * 3 * Nc, hipMemcpyDeviceToHost, h->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+    if (dfc) (void)hipFree(dfc);
+    if (dmc) (void)hipFree(dmc);
+    if (e != hipSuccess) return fail(h, IPPI_ERR_HIP, std::string("ippi_restrict: ") + hipGetErrorString(e));
+    return check_flag(h);
 }
 
 // Training pairs at half this solver's resolution (see k_restrict / k_labels).

@@ -9,7 +9,9 @@
   non-overlapping positions by rejection sampling, each marker support
   (4x4 delta stencil) kept `gap` lattice units clear of every other body's
   support and of the domain edges;
-- `markers_for`: marker count for ~1 lu spacing on the circle.
+- `markers_for`: marker count for ~1 lu spacing on the circle;
+- `init_blob` / `bf16_weights`: the starting point of training on solver
+  pairs, and W1 / W2 rounded to bf16 after it.
 
 The reference's own cylinder markers follow mesh.cpp:180-181; its network
 is a libtorch model loaded per call (mesh.cpp:2767-3260) and has no
@@ -53,6 +55,32 @@ def xavier_blob(seed, b3=-5.0, mu=(1.0, 0.05, 0.0, 2e-3, 0.0, 0.0), sig=(2e-3, 0
     p[1313:1319] = mu
     p[1319:1325] = 1.0 / np.asarray(sig, dtype=np.float64)
     return p
+
+
+def init_blob(x, omega, seed):
+    """Starting point for training on features x [n, 6] and labels omega
+    [n]: Xavier-uniform W1, W2, W3 (numpy PCG64 seeded), mu / 1/sigma from
+    the features, and b3 placing softplus(b3) + 0.5 at the median label."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MLP_BLOB)
+    a1, a2 = math.sqrt(6 / 38), math.sqrt(6 / 64)
+    p[0:192] = rng.uniform(-a1, a1, 192)
+    p[224:1248] = rng.uniform(-a2, a2, 1024)
+    p[1280:1312] = rng.uniform(-0.3, 0.3, 32)
+    tau = 1.0 / float(np.median(omega))
+    p[1312] = math.log(math.expm1(max(tau - 0.5, 1e-6)))
+    p[1313:1319] = x.mean(0)
+    p[1319:1325] = 1.0 / np.maximum(x.std(0), 1e-30)
+    return p
+
+
+def bf16_weights(blob):
+    """The blob with W1 and W2 rounded to bf16 values (the FP32 path then
+    multiplies them exactly, two MFMAs per product)."""
+    b = np.array(blob, dtype=np.float64)
+    b[0:192] = round_bf16(b[0:192])
+    b[224:1248] = round_bf16(b[224:1248])
+    return b
 
 
 BNN_PARAMS = 15 * 128 + 128 + 6 * (128 * 128 + 128) + 128 + 1

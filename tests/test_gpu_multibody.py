@@ -180,3 +180,38 @@ def test_multibody_16_bodies_full_size(ippi):
     assert err.max() <= 1e-4
     assert ev <= FP32_U_TOL
     assert np.all(m64 > 0)  # every cylinder in the inflow is dragged downstream
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_large_body_separate_kernel_matches_oracle(oracle, ippi, precision):
+    """One cylinder of diameter 320 (1005 markers, about 4,100 support
+    nodes: more than 3,072, within the separate IB kernel's 6,144) on
+    768 x 448, RR + LES: the body of the 2x finer training run."""
+    nx, ny, D = 768, 448, 320.0
+    M = int(round(np.pi * D))
+    kw = dict(les=True, tau0=0.5077, cs=0.1, u_in=(0.08, 0.0))
+    o = oracle.Sim(nx, ny, "rr", "inflow", threads=16, **kw)
+    g = ippi.Solver(nx, ny, "rr", precision, "inflow", **kw)
+    for s in (o, g):
+        s.init_fields(5, 1.0, (0.08, 0.0), 1e-3)
+        s.add_body(384.0, 224.0, D, M)
+    steps = 40
+    print(f"large body D={D} markers {M} precision {precision} steps {steps}", flush=True)
+    assert o.step(steps) == 0
+    g.step(1)
+    g.set_timing(True)
+    g.step(steps - 1)
+    _, _, _, nib = g.timing()
+    g.set_timing(False)
+    assert nib == steps - 1  # the separate kernel ran
+    r0, r1 = o.fields(), g.fields()
+    d0, d1 = o.forces(), g.body_forces()
+    if precision == "fp64":
+        assert vel_err(r1, r0) <= FP64_TOL
+        assert rel_l2(d1[:, 0], d0[:, 0]) <= FP64_TOL
+    else:
+        assert vel_err(r1, r0) <= FP32_U_TOL
+        assert float(np.abs(d1[:, 0] - d0[:, 0]).max() / np.abs(d0[:, 0]).max()) <= 1e-3
+    print(f"FP{precision[2:]}: rel L2 u {vel_err(r1, r0):.3e}, drag record rel L2 {rel_l2(d1[:, 0], d0[:, 0]):.3e}")
+    g.close()
+    o.close()

@@ -213,73 +213,80 @@ def test_stress_pairs_exclude_body(oracle, ippi):
 
 def test_learned_tau_beats_fallbacks(ippi):
     """Pairs (stress matching) from a 2x finer double shear layer (RR + LES
-    Cs = 0.17, 256 x 256), GPU training, then coarse FP32 runs (128 x 128,
-    RR) from a restricted fine state with (a) constant tau, (b) hybrid
-    Smagorinsky Cs = 0.1, (c) the trained network with each fallback;
-    velocity error against the restricted fine solution after 300 coarse
-    steps. The network must beat both fallbacks on the run it was trained
-    on (a later state) and on two held-out flows (other perturbations); a
-    fallback run that diverges counts as infinite error."""
+    Cs = 0.17, 256 x 256), GPU training with the eddy-viscosity scale
+    calibrated on a validation window (training.calibrate), then coarse
+    FP32 runs (128 x 128, RR) from a restricted fine state with (a) constant
+    tau, (b) hybrid Smagorinsky Cs = 0.1, (c) the trained network with each
+    fallback; velocity error against the restricted fine solution after 300
+    coarse steps. The network must beat both fallbacks on a later window of
+    the run it was trained on and on two held-out flows (other
+    perturbations); a fallback run that diverges counts as infinite error."""
+    from ippi_amd import training
     nxc = nyc = 128
     U, nu_c = 0.08, 1e-4
     tau_c, tau_f = 0.5 + 3 * nu_c, 0.5 + 6 * nu_c
     nx, ny = 2 * nxc, 2 * nyc
-    ux, uy = shear_layer(nx, ny, U, 0.05, 1)
-    fine = ippi.Solver(nx, ny, "rr", "fp32", "periodic_bb", les=True, tau0=tau_f, cs=0.17)
-    fine.set_fields(np.ones_like(ux), ux, uy)
-    fine.step(3000)
-    X, Y, Wt = [], [], []
-    for _ in range(40):
-        f, om, w = fine.coarse_pairs(tau_c, criterion="stress")
-        ok = w > 0
-        X.append(f[ok])
-        Y.append(om[ok])
-        Wt.append(w[ok])
-        fine.step(18)
-    X, Y, Wt = np.concatenate(X), np.concatenate(Y), np.concatenate(Wt)
-    Wt = np.minimum(Wt / np.median(Wt), 10.0)
-    blob = random_blob(np.random.default_rng(1), X)
-    blob[1312] = np.log(np.expm1(1.0 / np.median(Y) - 0.5))
-    tr = ippi.Trainer(len(Y))
-    tr.set_data(X, Y, Wt)
-    tr.set_params(blob)
-    l0, _ = tr.loss_grad()
-    tr.adam(2000, 8192, 1e-3, seed=1)
-    l1, _ = tr.loss_grad()
-    blob = tr.params()
-    tr.close()
-    print(f"{len(Y)} pairs; training loss {l0:.3e} -> {l1:.3e}")
-    assert l1 < 0.5 * l0
-    fine.step(300)
-    flows = [("trained run, later state", fine)]
-    for amp, kx, ph in ((0.08, 2, 1.0), (0.03, 3, 2.0)):
-        ux, uy = shear_layer(nx, ny, U, amp, kx, ph)
-        h = ippi.Solver(nx, ny, "rr", "fp32", "periodic_bb", les=True, tau0=tau_f, cs=0.17)
-        h.set_fields(np.ones_like(ux), ux, uy)
-        h.step(3500)
-        flows.append((f"held out: amplitude {amp}, wavenumber {kx}", h))
-    for name, fl in flows:
-        fc, _, _, _, _ = restrict_np(fl.populations(), 2 * tau_c / tau_f)
-        fl.step(600)
-        _, _, xf, yf, _ = restrict_np(fl.populations(), 1.0)
-        fl.close()
 
-        def err(mode, les):
-            s = ippi.Solver(nxc, nyc, "rr", "fp32", "periodic_bb", les=les, tau_mode=mode, tau0=tau_c, cs=0.1)
-            s.set_populations(fc)
-            if mode == "nn":
-                s.set_mlp(blob)
-            try:
-                s.step(300)
-                _, a, b = s.fields()
-            except ippi.IppiError:
-                return float("inf")
-            finally:
-                s.close()
-            return float(np.sqrt(np.sum((a - xf) ** 2 + (b - yf) ** 2) / np.sum(xf ** 2 + yf ** 2)))
-        e = {"const": err("const", False), "les": err("const", True), "nn+const": err("nn", False),
-             "nn+les": err("nn", True)}
+    def fine_run(amp, kx, ph, steps):
+        ux, uy = shear_layer(nx, ny, U, amp, kx, ph)
+        f = ippi.Solver(nx, ny, "rr", "fp32", "periodic_bb", les=True, tau0=tau_f, cs=0.17)
+        f.set_fields(np.ones_like(ux), ux, uy)
+        f.step(steps)
+        return f
+
+    def err(fc, target, mode, les, blob=None, steps=300):
+        s = ippi.Solver(nxc, nyc, "rr", "fp32", "periodic_bb", les=les, tau_mode=mode, tau0=tau_c, cs=0.1)
+        s.set_populations(fc)
+        if blob is not None:
+            s.set_mlp(blob)
+        try:
+            s.step(steps)
+            return training.velocity_error(s.fields(), target)
+        except ippi.IppiError:
+            return float("inf")
+        finally:
+            s.close()
+
+    def window(f, coarse_steps):
+        fc, _ = f.restrict(tau_c)
+        f.step(2 * coarse_steps)
+        _, (_, xf, yf) = f.restrict(tau_c, populations=False)
+        return fc, (xf, yf)
+
+    fine = fine_run(0.05, 1, 0.0, 3000)
+    X, Y, W = training.collect_pairs(fine, tau_c, snaps=40, every=18)
+    fine.step(300)
+    fc_v, tgt_v = window(fine, 150)  # validation window: picks the eddy-viscosity scale
+    alpha, blob, errs = training.calibrate(ippi, X, Y, W, tau_c, lambda b: err(fc_v, tgt_v, "nn", True, b, 150),
+                                           alphas=(1, 2, 4, 8), seed=1, iters=2000)
+    print(f"{len(Y)} pairs; validation error per eddy scale {errs}; chosen {alpha}")
+    fine.step(300)
+    flows = [("trained run, later window", fine), ("held out: amplitude 0.08, wavenumber 2", fine_run(0.08, 2, 1.0, 3500)),
+             ("held out: amplitude 0.03, wavenumber 3", fine_run(0.03, 3, 2.0, 3500))]
+    for name, fl in flows:
+        fc, tgt = window(fl, 300)
+        fl.close()
+        e = {"const": err(fc, tgt, "const", False), "les": err(fc, tgt, "const", True),
+             "nn+const": err(fc, tgt, "nn", False, blob), "nn+les": err(fc, tgt, "nn", True, blob)}
         print(f"{name}: velocity error {e}; gain (fallback / learned): const {e['const'] / e['nn+const']:.3f}, "
               f"les {e['les'] / e['nn+les']:.3f}")
         assert np.isfinite(e["nn+const"]) and np.isfinite(e["nn+les"])
         assert e["nn+const"] < e["const"] and e["nn+les"] < e["les"]
+
+
+@pytest.mark.parametrize("prec,bc", [("fp64", "periodic_bb"), ("fp32", "inflow")])
+def test_restrict_matches_numpy(ippi, prec, bc):
+    nx, ny, tau_f, tau_c = 96, 64, 0.53, 0.515
+    g = ippi.Solver(nx, ny, "bgk", prec, bc, tau0=tau_f, u_in=(0.05, 0.0))
+    g.init_fields(8, 1.0, (0.05, 0.0), 1e-2)
+    g.step(20)
+    fc, (rho, ux, uy) = g.restrict(tau_c)
+    f2, r2, x2, y2, _ = restrict_np(g.populations(), 2 * tau_c / tau_f)
+    tol = 1e-13 if prec == "fp64" else 1e-7
+    np.testing.assert_allclose(fc, f2, rtol=tol, atol=tol)
+    np.testing.assert_allclose(rho, r2, rtol=tol)
+    np.testing.assert_allclose(ux, x2, rtol=0, atol=tol)
+    np.testing.assert_allclose(uy, y2, rtol=0, atol=tol)
+    _, m = g.restrict(tau_c, populations=False)
+    assert np.array_equal(m[1], ux)
+    g.close()
