@@ -2019,7 +2019,12 @@ __device__ __forceinline__ void wait_ib_forces(const StepArgs& a) {
     }
 }
 
-template <int COLL, int BC, int NN, bool NT>
+// NPX nodes per thread, BX columns apart: with the exact-weight network
+// (issue-bound, four waves per SIMD) two nodes' loads go out together and
+// their two networks run back to back on the data (2% faster at 4096 x
+// 2048; the compensated network spills and is 37% slower, so it keeps one).
+// Grid x: max(ceil(nx / (NPX BX)), bodies).
+template <int COLL, int BC, int NN, bool NT, int NPX>
 __global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? IPPI_NN_WAVES : 8)))
 k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const double* __restrict__ mac, StepArgs a) {
     // the IB block's velocities; the lattice blocks of the exact-weight
@@ -2034,19 +2039,29 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     }
     int bxi = blockIdx.x, g = (int)blockIdx.y - 1;
     if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, ny, bxi, g);
+    const int xs = bxi * BX * NPX;  // first column of this block
+    if (xs >= nx) return;           // block-uniform: the grid is widened to one block per body
     int y = g + a.rot;  // rot < ny
     if (y >= ny) y -= ny;
     if (y >= a.band0 && y < a.band1) wait_ib_forces(a);  // block-uniform
-    const bool active = (int)(bxi * BX + threadIdx.x) < nx;
-    const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny;
     u32x4* lds_frag = reinterpret_cast<u32x4*>(U);
     if (NN == 3) mlp_frage_to_lds<BX>(a.mlp, lds_frag);  // in flight with the population loads
-    float cur[9];
-    load_raw<float, BC, true>(src, x, y, nx, ny, N, cur, a.ref.bb);
+    float cur[NPX][9];
+#pragma unroll
+    for (int p = 0; p < NPX; p++) {
+        const int x = min(xs + p * BX + (int)threadIdx.x, nx - 1);
+        load_raw<float, BC, true>(src, x, y, nx, ny, N, cur[p], a.ref.bb);
+    }
     if (NN == 3) __syncthreads();
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
-    node_f32<COLL, BC, true, NN, NT, true>(cur, x, y, N, src, mac, dst, a, active, lds, bxi * BX, lds_frag);
+#pragma unroll
+    for (int p = 0; p < NPX; p++) {
+        const int x0 = xs + p * BX;
+        const bool active = x0 + (int)threadIdx.x < nx;
+        const int x = min(x0 + (int)threadIdx.x, nx - 1);
+        node_f32<COLL, BC, true, NN, NT, true>(cur[p], x, y, N, src, mac, dst, a, active, lds, x0, lds_frag);
+    }
 }
 
 // FP64 variant: four rows per thread as k_step_f64, the rotation and the
@@ -2514,6 +2529,8 @@ struct ippi_handle {
     bool ink_enabled = true;      // environment IPPI_IB_IN_STEP=0 turns it off
     bool ink_wait_skew = false;   // environment IPPI_IB_WAIT_SKEW=1: the band waits for a value never
                                   // stored, so its bounded wait times out (tests of that path only)
+    int npx_nn = 2;               // nodes per thread of the in-step kernel with the exact-weight
+                                  // network (environment IPPI_NN_NPX=1 for one)
     bool adj_overlap = true;      // environment IPPI_ADJ_OVERLAP=0: the adjoint recomputes each segment on
                                   // the handle's stream, between the sweeps (to time the sweep's kernels alone)
     bool round32 = false;         // environment IPPI_FP64_ROUND_STORE=1: the FP64 path rounds each stored
@@ -2614,12 +2631,12 @@ StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
     return nullptr;
 }
 
-template <int COLL, int BC>
+template <int COLL, int BC, int NPX>
 StepKernel<float> pick_step_ib_bc(int nn, bool nt) {
-    if (nn == 3) return nt ? k_step_f32_ib<COLL, BC, 3, true> : k_step_f32_ib<COLL, BC, 3, false>;
-    if (nn == 2) return nt ? k_step_f32_ib<COLL, BC, 2, true> : k_step_f32_ib<COLL, BC, 2, false>;
-    if (nn == 1) return nt ? k_step_f32_ib<COLL, BC, 1, true> : k_step_f32_ib<COLL, BC, 1, false>;
-    return nt ? k_step_f32_ib<COLL, BC, 0, true> : k_step_f32_ib<COLL, BC, 0, false>;
+    if (nn == 3) return nt ? k_step_f32_ib<COLL, BC, 3, true, NPX> : k_step_f32_ib<COLL, BC, 3, false, NPX>;
+    if (nn == 2) return nt ? k_step_f32_ib<COLL, BC, 2, true, NPX> : k_step_f32_ib<COLL, BC, 2, false, NPX>;
+    if (nn == 1) return nt ? k_step_f32_ib<COLL, BC, 1, true, NPX> : k_step_f32_ib<COLL, BC, 1, false, NPX>;
+    return nt ? k_step_f32_ib<COLL, BC, 0, true, 1> : k_step_f32_ib<COLL, BC, 0, false, 1>;
 }
 
 template <int COLL, int BC>
@@ -2641,10 +2658,11 @@ StepKernel<double> pick_step_ib64(int coll, int bc, bool nt) {
     return nullptr;
 }
 
-// FP32 pull step with the IB inside (periodic and inflow boundaries)
-StepKernel<float> pick_step_ib(int coll, int bc, int nn, bool nt) {
+// FP32 pull step with the IB inside (periodic and inflow boundaries); npx
+// nodes per thread (2 with the network, else 1)
+StepKernel<float> pick_step_ib(int coll, int bc, int nn, bool nt, int npx) {
 #define IPPI_PICK_IB(C, B) \
-    if (coll == C && bc == B) return pick_step_ib_bc<C, B>(nn, nt);
+    if (coll == C && bc == B) return npx == 2 ? pick_step_ib_bc<C, B, 2>(nn, nt) : pick_step_ib_bc<C, B, 1>(nn, nt);
     IPPI_PICK_IB(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
     IPPI_PICK_IB(IPPI_COLL_MRT, IPPI_BC_PERIODIC_BB)
     IPPI_PICK_IB(IPPI_COLL_RR, IPPI_BC_PERIODIC_BB)
@@ -3034,11 +3052,13 @@ int run_steps(ippi_handle* h, int nsteps) {
             const int at = (int)((nn ? INK_BAND_AT_NN : INK_BAND_AT) * ny);
             if (h->timing == 1) h->ev_step.push_back(new_event(h));
             if (sizeof(T) == 4) {
-                StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nk, a.nt);
+                const int npx = nk == 3 ? h->npx_nn : 1;  // the compensated network needs the registers
+                StepKernel<float> k = pick_step_ib(h->c.collision, h->c.bc, nk, a.nt, npx);
                 if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
                 ai.rot = ((h->band0 - at) % ny + ny) % ny;
                 ai.xcd = !a.nt;
-                hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, ny + 1), dim3(BX), 0, h->s,
+                const int gx = std::max((nx + npx * BX - 1) / (npx * BX), nb);
+                hipLaunchKernelGGL(k, dim3(gx, ny + 1), dim3(BX), 0, h->s,
                                    (const float*)h->buf[h->cur], (float*)h->buf[h->cur ^ 1], h->mac, ai);
             } else {
                 StepKernel<double> k = pick_step_ib64(h->c.collision, h->c.bc, a.nt);
@@ -3433,6 +3453,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (const char* e = std::getenv("IPPI_IB_WAIT_SKEW")) h->ink_wait_skew = std::strcmp(e, "1") == 0;
     if (const char* e = std::getenv("IPPI_FP64_ROUND_STORE")) h->round32 = std::strcmp(e, "1") == 0;
     if (const char* e = std::getenv("IPPI_ADJ_OVERLAP")) h->adj_overlap = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("IPPI_NN_NPX")) h->npx_nn = std::strcmp(e, "1") == 0 ? 1 : 2;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;
     // reference state of the FP32 storage: the free stream (u_in_x, 0)

@@ -110,3 +110,38 @@ def test_wait_time_out_is_reported(precision):
         s.step(2)  # the first collision, then one pull step with the IB inside
         with pytest.raises(ippi.IppiError, match="not stored in time"):
             s.fields()
+
+
+@pytest.mark.parametrize("nx,bodies", [
+    (768, [(100.2, 60.4, 20.0, 63), (360.7, 59.1, 24.0, 75), (620.3, 61.0, 20.0, 63)]),  # 3 bodies, 2 column blocks
+    (600, [(150.6, 60.2, 24.0, 75)]),  # ragged width: the second node of the last block is outside
+])
+def test_two_nodes_per_thread_bitwise(nx, bodies):
+    """The exact-weight network's in-step kernel takes two nodes per
+    thread, 256 columns apart (grid widened to one block per body when the
+    bodies outnumber the column blocks): fields, marker state and records
+    equal the one-node kernel (IPPI_NN_NPX=1) bit for bit."""
+    ippi = _pkg.load()
+    import ippi_amd.workloads as wl
+    out = []
+    for npx in ("2", "1"):
+        os.environ["IPPI_NN_NPX"] = npx
+        try:
+            s = ippi.Solver(nx, 120, "mrt", "fp32", "inflow", les=True, tau0=0.56, cs=0.12, u_in=(0.06, 0.0),
+                            tau_mode="nn")
+        finally:
+            del os.environ["IPPI_NN_NPX"]
+        with s:
+            s.init_fields(21, 1.0, (0.06, 0.0), 1e-3)
+            s.set_mlp(wl.xavier_blob(5))  # bf16 W1, W2: the exact-weight network
+            for (cx, cy, d, m) in bodies:
+                s.add_body(cx, cy, d, m)
+            s.step(3)
+            s.set_timing(True)
+            s.step(97)
+            _, nstep, _, nib = s.timing()
+            assert nstep == 97 and nib == 0  # the IB ran inside the step kernel
+            out.append((s.fields(), s.body_forces(), s.markers()))
+    (f2, d2, m2), (f1, d1, m1) = out
+    assert all(np.array_equal(a, b) for a, b in zip(f2, f1))
+    assert np.array_equal(d2, d1) and np.array_equal(m2, m1)
