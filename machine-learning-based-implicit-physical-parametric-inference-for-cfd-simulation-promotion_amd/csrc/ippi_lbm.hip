@@ -68,6 +68,7 @@ struct StepArgs {
     int nx, ny;
     RefState ref;           // FP32 storage offsets
     float duinxf;           // FP32 inlet velocity minus U
+    float qin[9];           // FP32 inlet equilibrium deviation feq(1, u_in) - c, computed once on the device
     double tau0, cs, uinx, uiny;
     float tx0f, csf, uinxf, uinyf;  // FP32 path: tx0f = tau0 - 1/2
     int les;
@@ -309,10 +310,22 @@ __device__ __forceinline__ void moments32(const float* g, float U, Mo& o) {
 
 // Non-equilibrium extrapolation in deviations: g = q(b) + (g - q(n)), the
 // node taking rho_b = 1 + drb and u_b = (U + dbx, dby).
-__device__ __forceinline__ void nee32(float* g, Mo& o, float drb, float dbx, float dby, const RefState& r) {
+// qbc: the boundary node's equilibrium deviation when the caller has it
+// (the inlet's, StepArgs::qin), else computed here; rho1: drb is 0 (the
+// same values without the c drb terms, which the compiler would otherwise
+// evaluate for every node ahead of the boundary branch)
+__device__ __forceinline__ void nee32(float* g, Mo& o, float drb, float dbx, float dby, const RefState& r,
+                                      const float* qbc = nullptr, bool rho1 = false) {
 #pragma clang fp contract(fast)
     float qb[9], qn[9];
-    sh::geq_dev(drb, 1.0f + drb, dbx, dby, r.Uf, r.cf, qb);
+    if (qbc) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) qb[k] = qbc[k];
+    } else if (rho1) {
+        sh::geq_dev0(dbx, dby, r.Uf, qb);
+    } else {
+        sh::geq_dev(drb, 1.0f + drb, dbx, dby, r.Uf, r.cf, qb);
+    }
     sh::geq_dev(o.drho, o.rho, o.dux, o.duy, r.Uf, r.cf, qn);
 #pragma unroll
     for (int k = 0; k < 9; k++) g[k] = qb[k] + (g[k] - qn[k]);
@@ -757,11 +770,14 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         moments32(g, U, o);
     }
     if (PULL && BC != IPPI_BC_FLATPLATE) {
-        if (BC == IPPI_BC_INFLOW && (x == 0 || x == nx - 1)) {  // NEE, mesh.cpp:239-286
+        // NEE, mesh.cpp:239-286. The outer test is block-uniform (only the
+        // first and last blocks of a row hold x = 0 or x = nx - 1), so the
+        // other blocks skip the inlet equilibrium altogether.
+        if (BC == IPPI_BC_INFLOW && (bx0 == 0 || bx0 + BX >= nx)) {
             if (x == 0)
-                nee32(g, o, 0.f, a.duinxf, a.uinyf, r);
-            else
-                nee32(g, o, 0.f, o.dux, o.duy, r);
+                nee32(g, o, 0.f, a.duinxf, a.uinyf, r, a.qin);
+            else if (x == nx - 1)
+                nee32(g, o, 0.f, o.dux, o.duy, r, nullptr, true);
         }
     } else if (!PULL) {
         o.rho = (float)mac[n];
@@ -785,7 +801,8 @@ __device__ __forceinline__ void node_f32(float* g, int x, int y, unsigned N, con
         }
     }
     // relaxation as tx = tau - 1/2 (see lbm_node.hpp)
-    const float txb = a.tau_field ? ld(static_cast<const float*>(a.tau_ptr), n) : a.tx0f;
+    float txb = a.tx0f;
+    if (a.tau_field) txb = ld(static_cast<const float*>(a.tau_ptr), n);  // uniform branch, no load otherwise
     float tx = txb;
     const float rho = o.rho, drho = o.drho, dux = o.dux, duy = o.duy;
     const float ux = U + dux, uy = duy;
@@ -2486,6 +2503,7 @@ __global__ void k_adj_flush(T* __restrict__ part, double* __restrict__ total, si
 struct ippi_handle {
     ippi_config c{};
     RefState ref{};  // FP32 storage offsets: s = f - feq(1, (u_in_x, 0))
+    float qin[9] = {0};  // FP32 inlet equilibrium deviation (k_qin: the step kernels' own arithmetic)
     size_t N = 0;
     size_t esz = 8;
     hipStream_t s = nullptr;
@@ -2673,12 +2691,21 @@ StepKernel<float> pick_step_ib(int coll, int bc, int nn, bool nt, int npx) {
     return nullptr;
 }
 
+// FP32 inlet equilibrium deviation feq(1, u_in) - c (nee32's boundary term)
+__global__ void k_qin(RefState r, float duinx, float uiny, float* q) {
+    if (threadIdx.x == 0) {
+        const float drb = 0.f;
+        sh::geq_dev(drb, 1.0f + drb, duinx, uiny, r.Uf, r.cf, q);
+    }
+}
+
 StepArgs make_args(ippi_handle* h) {
     StepArgs a{};
     a.nx = h->c.nx;
     a.ny = h->c.ny;
     a.ref = h->ref;
     a.duinxf = (float)(h->c.u_in_x - h->ref.U);
+    for (int k = 0; k < 9; k++) a.qin[k] = h->qin[k];
     a.tau0 = h->c.tau0;
     a.cs = h->c.cs;
     a.uinx = h->c.u_in_x;
@@ -3476,6 +3503,14 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (hipMalloc(&h->flag, sizeof(int)) != hipSuccess) return bail(IPPI_ERR_NOMEM);
     if (hipMalloc(&h->part, sizeof(double) * 1024) != hipSuccess) return bail(IPPI_ERR_NOMEM);
     if (hipMemset(h->flag, 0, sizeof(int)) != hipSuccess) return bail(IPPI_ERR_HIP);
+    // the inlet's FP32 equilibrium deviation, evaluated once by the device's
+    // arithmetic (the step kernels then read it from their arguments)
+    k_qin<<<1, 64, 0, h->s>>>(h->ref, (float)(cfg->u_in_x - h->ref.U), (float)cfg->u_in_y,
+                              reinterpret_cast<float*>(h->part));
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h->qin, h->part, sizeof(h->qin), hipMemcpyDeviceToHost, h->s) != hipSuccess ||
+        hipStreamSynchronize(h->s) != hipSuccess)
+        return bail(IPPI_ERR_HIP);
     *out = h;
     return IPPI_OK;
 }
@@ -3793,9 +3828,23 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
                 F.a1h[mb][j] = hi(w);
                 F.a1l[mb][j] = lo(w);
             }
-            for (int j = 0; j < 4; j++) {  // [W1, 0, 0, W1, 0, 0] for the exact-weight network
-                const int k = (4 * g + j) % 8;
-                F.a1e[mb][j] = (__bf16)(float)(k < MLP_NIN ? blob[OFF_W1 + o * MLP_NIN + k] : 0.0);
+            // [W1, p0, p1, W1, p2, 0] for the exact-weight network: the bias
+            // -L b1 as three bf16 parts (remainders taken in FP64, so their
+            // sum is -L b1 to ~2^-27 relative) against the activations' 1.0
+            // entries
+            double bp[3];
+            {
+                double r = -L * blob[OFF_B1 + o];
+                for (int q = 0; q < 3; q++) {
+                    bp[q] = (double)(float)(__bf16)(float)r;
+                    r -= bp[q];
+                }
+            }
+            for (int j = 0; j < 4; j++) {
+                const int k = 4 * g + j;
+                const double v = k == 6 ? bp[0] : k == 7 ? bp[1] : k == 14 ? bp[2]
+                                 : (k % 8) < MLP_NIN ? blob[OFF_W1 + o * MLP_NIN + k % 8] : 0.0;
+                F.a1e[mb][j] = (__bf16)(float)v;
             }
             for (int j = 0; j < 8; j++) {
                 const int k = (j / 4) * 16 + 4 * g + (j % 4);  // order of the layer-1 outputs in the lanes
@@ -3816,12 +3865,11 @@ int ippi_set_mlp(ippi_handle* h, const double* blob, int n) {
         struct {
             bf16x4 a1e[2];
             bf16x8 a2h[2];
-            float b1[8], b2[8], w3[8];
+            float b2[8], w3[8];
         } e;
         static_assert(sizeof(e) == 16 * MLP_FRAGE_CHUNKS, "fragment layout");
         memcpy(e.a1e, fr[l].a1e, sizeof(e.a1e));
         memcpy(e.a2h, fr[l].a2h, sizeof(e.a2h));
-        memcpy(e.b1, fr[l].b1, sizeof(e.b1));
         memcpy(e.b2, fr[l].b2, sizeof(e.b2));
         memcpy(e.w3, fr[l].w3, sizeof(e.w3));
         for (int c = 0; c < MLP_FRAGE_CHUNKS; c++) memcpy(&fe[c * 64 + l], reinterpret_cast<const char*>(&e) + 16 * c, 16);

@@ -335,6 +335,24 @@ IPPI_HD void geq_dev(float drho, float rho, float dux, float duy, float U, const
     one(6, -U, -bd);
 }
 
+// geq_dev at drho = 0, rho = 1: the same values (c_k 0 + x = x, w_k 1 = w_k)
+IPPI_HD void geq_dev0(float dux, float duy, float U, float* q) {
+#pragma clang fp contract(fast)
+    const float Q = dux * (2.0f * U + dux) + duy * duy;
+    const float hq = 1.5f * Q;
+    const float bs = dux + duy, bd = dux - duy;
+    auto one = [&](int k, float a, float b) { q[k] = WF[k] * (b * (3.0f + 4.5f * (2.0f * a + b)) - hq); };
+    one(0, 0.0f, 0.0f);
+    one(1, U, dux);
+    one(3, -U, -dux);
+    one(2, 0.0f, duy);
+    one(4, 0.0f, -duy);
+    one(5, U, bs);
+    one(7, -U, -bs);
+    one(8, U, bd);
+    one(6, -U, -bd);
+}
+
 // 2 - omega = 4 tx / (1 + 2 tx) for tau = 1/2 + tx: the hardware reciprocal
 // and one Newton correction of the quotient (within an ulp of the correctly
 // rounded division, in 4 instructions instead of 10)

@@ -28,7 +28,8 @@
 // bf16 value, the products need no lo part of the weights: a product is
 // W x_hi + W x_lo, exact in FP32 up to the 16-bit representation of x.
 // Layer 1 then takes one 16x16x16 MFMA per 16 outputs, its K holding
-// [x_hi(6), 0, 0, x_lo(6), 0, 0] against [W(6), 0, 0, W(6), 0, 0]; layer 2
+// [x_hi(6), 1, 1, x_lo(6), 1, 0] against [W(6), p0, p1, W(6), p2, 0], where
+// p0 + p1 + p2 are the bf16 parts of the scaled layer-1 bias; layer 2
 // two 16x16x32 MFMAs (h_hi, h_lo). 24 MFMAs per wave instead of 48. The
 // -log2(e) of the activations is folded into the feature scaling (isig)
 // and the biases, so W1 stays exact.
@@ -50,6 +51,7 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 // Per-lane fragments of the FP32/MFMA path (built on the host from the blob).
 // Constants are folded so that every hidden unit costs exp2 + add + rcp +
@@ -62,14 +64,15 @@ struct MlpFrag {
     bf16x4 a1h[2], a1l[2];  // layer-1 A tiles (two 16-output blocks), hi / lo
     bf16x8 a2h[2], a2l[2];  // layer-2 A tiles, columns in lane order
     float b1[8], b2[8], w3[8];  // scaled bias / output weight of this lane's 8 outputs
-    bf16x4 a1e[2];          // exact weights: layer-1 A tiles [W, W] (unscaled W1)
+    bf16x4 a1e[2];          // exact weights: layer-1 A tiles [W, b1 parts, W, b1 part, 0] (unscaled W1)
 };
-// Exact-weight network: the per-lane fragment (144 bytes) is read from LDS,
+// Exact-weight network: the per-lane fragment (112 bytes) is read from LDS,
 // where each workgroup copies it once (global layout chunk-major
 // [MLP_FRAGE_CHUNKS][64 lanes] of 16-byte chunks, so that a wave's reads of
-// one chunk are contiguous): chunk 0 = a1e[0..1], 1..2 = a2h[0..1] (exact
-// W2), 3..4 = b1, 5..6 = b2, 7..8 = w3, with the foldings of MlpFrag.
-constexpr int MLP_FRAGE_CHUNKS = 9;
+// one chunk are contiguous): chunk 0 = a1e[0..1] (W1 with the layer-1 bias
+// parts), 1..2 = a2h[0..1] (exact W2), 3..4 = b2, 5..6 = w3, with the
+// foldings of MlpFrag.
+constexpr int MLP_FRAGE_CHUNKS = 7;
 struct MlpDev {
     const MlpFrag* frag;  // [64 lanes]
     float mu[6], isig[6], b3;
@@ -208,7 +211,9 @@ __device__ __forceinline__ float mlp_tau_wave(const float* feat, const MlpDev& m
 
 // Network with bf16-exact W1 and W2 (see the header): same layouts and
 // folding as mlp_tau_wave, two MFMAs per product in layer 2 and one in
-// layer 1.
+// layer 1. The layer-1 bias enters the MFMA as three bf16 parts of the
+// weights against 1.0 entries in K slots 6, 7 and 14 of the activations
+// (so the accumulator starts at zero and b1 takes no registers).
 __device__ __forceinline__ float mlp_tau_wave_exact(const float* feat, const MlpDev& m, __bf16* lds,
                                                     const u32x4* lf) {
 #pragma clang fp contract(fast)
@@ -217,7 +222,7 @@ __device__ __forceinline__ float mlp_tau_wave_exact(const float* feat, const Mlp
     struct {
         bf16x4 a1e[2];
         bf16x8 a2h[2];
-        float b1[8], b2[8], w3[8];
+        float b2[8], w3[8];
     } fr;
     static_assert(sizeof(fr) == 16 * MLP_FRAGE_CHUNKS, "fragment layout");
     u32x4* fc = reinterpret_cast<u32x4*>(&fr);
@@ -229,20 +234,21 @@ __device__ __forceinline__ float mlp_tau_wave_exact(const float* feat, const Mlp
         const float a = (feat[k] - m.mu[k]) * m.isig_e[k], b = (feat[k + 1] - m.mu[k + 1]) * m.isig_e[k + 1];
         split2(a, b, vh[k / 2], vl[k / 2]);
     }
-    *reinterpret_cast<u32x4*>(lds + 16 * l) = (u32x4){vh[0], vh[1], vh[2], 0u};
-    *reinterpret_cast<u32x4*>(lds + 16 * l + 8) = (u32x4){vl[0], vl[1], vl[2], 0u};
+    // the node's K row: [x_hi(6), 1, 1, x_lo(6), 1, 0] (bf16 1.0 = 0x3f80)
+    *reinterpret_cast<u32x4*>(lds + 16 * l) = (u32x4){vh[0], vh[1], vh[2], 0x3f803f80u};
+    *reinterpret_cast<u32x4*>(lds + 16 * l + 8) = (u32x4){vl[0], vl[1], vl[2], 0x00003f80u};
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     float part[4];
 #pragma unroll
     for (int nb = 0; nb < 4; nb++) {
         const int node = nb * 16 + (l & 15);
-        // K rows 4g..4g+3 of [x_hi, 0, 0, x_lo, 0, 0]: the node's 16 bf16 in LDS as written above
+        // K rows 4g..4g+3 of the node's row as written above
         const bf16x4 bx = *reinterpret_cast<const bf16x4*>(lds + 16 * node + 4 * g);
         unsigned ph[4], pl[4];
 #pragma unroll
         for (int mb = 0; mb < 2; mb++) {
-            f32x4 c = {fr.b1[4 * mb], fr.b1[4 * mb + 1], fr.b1[4 * mb + 2], fr.b1[4 * mb + 3]};
+            f32x4 c = {0.f, 0.f, 0.f, 0.f};
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr.a1e[mb], bx, c, 0, 0, 0);
 #pragma unroll
             for (int r = 0; r < 4; r += 2) {

@@ -184,3 +184,33 @@ def test_nn_bf16_plain_products(oracle, ippi):
     assert errs[("nn", "fp32")] <= 2e-5
     assert errs[("nn", "fp32")] < errs[("nn_bf16", "fp32")] / 10
     assert errs[("nn_bf16", "fp64")] <= 1e-12
+
+
+def test_nn_fp32_long_horizon(ippi):
+    """FP32 (deviation storage, bf16-exact network) against FP64 (same
+    network) over 5,000 steps on MRT + LES + IB + network tau, 512 x 256,
+    cylinder D = 24, Re = 200: velocity within the 1e-5 bar after the whole
+    run, and at each of 1,000 / 2,000 / 5,000 steps."""
+    from ippi_amd import workloads
+    nx, ny, u0, d = 512, 256, 0.08, 24.0
+    tau0 = 0.5 + 3.0 * u0 * d / 200.0
+    blob = workloads.xavier_blob(7)
+    runs = {}
+    for prec in ("fp32", "fp64"):
+        s = ippi.Solver(nx, ny, "mrt", prec, "inflow", les=True, tau_mode="nn", tau0=tau0, cs=0.1, u_in=(u0, 0.0))
+        s.init_fields(11, 1.0, (u0, 0.0), 1e-3)
+        s.add_body(128.0, 128.0, d, workloads.markers_for(d))
+        s.set_mlp(blob)
+        runs[prec] = s
+    done = 0
+    for m in (1000, 2000, 5000):
+        print(f"long horizon 512x256: to step {m}", flush=True)
+        for s in runs.values():
+            s.step(m - done)
+        done = m
+        e = vel_err(runs["fp32"].fields(), runs["fp64"].fields())
+        print(f"  FP32 vs FP64 velocity rel L2 {e:.3e}", flush=True)
+        assert e <= 1e-5, (m, e)
+    assert np.mean(runs["fp64"].tau() > 0.501) > 0.99  # the network's values are the ones used
+    for s in runs.values():
+        s.close()
