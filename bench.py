@@ -17,6 +17,9 @@ Reported beside it:
   fp64_mlups        the plain FP64 path on the same grid and body (MRT + LES +
                     IB, no network), and speedup_vs_fp64 = value / it;
   fp64_nn_mlups     the FP64 path with the same network (evaluated in FP64);
+  plain_fp32        the FP32 path without the network on the same grid, with
+                    and without the body: MLUPS, fused-kernel time and its
+                    fraction of the HBM roofline (72 B/node);
   rel_l2_u_vs_fp64  FP32 against the FP64 path with the network after the
                     timed steps (same seeded start); accuracy_gate_met
                     uses the 10,000-step figure of the accuracy leg when
@@ -726,6 +729,20 @@ def main():
                  "drag_rel_err_vs_fp64": float(np.abs(d32[-n:, 0, 0] - d64[-n:, 0, 0]).mean() /
                                                np.abs(d64[-n:, 0, 0]).mean())}
         s64.close()
+    # the plain FP32 step (no network) on the same grid, with the body (IB inside the step kernel) and
+    # without it: MLUPS and the fused kernel's HBM fraction at 72 algorithmic bytes per node
+    plain = {}
+    for tag, body in (("with_body", True), ("no_body", False)):
+        sp = ippi.Solver(NX, NY, "mrt", "fp32", "inflow", les=True, tau0=TAU0, cs=CS, u_in=(U0, 0.0))
+        sp.init_fields(SEED, 1.0, (U0, 0.0), 1e-3)
+        if body:
+            sp.add_body(CX, CY, DIAM, NMARK)
+        log(f"fp32 plain MRT+LES{'+IB' if body else ''} {NX}x{NY}: warmup {W}, timed {K} steps")
+        dtq, kq = timed_run_events(sp, K, W)
+        sp.close()
+        plain[tag] = {"mlups": round(NX * NY * K / dtq / 1e6, 1), "kernel_avg_us": round(kq, 2) if kq else None,
+                      "roofline_frac": round(72 * NX * NY / (kq * 1e-6) / 8e12, 4) if kq else None}
+    extra["plain_fp32"] = plain
 
     # the roofline uses the timed steps' own launches (HIP events around the timed call, the kernels back
     # to back); per-launch events (each launch bracketed) are reported beside it

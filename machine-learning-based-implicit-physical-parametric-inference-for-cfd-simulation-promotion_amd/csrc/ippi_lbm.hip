@@ -121,6 +121,8 @@ struct StepArgs {
     int bnn_on;
     int rot;                // lattice row of grid row 1 (the rows follow cyclically)
     int xcd;                // XCD-grouped block order (xcd_block)
+    int ink_bodies;         // k_step_f32_ib / k_step_f64_ib: body blocks in grid row 0 (0: forces from k_ib)
+    const unsigned* rowmask;  // [ny] bit b set where body b's patch covers the row (nullptr: test every body)
     Patch patch[IPPI_MAXB];
 };
 
@@ -462,7 +464,12 @@ __device__ __forceinline__ bool fp_state32(const float* __restrict__ s, int x, i
 template <bool COH = false>
 __device__ __forceinline__ bool patch_force(const StepArgs& a, int x, int y, int bx0, int bx1, int by0, int by1,
                                             double& fx, double& fy) {
-    for (int b = 0; b < a.nbody; ++b) {
+    // the bodies whose patches cover the block's row (one scalar load), in increasing order
+    unsigned m = a.nbody >= 32 ? ~0u : (1u << a.nbody) - 1u;
+    if (a.rowmask && by0 == by1) m &= __builtin_amdgcn_readfirstlane(a.rowmask[by0]);
+    while (m) {
+        const int b = __builtin_ctz(m);
+        m &= m - 1u;
         const Patch p = a.patch[b];
         if (bx1 < p.x0 || bx0 >= p.x0 + p.w || by1 < p.y0 || by0 >= p.y0 + p.h) continue;
         if (x >= p.x0 && x < p.x0 + p.w && y >= p.y0 && y < p.y0 + p.h) {
@@ -2051,7 +2058,7 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     __shared__ __attribute__((aligned(16))) __bf16 lds_mlp[NN ? BX * 16 : 1];
     const int nx = a.nx, ny = a.ny;
     if (blockIdx.y == 0) {
-        if ((int)blockIdx.x < a.nbody) ib_in_step(a, blockIdx.x, src, U);
+        if ((int)blockIdx.x < a.ink_bodies) ib_in_step(a, blockIdx.x, src, U);
         return;  // block-uniform
     }
     int bxi = blockIdx.x, g = (int)blockIdx.y - 1;
@@ -2089,7 +2096,7 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
     __shared__ double U[2 * INK_MAXACT64];
     const int nx = a.nx, ny = a.ny;
     if (blockIdx.y == 0) {
-        if ((int)blockIdx.x < a.nbody) ib_in_step(a, blockIdx.x, src, U);
+        if ((int)blockIdx.x < a.ink_bodies) ib_in_step(a, blockIdx.x, src, U);
         return;  // block-uniform
     }
     const int G = (ny + NPT - 1) / NPT;
@@ -2541,6 +2548,7 @@ struct ippi_handle {
     // IB inside the FP32 step kernel (k_step_f32_ib)
     IbBlk* d_lead = nullptr;      // [body] lead block entry
     unsigned* d_ibflag = nullptr;  // [body] sequence number of the last stored forces
+    unsigned* d_rowmask = nullptr; // [ny] bodies whose patches cover each row
     unsigned ib_seq = 0;
     int band0 = 0, band1 = 0;     // rows that read marker forces (the patches)
     bool ink_ok = false;          // bodies fit the in-step IB
@@ -2739,6 +2747,8 @@ StepArgs make_args(ippi_handle* h) {
     a.bnn_on = h->bnn_on;
     a.mac = h->mac;
     for (size_t b = 0; b < h->bodies.size(); b++) a.patch[b] = h->bodies[b].p;
+    a.rowmask = h->d_rowmask;
+    a.ink_bodies = 0;
     return a;
 }
 
@@ -2752,6 +2762,7 @@ void free_ib(ippi_handle* h) {
     dfree(h->d_slot);
     dfree(h->d_pslot);
     dfree(h->d_lead);
+    dfree(h->d_rowmask);
     dfree(h->d_ibflag);
     h->ink_ok = false;
     h->slots_ok = false;
@@ -2894,6 +2905,14 @@ int upload_ib(ippi_handle* h) {
         }
         h->band0 = std::max(b0, 0);
         h->band1 = std::min(b1, h->c.ny);
+        // per-row body masks for the step kernels' patch test
+        std::vector<unsigned> rm(h->c.ny, 0u);
+        for (int bi = 0; bi < nb; bi++) {
+            const Patch& p = h->bodies[bi].p;
+            for (int y = std::max(p.y0, 0); y < std::min(p.y0 + p.h, h->c.ny); y++) rm[y] |= 1u << bi;
+        }
+        HIPC(h, hipMalloc(&h->d_rowmask, sizeof(unsigned) * rm.size()));
+        HIPC(h, hipMemcpy(h->d_rowmask, rm.data(), sizeof(unsigned) * rm.size(), hipMemcpyHostToDevice));
         // worth it only while the band is a minority of the rows
         h->ink_ok = fit && h->band1 > h->band0 && 2 * (h->band1 - h->band0) <= h->c.ny;
         if (h->ink_ok) {
@@ -3076,6 +3095,7 @@ int run_steps(ippi_handle* h, int nsteps) {
             ai.ib_wait = ai.ib_seq + (h->ink_wait_skew ? 1u : 0u);
             ai.band0 = h->band0;
             ai.band1 = h->band1;
+            ai.ink_bodies = nb;
             const int at = (int)((nn ? INK_BAND_AT_NN : INK_BAND_AT) * ny);
             if (h->timing == 1) h->ev_step.push_back(new_event(h));
             if (sizeof(T) == 4) {
@@ -3113,6 +3133,32 @@ int run_steps(ippi_handle* h, int nsteps) {
             if (r) return r;
             if (h->timing == 1) h->ev_ib.push_back(new_event(h));
             h->batch_ib++;
+        }
+        if constexpr (sizeof(T) == 4) {
+            // exact-weight network on a pull step without in-step bodies: the
+            // in-step kernel's two nodes per thread, with no body blocks (the
+            // forces, if any, come from the k_ib launch above)
+            StepKernel<float> k2 = pull && nk == 3 ? pick_step_ib(h->c.collision, h->c.bc, 3, a.nt, h->npx_nn) : nullptr;
+            if (k2) {
+                StepArgs ai = a;
+                ai.ink_bodies = 0;
+                ai.band0 = ai.band1 = 0;
+                ai.rot = 0;
+                ai.xcd = !a.nt;
+                const int gx = (nx + h->npx_nn * BX - 1) / (h->npx_nn * BX);
+                if (h->timing == 1) h->ev_step.push_back(new_event(h));
+                hipLaunchKernelGGL(k2, dim3(gx, ny + 1), dim3(BX), 0, h->s, (const float*)h->buf[h->cur],
+                                   (float*)h->buf[h->cur ^ 1], h->mac, ai);
+                HIPC(h, hipGetLastError());
+                if (h->timing == 1) h->ev_step.push_back(new_event(h));
+                h->batch_steps++;
+                h->slots_ok = false;
+                h->prev_pull = true;
+                h->cur ^= 1;
+                h->phase = 1;
+                h->steps++;
+                continue;
+            }
         }
         StepKernel<T> k = pull ? (nk == 3   ? pick_step<T, true, 3>(h->c.collision, h->c.bc, a.nt)
                                   : nk == 2 ? pick_step<T, true, 2>(h->c.collision, h->c.bc, a.nt)
