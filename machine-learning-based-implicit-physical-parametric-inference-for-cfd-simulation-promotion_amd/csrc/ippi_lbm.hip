@@ -121,7 +121,8 @@ struct StepArgs {
     int bnn_on;
     int rot;                // lattice row of grid row 1 (the rows follow cyclically)
     int xcd;                // XCD-grouped block order (xcd_block)
-    int ink_bodies;         // k_step_f32_ib / k_step_f64_ib: body blocks in grid row 0 (0: forces from k_ib)
+    int ink_bodies;         // k_step_f32_ib / k_step_f64_ib: body blocks (0: forces from k_ib)
+    int brows;              // grid rows of the body blocks before the lattice rows (the lattice's own width)
     const unsigned* rowmask;  // [ny] bit b set where body b's patch covers the row (nullptr: test every body)
     Patch patch[IPPI_MAXB];
 };
@@ -2047,7 +2048,9 @@ __device__ __forceinline__ void wait_ib_forces(const StepArgs& a) {
 // (issue-bound, four waves per SIMD) two nodes' loads go out together and
 // their two networks run back to back on the data (2% faster at 4096 x
 // 2048; the compensated network spills and is 37% slower, so it keeps one).
-// Grid x: max(ceil(nx / (NPX BX)), bodies).
+// Grid: ceil(nx / (NPX BX)) columns (the lattice's own width, so that the
+// block order's XCD grouping holds); the body blocks fill the first
+// a.brows rows, the lattice rows follow.
 template <int COLL, int BC, int NN, bool NT, int NPX>
 __global__ void __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(NN ? IPPI_NN_WAVES : 8)))
 k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const double* __restrict__ mac, StepArgs a) {
@@ -2057,14 +2060,14 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     static_assert(NN != 3 || sizeof(U) >= 16 * MLP_FRAGE_CHUNKS * 64, "LDS for the network's fragments");
     __shared__ __attribute__((aligned(16))) __bf16 lds_mlp[NN ? BX * 16 : 1];
     const int nx = a.nx, ny = a.ny;
-    if (blockIdx.y == 0) {
-        if ((int)blockIdx.x < a.ink_bodies) ib_in_step(a, blockIdx.x, src, U);
-        return;  // block-uniform
+    if ((int)blockIdx.y < a.brows) {  // the body blocks' rows (block-uniform)
+        const int j = blockIdx.y * gridDim.x + blockIdx.x;
+        if (j < a.ink_bodies) ib_in_step(a, j, src, U);
+        return;
     }
-    int bxi = blockIdx.x, g = (int)blockIdx.y - 1;
+    int bxi = blockIdx.x, g = (int)blockIdx.y - a.brows;
     if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, ny, bxi, g);
     const int xs = bxi * BX * NPX;  // first column of this block
-    if (xs >= nx) return;           // block-uniform: the grid is widened to one block per body
     int y = g + a.rot;  // rot < ny
     if (y >= ny) y -= ny;
     if (y >= a.band0 && y < a.band1) wait_ib_forces(a);  // block-uniform
@@ -2095,12 +2098,13 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
     constexpr int NPT = step_npt<double, true, false>();
     __shared__ double U[2 * INK_MAXACT64];
     const int nx = a.nx, ny = a.ny;
-    if (blockIdx.y == 0) {
-        if ((int)blockIdx.x < a.ink_bodies) ib_in_step(a, blockIdx.x, src, U);
-        return;  // block-uniform
+    if ((int)blockIdx.y < a.brows) {  // the body blocks' rows (block-uniform)
+        const int j = blockIdx.y * gridDim.x + blockIdx.x;
+        if (j < a.ink_bodies) ib_in_step(a, j, src, U);
+        return;
     }
     const int G = (ny + NPT - 1) / NPT;
-    int bxi = blockIdx.x, g = (int)blockIdx.y - 1;
+    int bxi = blockIdx.x, g = (int)blockIdx.y - a.brows;
     if (a.xcd) xcd_block(g * (int)gridDim.x + bxi, gridDim.x, G, bxi, g);
     int gi = g + a.rot;  // rot < G
     if (gi >= G) gi -= G;
@@ -2188,6 +2192,16 @@ struct AdjArgs {
     int nbody;
     int store;              // write the adjoint populations
     int xcd;                // XCD-grouped block order (xcd_block)
+    // k_adj_fused: the bodies' blocks publish the velocity adjoint with a
+    // per-body flag; lattice rows in [band0, band1) wait for the flags of
+    // the bodies rowmask names
+    const unsigned* rowmask;  // [ny] bit b: body b's patch covers the row
+    unsigned* flags;          // [body] sequence number of the last published step
+    unsigned* cnt;            // [body] blocks of the body done with the force adjoint this step
+    unsigned seq;             // this reverse step's sequence number
+    int band0, band1, rot;    // band rows; lattice row of the first lattice grid row (rows follow cyclically)
+    int brows;                // grid rows of the bodies' blocks before the lattice rows
+    int* err;                 // set to 2 when a wait times out
     Patch patch[IPPI_MAXB];
 };
 
@@ -2213,13 +2227,31 @@ struct AdjIb {
     const AdjBody* body;
     double* patch_l;      // [patch node][2]
     double* gc;           // [body][2] centre gradient (accumulated)
+    // k_adj_fused: the markers' velocity-adjoint weights (published for the
+    // lattice rows' gather), the pairs' centre-gradient terms, patch node ->
+    // active node (-1: none)
+    double* lu;           // [marker][2]
+    double* pc;           // [m][16][2]
+    const int* pslot;
 };
 
-__device__ __forceinline__ int adj_patch(const AdjArgs& a, int x, int y, int bx0, int bx1) {
+// agent-coherent load of a value another block of the same kernel stored
+// with an agent-scope atomic (the XCDs' L2s are not coherent with each other)
+__device__ __forceinline__ double ld_coh(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int adj_patch(const AdjArgs& a, int x, int y, int bx0, int bx1, int* body = nullptr) {
     for (int b = 0; b < a.nbody; ++b) {
         const Patch p = a.patch[b];
         if (bx1 < p.x0 || bx0 >= p.x0 + p.w || y < p.y0 || y >= p.y0 + p.h) continue;
-        if (x >= p.x0 && x < p.x0 + p.w) return p.off + (y - p.y0) * p.w + (x - p.x0);
+        if (x >= p.x0 && x < p.x0 + p.w) {
+            if (body) *body = b;
+            return p.off + (y - p.y0) * p.w + (x - p.x0);
+        }
     }
     return -1;
 }
@@ -2285,26 +2317,26 @@ __device__ __forceinline__ A adj_tx(const AdjArgs& a, unsigned n) {
 }
 // S: type of the stored forward populations, A: adjoint arithmetic and
 // storage.
+// Force adjoint of active node i (its velocity kept beside it): FP64
+// throughout (a few thousand nodes): the centre gradient sums the force
+// adjoint and the velocity against the delta's derivative, whose weights
+// sum to zero, so the rounding of FP32 arithmetic on these quantities would
+// not cancel with the rest.
 template <typename S, typename A, bool PULL>
-__global__ void __launch_bounds__(256) k_adj_act(const A* __restrict__ lin, const S* __restrict__ src,
-                                                 const double* __restrict__ mac, AdjIb d, AdjArgs a) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= d.nact) return;
+__device__ __forceinline__ void adj_act_vals(int i, const A* __restrict__ lin, const S* __restrict__ src,
+                                             const double* __restrict__ mac, const AdjIb& d, const AdjArgs& a,
+                                             double& lfx, double& lfy, double& ux, double& uy) {
     const int nx = a.nx, ny = a.ny;
     const unsigned N = (unsigned)nx * ny;
     const int node = d.act_node[i], r = d.act[i];
     const int x = node % nx, y = node / nx;
-    // FP64 throughout (a few thousand nodes): the centre gradient sums the
-    // force adjoint and the velocity against the delta's derivative, whose
-    // weights sum to zero, so the rounding of FP32 arithmetic on these
-    // quantities would not cancel with the rest
     A la[9];
     pull_adj(lin, x, y, nx, ny, N, la);
     const A c0 = lin[0];
     double l[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) l[k] = (double)(la[k] - c0);
-    double rho, ux, uy;
+    double rho;
     if (PULL) {
         S fs[9];
         load_raw<S, IPPI_BC_PERIODIC_BB, true>(src, x, y, nx, ny, N, fs, a.ref.bb);
@@ -2321,12 +2353,24 @@ __global__ void __launch_bounds__(256) k_adj_act(const A* __restrict__ lin, cons
                                                          adj_tx<S, double>(a, (unsigned)node));
     const adj::WMom<double> m = adj::wmoments(l);
     const adj::UsAdj<double> ua = adj::us_adjoint(m, s, true);
-    double lfx, lfy;
     adj::force_adjoint(ua, s, lfx, lfy);
+}
+template <typename S, typename A, bool PULL>
+__device__ __forceinline__ void adj_act_node(int i, const A* __restrict__ lin, const S* __restrict__ src,
+                                             const double* __restrict__ mac, const AdjIb& d, const AdjArgs& a) {
+    double lfx, lfy, ux, uy;
+    adj_act_vals<S, A, PULL>(i, lin, src, mac, d, a, lfx, lfy, ux, uy);
     d.lf[2 * i] = lfx;
     d.lf[2 * i + 1] = lfy;
     d.u[2 * i] = ux;
     d.u[2 * i + 1] = uy;
+}
+
+template <typename S, typename A, bool PULL>
+__global__ void __launch_bounds__(256) k_adj_act(const A* __restrict__ lin, const S* __restrict__ src,
+                                                 const double* __restrict__ mac, AdjIb d, AdjArgs a) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < d.nact) adj_act_node<S, A, PULL>(i, lin, src, mac, d, a);
 }
 
 // One block per body. cj = weight of this step's force record in the
@@ -2443,15 +2487,14 @@ __global__ void __launch_bounds__(IB_T) k_adj_mk(AdjIb d, double cj) {
     }
 }
 
-template <typename S, typename A, bool PULL>
-__global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* __restrict__ lout,
-                                                 const S* __restrict__ src, const double* __restrict__ mac,
-                                                 A* __restrict__ gacc, AdjArgs a) {
+// Reverse step at node (x, y) of all-node block column bxi. COH: the
+// velocity adjoint of the patch nodes was published by another block of the
+// same kernel (k_adj_fused) and is read with agent-coherent loads.
+template <typename S, typename A, bool PULL, bool COH>
+__device__ __forceinline__ void adj_node(const A* __restrict__ lin, A* __restrict__ lout, const S* __restrict__ src,
+                                         const double* __restrict__ mac, A* __restrict__ gacc, const AdjArgs& a,
+                                         int x, int y, int bxi, const AdjIb* dg = nullptr) {
     const int nx = a.nx, ny = a.ny;
-    int bxi = blockIdx.x, y = blockIdx.y;
-    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, bxi, y);
-    const int x = bxi * BX + threadIdx.x;
-    if (x >= nx) return;
     const unsigned N = (unsigned)nx * ny, n = (unsigned)y * nx + x;
     // arithmetic in FP64 whatever the adjoint's storage: with FP32 arithmetic
     // the rounding of each reverse step accumulates into the centre gradient,
@@ -2469,13 +2512,30 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
     adj_fwd<S, PULL>(src, mac, x, y, nx, ny, N, a.ref, f, drho, rho, ux, uy);
     C fx = C(0), fy = C(0), lix = C(0), liy = C(0);
     const int bx0 = bxi * BX;
-    const int r = a.nbody > 0 ? adj_patch(a, x, y, bx0, bx0 + BX - 1) : -1;
+    int body = 0;
+    const int r = a.nbody > 0 ? adj_patch(a, x, y, bx0, bx0 + BX - 1, &body) : -1;
     const bool forced = r >= 0;
     if (forced) {
         fx = (C)a.patch_f[2 * r];
         fy = (C)a.patch_f[2 * r + 1];
-        lix = (C)a.patch_l[2 * r];
-        liy = (C)a.patch_l[2 * r + 1];
+        if (COH) {  // k_adj_mk's gather, from the weights the body's blocks published
+            const int ia = dg->pslot[r];
+            if (ia >= 0) {
+                const int cnt = dg->ecnt[ia], m0 = a.patch[body].m0;
+                double sx = 0.0, sy = 0.0;
+                for (int e = 0; e < cnt; e++) {
+                    const int mk = m0 + dg->emk[(size_t)dg->E * ia + e];
+                    const double wg = dg->ecw[(size_t)dg->E * ia + e];
+                    sx += wg * ld_coh(dg->lu + 2 * mk);
+                    sy += wg * ld_coh(dg->lu + 2 * mk + 1);
+                }
+                lix = (C)sx;
+                liy = (C)sy;
+            }
+        } else {
+            lix = (C)a.patch_l[2 * r];
+            liy = (C)a.patch_l[2 * r + 1];
+        }
     }
     const adj::NodeFwd<C> s = adj::node_fwd<C>((C)rho, (C)ux, (C)uy, fx, fy, adj_tx<S, C>(a, n));
     const adj::WMom<C> m = adj::wmoments(l);
@@ -2492,6 +2552,168 @@ __global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* _
         for (int k = 0; k < 9; k++) lout[k * N + n] = (A)nu[k];
     }
     gacc[n] += (A)dt;
+}
+
+template <typename S, typename A, bool PULL>
+__global__ void __launch_bounds__(BX) k_adj_step(const A* __restrict__ lin, A* __restrict__ lout,
+                                                 const S* __restrict__ src, const double* __restrict__ mac,
+                                                 A* __restrict__ gacc, AdjArgs a) {
+    int bxi = blockIdx.x, y = blockIdx.y;
+    if (a.xcd) xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, bxi, y);
+    const int x = bxi * BX + threadIdx.x;
+    if (x >= a.nx) return;
+    adj_node<S, A, PULL, false>(lin, lout, src, mac, gacc, a, x, y, bxi);
+}
+
+// The bodies' part of k_adj_fused. A block takes BX / 16 markers of one
+// body, a thread one (marker, support) pair in k_adj_mk's numbering: it
+// computes the force adjoint and velocity at its support node itself
+// (k_adj_act's arithmetic; nodes shared by several markers are computed by
+// each), the marker's velocity-adjoint weight from the 16-lane sum of
+// k_adj_mk, and its pair's centre-gradient terms. Weights and terms are
+// stored with agent-scope atomics; after the block's stores are
+// acknowledged it counts itself done, and the body's last block sets the
+// body's flag for the lattice rows, then sums the centre gradient over the
+// pairs in k_adj_mk's lane, wave and block order (off the lattice rows'
+// path), so the results are bit for bit k_adj_act's and k_adj_mk's.
+constexpr int ADJ_MPB = BX / 16;  // markers per block
+static_assert(IB_T % BX == 0, "k_adj_mk lanes per thread");
+template <typename S, typename A, bool PULL>
+__device__ __forceinline__ void adj_body_block(const A* __restrict__ lin, const S* __restrict__ src,
+                                               const double* __restrict__ mac, const AdjIb& d, const AdjArgs& a,
+                                               double cj, double* red) {
+    int body = -1, mb0 = 0, nblk = 0, j = blockIdx.y * gridDim.x + blockIdx.x;
+    for (int b = 0; b < a.nbody; b++) {
+        const int n = (d.body[b].M + ADJ_MPB - 1) / ADJ_MPB;
+        if (j < n) {
+            body = b;
+            mb0 = j * ADJ_MPB;
+            nblk = n;
+            break;
+        }
+        j -= n;
+    }
+    if (body < 0) return;  // block-uniform
+    const AdjBody b = d.body[body];
+    const int t = threadIdx.x;
+    const int item = 16 * mb0 + t;  // the pair in k_adj_mk's numbering within the body
+    const bool ok = item < 16 * b.M;
+    const int qq = 16 * b.m0 + (ok ? item : 0);
+    const int i = b.a0 + d.pidx[qq];
+    double lfx, lfy, ux, uy;
+    adj_act_vals<S, A, PULL>(i, lin, src, mac, d, a, lfx, lfy, ux, uy);
+    const double w = ok ? d.w[qq] : 0.0, wxs = ok ? d.wx[qq] : 0.0, wys = ok ? d.wy[qq] : 0.0;
+    const int ml = min(item, 16 * b.M - 1) >> 4;
+    double ax = w * lfx, ay = w * lfy;
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+        ax += __shfl_xor(ax, o, 16);
+        ay += __shfl_xor(ay, o, 16);
+    }
+    // record = - sum of marker forces; F = 2 (U_d - U)
+    const double lux = -2.0 * (ax - cj * b.wx), luy = -2.0 * (ay - cj * b.wy);
+    const int m = b.m0 + ml;
+    if (ok && (t & 15) == 0) {
+        st_coh(d.lu + 2 * m, lux);
+        st_coh(d.lu + 2 * m + 1, luy);
+    }
+    const double Fx = d.F[2 * m], Fy = d.F[2 * m + 1];
+    const double cc = lux * ux + luy * uy + lfx * Fx + lfy * Fy;
+    if (ok) {
+        st_coh(d.pc + 2 * (size_t)qq, wxs * cc);
+        st_coh(d.pc + 2 * (size_t)qq + 1, wys * cc);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    __shared__ int last;
+    if (t == 0)
+        last = __hip_atomic_fetch_add(a.cnt + body, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+               (unsigned)(nblk - 1);
+    __syncthreads();
+    if (!last) return;  // block-uniform
+    if (t == 0) {
+        __hip_atomic_store(a.cnt + body, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(a.flags + body, a.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // centre gradient: lane L of k_adj_mk (thread t, L = q BX + t) sums the
+    // terms of its pairs L, L + IB_T, ... in order; then k_adj_mk's trees
+    constexpr int NV = IB_T / BX;
+    double gx[NV], gy[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        gx[q] = 0.0;
+        gy[q] = 0.0;
+        for (int it = q * BX + t; it < 16 * b.M; it += IB_T) {
+            gx[q] += ld_coh(d.pc + 2 * (size_t)(16 * b.m0 + it));
+            gy[q] += ld_coh(d.pc + 2 * (size_t)(16 * b.m0 + it) + 1);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            gx[q] += __shfl_down(gx[q], off, 64);
+            gy[q] += __shfl_down(gy[q], off, 64);
+        }
+        if ((t & 63) == 0) {  // k_adj_mk's wave (q BX + t) / 64
+            red[2 * ((q * BX + t) >> 6)] = gx[q];
+            red[2 * ((q * BX + t) >> 6) + 1] = gy[q];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        double sx = 0.0, sy = 0.0;
+        for (int wv = 0; wv < IB_T / 64; wv++) {
+            sx += red[2 * wv];
+            sy += red[2 * wv + 1];
+        }
+        d.gc[2 * body] += sx;
+        d.gc[2 * body + 1] += sy;
+    }
+}
+
+// One reverse step in one launch: the first grid rows hold the bodies'
+// blocks (adj_body_block), the lattice rows follow in rotated order with the
+// bodies' patch rows last; a patch row waits for the flags of the bodies
+// covering it (bounded: a time-out sets *err = 2 and the sweep reports the
+// failure, never hangs) and its forced nodes gather their velocity adjoint
+// from the published marker weights. Ordering as in the forward step's
+// in-step IB: weights, terms and flag are agent-scope atomics, s_waitcnt(0)
+// holds a block's count until every store of the block is acknowledged, and
+// readers load the weights only after a poll returned the flag. Bit for bit
+// the results of k_adj_act, k_adj_mk and k_adj_step in sequence.
+template <typename S, typename A, bool PULL>
+__global__ void __launch_bounds__(BX) k_adj_fused(const A* __restrict__ lin, A* __restrict__ lout,
+                                                  const S* __restrict__ src, const double* __restrict__ mac,
+                                                  A* __restrict__ gacc, AdjArgs a, AdjIb d, double cj) {
+    __shared__ double red[2 * (IB_T / 64)];
+    const int nx = a.nx, ny = a.ny;
+    if ((int)blockIdx.y < a.brows) {  // the bodies' rows of blocks (block-uniform)
+        adj_body_block<S, A, PULL>(lin, src, mac, d, a, cj, red);
+        return;
+    }
+    // the lattice's own grid width, so that each row of blocks stays on one
+    // XCD in the grouped order (the body rows shift every block's XCD alike)
+    const int gx = (nx + BX - 1) / BX;
+    int bxi = blockIdx.x, g = (int)blockIdx.y - a.brows;
+    if (a.xcd) xcd_block(g * gx + bxi, gx, ny, bxi, g);
+    int y = g + a.rot;
+    if (y >= ny) y -= ny;
+    if (y >= a.band0 && y < a.band1) {  // block-uniform: wait for the bodies covering the row
+        unsigned m = __builtin_amdgcn_readfirstlane(a.rowmask[y]);
+        while (m) {
+            const int b = __builtin_ctz(m);
+            m &= m - 1u;
+            int spins = 0;
+            while (__hip_atomic_load(a.flags + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.seq) {
+                if (++spins > INK_SPIN) {
+                    *a.err = 2;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+    const int x = bxi * BX + threadIdx.x;
+    if (x >= nx) return;
+    adj_node<S, A, PULL, true>(lin, lout, src, mac, gacc, a, x, y, bxi, &d);
 }
 
 // total += part; part = 0 (per-segment tau-gradient partial sums).
@@ -2559,6 +2781,8 @@ struct ippi_handle {
                                   // network (environment IPPI_NN_NPX=1 for one)
     bool adj_overlap = true;      // environment IPPI_ADJ_OVERLAP=0: the adjoint recomputes each segment on
                                   // the handle's stream, between the sweeps (to time the sweep's kernels alone)
+    bool adj_fused = true;        // environment IPPI_ADJ_FUSED=0: the reverse step as three kernels
+                                  // (k_adj_act, k_adj_mk, k_adj_step) instead of k_adj_fused
     bool round32 = false;         // environment IPPI_FP64_ROUND_STORE=1: the FP64 path rounds each stored
                                   // population as the FP32 path stores it (accuracy diagnostic)
     std::vector<int> h_act;      // active node -> patch node (host copy)
@@ -2620,7 +2844,7 @@ int check_flag(ippi_handle* h) {
     int v = 0;
     HIPC(h, hipMemcpyAsync(&v, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->s));
     HIPC(h, hipStreamSynchronize(h->s));
-    if (v == 2) return fail(h, IPPI_ERR_HIP, "in-step IB: the marker forces of a step were not stored in time");
+    if (v == 2) return fail(h, IPPI_ERR_HIP, "in-kernel IB: marker forces or velocity adjoint of a step not stored in time");
     if (v) return fail(h, IPPI_ERR_DIVERGED, "density left [0, 10] or became NaN (ErrorCatch)");
     return IPPI_OK;
 }
@@ -2749,6 +2973,7 @@ StepArgs make_args(ippi_handle* h) {
     for (size_t b = 0; b < h->bodies.size(); b++) a.patch[b] = h->bodies[b].p;
     a.rowmask = h->d_rowmask;
     a.ink_bodies = 0;
+    a.brows = 0;
     return a;
 }
 
@@ -3104,8 +3329,10 @@ int run_steps(ippi_handle* h, int nsteps) {
                 if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
                 ai.rot = ((h->band0 - at) % ny + ny) % ny;
                 ai.xcd = !a.nt;
-                const int gx = std::max((nx + npx * BX - 1) / (npx * BX), nb);
-                hipLaunchKernelGGL(k, dim3(gx, ny + 1), dim3(BX), 0, h->s,
+                // the lattice's width; the bodies take as many leading rows as they need
+                const int gx = (nx + npx * BX - 1) / (npx * BX);
+                ai.brows = (nb + gx - 1) / gx;
+                hipLaunchKernelGGL(k, dim3(gx, ny + ai.brows), dim3(BX), 0, h->s,
                                    (const float*)h->buf[h->cur], (float*)h->buf[h->cur ^ 1], h->mac, ai);
             } else {
                 StepKernel<double> k = pick_step_ib64(h->c.collision, h->c.bc, a.nt);
@@ -3114,7 +3341,9 @@ int run_steps(ippi_handle* h, int nsteps) {
                 const int G = (ny + NPT - 1) / NPT;
                 ai.rot = ((h->band0 / NPT - at / NPT) % G + G) % G;
                 ai.xcd = !a.nt;
-                hipLaunchKernelGGL(k, dim3((nx + BX - 1) / BX, G + 1), dim3(BX), 0, h->s,
+                const int gx = (nx + BX - 1) / BX;
+                ai.brows = (nb + gx - 1) / gx;
+                hipLaunchKernelGGL(k, dim3(gx, G + ai.brows), dim3(BX), 0, h->s,
                                    (const double*)h->buf[h->cur], (double*)h->buf[h->cur ^ 1], h->mac, ai);
             }
             HIPC(h, hipGetLastError());
@@ -3147,7 +3376,8 @@ int run_steps(ippi_handle* h, int nsteps) {
                 ai.xcd = !a.nt;
                 const int gx = (nx + h->npx_nn * BX - 1) / (h->npx_nn * BX);
                 if (h->timing == 1) h->ev_step.push_back(new_event(h));
-                hipLaunchKernelGGL(k2, dim3(gx, ny + 1), dim3(BX), 0, h->s, (const float*)h->buf[h->cur],
+                ai.brows = 0;
+                hipLaunchKernelGGL(k2, dim3(gx, ny), dim3(BX), 0, h->s, (const float*)h->buf[h->cur],
                                    (float*)h->buf[h->cur ^ 1], h->mac, ai);
                 HIPC(h, hipGetLastError());
                 if (h->timing == 1) h->ev_step.push_back(new_event(h));
@@ -3365,6 +3595,31 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     for (int b = 0; b < nb; b++) aa.patch[b] = h->bodies[b].p;
     AdjIb ad{h->nact, h->ell, h->d_act, dan, lf, uact, h->d_pidx, h->d_w, dwx, dwy, nullptr,
              h->d_ecnt, h->d_emk, h->d_ecw, dtab, pl, gc};
+    // one launch per reverse step (k_adj_fused) unless IPPI_ADJ_FUSED=0: the
+    // bodies' blocks publish through per-body flags, sequence numbers from 1
+    bool fused = h->adj_fused && nb > 0 && h->d_rowmask;
+    for (int b = 0; b < nb; b++) fused = fused && tab[b].M > 0;  // every body publishes its flag
+    unsigned* flags = nullptr;
+    unsigned seq = 0;
+    int nab = 0;  // row-0 blocks: ceil(markers / ADJ_MPB) per body
+    for (int b = 0; b < nb; b++) nab += (tab[b].M + ADJ_MPB - 1) / ADJ_MPB;
+    if (fused) {
+        HIPC(h, mem.alloc(&flags, sizeof(unsigned) * 2 * nb));
+        HIPC(h, hipMemsetAsync(flags, 0, sizeof(unsigned) * 2 * nb, h->s));
+        HIPC(h, mem.alloc(&ad.lu, sizeof(double) * 2 * (size_t)nm));
+        HIPC(h, mem.alloc(&ad.pc, sizeof(double) * 32 * (size_t)nm));
+        ad.pslot = h->d_pslot;
+        aa.cnt = flags + nb;
+        aa.rowmask = h->d_rowmask;
+        aa.flags = flags;
+        aa.err = h->flag;
+        aa.band0 = h->band0;
+        aa.band1 = h->band1;
+        aa.rot = h->band1 % ny;  // the patch rows last in the order
+    }
+    const int gxa = (nx + BX - 1) / BX;
+    aa.brows = (nab + gxa - 1) / gxa;
+    const dim3 grid_fused(gxa, ny + aa.brows);
     constexpr int npt = step_npt<T, true, false>();
     const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const dim3 grid_adj((nx + BX - 1) / BX, ny), grid_act((h->nact + 255) / 256);
@@ -3475,6 +3730,19 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
             aa.patch_f = pfb + 2 * (size_t)h->npn * i;
             aa.store = t > 0;
             ad.F = Fsb + 2 * (size_t)nm * i;
+            if (fused) {  // one launch: the bodies' blocks first, the lattice rows after
+                aa.seq = ++seq;
+                if (h->timing) h->ev_adj.push_back(new_event(h));
+                if (pull)
+                    hipLaunchKernelGGL((k_adj_fused<T, A, true>), grid_fused, dim3(BX), 0, h->s, lin, lout,
+                                       state(sg, i), h->mac, gacc, aa, ad, cj);
+                else
+                    hipLaunchKernelGGL((k_adj_fused<T, A, false>), grid_fused, dim3(BX), 0, h->s, lin, lout,
+                                       state(sg, i), h->mac, gacc, aa, ad, cj);
+                if (h->timing) h->ev_adj.push_back(new_event(h));
+                cl ^= 1;
+                continue;
+            }
             if (pull)
                 hipLaunchKernelGGL((k_adj_act<T, A, true>), grid_act, dim3(256), 0, h->s, lin, state(sg, i), h->mac,
                                    ad, aa);
@@ -3498,8 +3766,7 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     }
     if (g_tau) HIPC(h, hipMemcpyAsync(g_tau, gtot, sizeof(double) * N, hipMemcpyDeviceToHost, h->s));
     if (g_c) HIPC(h, hipMemcpyAsync(g_c, gc, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, h->s));
-    HIPC(h, hipStreamSynchronize(h->s));
-    return IPPI_OK;
+    return check_flag(h);  // synchronises; reports a k_adj_fused wait that timed out
 }
 
 }  // namespace
@@ -3526,6 +3793,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (const char* e = std::getenv("IPPI_IB_WAIT_SKEW")) h->ink_wait_skew = std::strcmp(e, "1") == 0;
     if (const char* e = std::getenv("IPPI_FP64_ROUND_STORE")) h->round32 = std::strcmp(e, "1") == 0;
     if (const char* e = std::getenv("IPPI_ADJ_OVERLAP")) h->adj_overlap = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("IPPI_ADJ_FUSED")) h->adj_fused = std::strcmp(e, "0") != 0;
     if (const char* e = std::getenv("IPPI_NN_NPX")) h->npx_nn = std::strcmp(e, "1") == 0 ? 1 : 2;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;

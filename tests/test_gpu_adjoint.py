@@ -87,6 +87,50 @@ def test_recompute_overlap_is_bitwise_neutral(ippi, monkeypatch, precision, ckpt
     assert np.array_equal(on[1], off[1]) and np.array_equal(on[2], off[2])
 
 
+@pytest.mark.parametrize("precision,ckpt", [("fp64", 7), ("fp32", 4), ("fp32", 1)])
+def test_fused_reverse_step_is_bitwise_neutral(ippi, monkeypatch, precision, ckpt):
+    """One launch per reverse step (k_adj_fused, the default: the body's
+    block computes the force adjoint and the marker transpose and publishes
+    the velocity adjoint to the lattice rows of the same launch) against the
+    three kernels in sequence (IPPI_ADJ_FUSED=0): bit-identical objective
+    and gradients."""
+    print(f"[adjoint] fused reverse step {precision} ckpt={ckpt}", flush=True)
+    tau = small_tau()
+    fused = small_gpu(ippi, precision, tau, 60, 31, ckpt)
+    monkeypatch.setenv("IPPI_ADJ_FUSED", "0")
+    split = small_gpu(ippi, precision, tau, 60, 31, ckpt)
+    monkeypatch.delenv("IPPI_ADJ_FUSED")
+    assert fused[0] == split[0]
+    assert np.array_equal(fused[1], split[1]) and np.array_equal(fused[2], split[2])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_reverse_step_two_bodies(ippi, monkeypatch, precision):
+    """Two bodies (two body blocks, patch rows of each waiting on their own
+    body's flag; the objective weights both records): fused and three-kernel
+    sweeps bit-identical."""
+    print(f"[adjoint] fused reverse step, two bodies, {precision}", flush=True)
+    nx, ny, nsteps, avg_from = 96, 48, 50, 21
+    tau = 0.6 + np.random.default_rng(3).uniform(-0.02, 0.02, (ny, nx))
+    bw = np.array([1.0, 0.0, 0.5, 0.25])
+
+    def run():
+        with ippi.Solver(nx, ny, "bgk", precision, "periodic_bb", tau_mode="field", tau0=0.6,
+                         force_history=nsteps) as s:
+            s.set_tau_field(tau)
+            s.init_fields(8, 1.0, (0.05, 0.003), 1e-3)
+            s.add_body(24.3, 15.6, 10.0, 31)
+            s.add_body(66.7, 31.2, 12.0, 38)
+            return s.adjoint_drag(nsteps, avg_from, 10, body_w=bw)
+    fused = run()
+    monkeypatch.setenv("IPPI_ADJ_FUSED", "0")
+    split = run()
+    monkeypatch.delenv("IPPI_ADJ_FUSED")
+    assert fused[0] == split[0]
+    assert np.array_equal(fused[1], split[1]) and np.array_equal(fused[2], split[2])
+    assert np.all(np.abs(fused[2]) > 0)  # both centres carry a gradient
+
+
 def test_fp32_adjoint_small(ippi, oracle):
     print("[adjoint] fp32 vs oracle, small", flush=True)
     tau = small_tau()
@@ -210,6 +254,18 @@ def test_c4_recompute_overlap_is_bitwise_neutral(c4, monkeypatch):
     on = c4["r32"]
     assert on[0] == off[0]
     assert np.array_equal(on[1], off[1]) and np.array_equal(on[2], off[2])
+
+
+def test_c4_fused_reverse_step_is_bitwise_neutral(c4, monkeypatch):
+    """At the full C4 size in FP32: the fused reverse step against the three
+    kernels in sequence (IPPI_ADJ_FUSED=0), bit for bit."""
+    monkeypatch.setenv("IPPI_ADJ_FUSED", "0")
+    print("[adjoint] C4 fp32 adjoint, three kernels per reverse step", flush=True)
+    split = c4_case(None, "fp32").gradient(c4["tau"], c4["center"])
+    monkeypatch.delenv("IPPI_ADJ_FUSED")
+    fused = c4["r32"]
+    assert fused[0] == split[0]
+    assert np.array_equal(fused[1], split[1]) and np.array_equal(fused[2], split[2])
 
 
 def test_autograd_function(ippi):
