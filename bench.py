@@ -397,7 +397,7 @@ def leg_c3(ippi, steps=10000, val_steps=500, gain_steps=1000, pre=6000, snaps=16
     return out
 
 
-def leg_mrt_les_ib(ippi, steps, warmup, profile_steps, marks=(1000, 2000, 5000, 10000, 20000)):
+def leg_mrt_les_ib(ippi, steps, warmup, profile_steps, err_steps=20000):
     """MRT + LES + IB without the network on 2048 x 1024 (D = 80, Re = 1e4):
     FP32 and FP64 MLUPS, the fused kernel's roofline, and the FP32-vs-FP64
     velocity error over 20,000 steps beside FP64 over FP32-rounded storage
@@ -422,6 +422,9 @@ def leg_mrt_les_ib(ippi, steps, warmup, profile_steps, marks=(1000, 2000, 5000, 
                         "fused_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4)})
         s.close()
     out["speedup_vs_fp64"] = round(out["mlups_fp32"] / out["mlups_fp64"], 3)
+    marks = [m for m in (1000, 2000, 5000, 10000, 20000) if m <= err_steps]
+    if not marks:
+        return out
     runs = {"fp32": make("fp32"), "fp64": make("fp64"), "fp64_perturbed": make("fp64", 1e-3 * (1 + 1e-6))}
     os.environ["IPPI_FP64_ROUND_STORE"] = "1"  # read when the handle is created
     try:
@@ -657,6 +660,8 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=10000, help="steps of the C3 run (FP32 vs FP64)")
     ap.add_argument("--acc-steps", type=int, default=10000, help="steps of the headline accuracy leg")
     ap.add_argument("--mrtles-steps", type=int, default=2000)
+    ap.add_argument("--mrtles-err-steps", type=int, default=20000,
+                    help="last step of the MRT+LES+IB leg's FP32-vs-FP64 error run (0: none)")
     ap.add_argument("--mb-steps", type=int, default=5000)
     ap.add_argument("--profile-steps", type=int, default=200,
                     help="steps of the per-launch event-timed pass reported beside the roofline figure")
@@ -688,7 +693,8 @@ def main():
     if "c3" in names:
         legs["c3_rr_trained_nn_4096x2048"] = leg_c3(ippi, args.c3_steps)
     if "mrtles" in names:
-        legs["mrt_les_ib_2048x1024"] = leg_mrt_les_ib(ippi, args.mrtles_steps, W, args.profile_steps)
+        legs["mrt_les_ib_2048x1024"] = leg_mrt_les_ib(ippi, args.mrtles_steps, W, args.profile_steps,
+                                                       args.mrtles_err_steps)
     if "adjoint" in names:
         legs["adjoint_1024x512"] = leg_adjoint(ippi)
     if "multibody" in names:
