@@ -663,6 +663,8 @@ def main():
     ap.add_argument("--mrtles-err-steps", type=int, default=20000,
                     help="last step of the MRT+LES+IB leg's FP32-vs-FP64 error run (0: none)")
     ap.add_argument("--mb-steps", type=int, default=5000)
+    ap.add_argument("--plate-steps", type=int, default=5000,
+                    help="steps of the flat-plate leg (its FP32-vs-FP64 errors at 1,000 / 5,000 when reached)")
     ap.add_argument("--profile-steps", type=int, default=200,
                     help="steps of the per-launch event-timed pass reported beside the roofline figure")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="MB",
@@ -700,7 +702,8 @@ def main():
     if "multibody" in names:
         legs["multibody_8192x4096"] = leg_multibody(ippi, args.mb_steps, W)
     if "flatplate" in names:
-        legs["flatplate_2401x501"] = leg_flatplate(ippi)
+        legs["flatplate_2401x501"] = leg_flatplate(ippi, args.plate_steps,
+                                                   tuple(m for m in (1000, 5000) if m <= args.plate_steps))
     if "tausearch" in names:
         legs["tausearch"] = leg_tausearch(ippi)
 

@@ -1520,14 +1520,31 @@ constexpr int BNN_NPARAM = BNN_NIN * BNN_NH + BNN_NH + BNN_NHID * (BNN_NH * BNN_
 constexpr int BNN_T = 2 * BNN_NH, BNN_NODES = 16, BNN_HALF = BNN_NODES / 2, BNN_KC = 64;
 
 struct BnnNode {
-    int x, y, slot, t;  // node, index into the tau arrays, type index (0: T01, 1: T03, 2: T05, 3: T07)
+    int x, y, slot, t;  // node, index into the tau arrays (-1: padding), type index (0: T01, 1: T03, 2: T05, 3: T07)
 };
+// The node list holds the set types one after the other, each padded to a
+// multiple of 16 entries (copies of its last node with slot -1), so a tile
+// of 16 nodes has one type.
+constexpr int BNN_PAD = 16;
 struct BnnDev {
     const BnnNode* nodes;
     int nnodes;
-    const float* w[4];  // per type, device layout (input-major), nullptr when not set
-    double* tau;        // [2 nx + 2 ny]
+    const float* w[4];   // per type, device layout (input-major), nullptr when not set
+    double* tau;         // [2 nx + 2 ny]
+    int tend[4];         // end of each type's entries in the list (types not set: empty ranges)
+    const float* wm[4];  // per type, the MFMA kernel's layout (bnm_layout), nullptr when not set
 };
+// The MFMA kernel's copy of a network, ordered so that each 16-byte load of
+// a wave reads 1 KB in one piece. Lane l = 16 kq + r of wave w takes the
+// weights of output 16 w + r and inputs 4 j + kq. Layer 0: [w 8][l 64][j 4]
+// = W0[4j + kq][16 w + r] (zero for input 15), then its 128 biases; hidden
+// layer h (1..6) at BNM_OFF_H1 + (h - 1) BNM_LAYER_H: [w 8][q 8][l 64][e 4]
+// = Wh[4 (4q + e) + kq][16 w + r], then its 128 biases; the output row and
+// bias (129) at BNM_OFF_OUT.
+constexpr int BNM_OFF_B0 = 8 * 4 * 16 * 4, BNM_LAYER_H = 8 * 4 * 16 * 32 + BNN_NH;
+constexpr int BNM_OFF_H1 = BNM_OFF_B0 + BNN_NH, BNM_OFF_OUT = BNM_OFF_H1 + BNN_NHID * BNM_LAYER_H;
+constexpr int BNM_NPARAM = BNM_OFF_OUT + BNN_NH + 1;
+static_assert(BNM_OFF_H1 % 4 == 0 && BNM_LAYER_H % 4 == 0, "16-byte loads of the MFMA layout");
 
 // neighbour offsets in the reference's input order (mesh.cpp:3149-3165,
 // 2788-2804, 3216-3232, 2854-2870)
@@ -1623,13 +1640,147 @@ __global__ void __launch_bounds__(BNN_T) k_bnn(const T* __restrict__ src, const 
             off += BNN_NH * BNN_NH + BNN_NH;
         }
         __syncthreads();
-        if (t < BNN_NODES && typ[t] == tp && i0 + t < d.nnodes) {
+        if (t < BNN_NODES && typ[t] == tp && i0 + t < d.nnodes && d.nodes[i0 + t].slot >= 0) {
             const float* Wl = W + off;
             float r = Wl[BNN_NH];
             for (int k = 0; k < BNN_NH; k++) r = __builtin_fmaf(Wl[k], act[cur][k][t], r);
             const double tau = (double)r;
             d.tau[d.nodes[i0 + t].slot] = tau >= 0.501 ? tau : 0.0;
         }
+    }
+}
+
+// The same networks on FP32 MFMA tiles (v_mfma_f32_16x16x4_f32), for FP32
+// and FP64 handles: a block of 8 waves serves one tile of 16 nodes of one
+// type; wave w computes outputs 16w..16w+15 of every hidden layer, with
+// FP32 products and FP32 accumulation, the bias as the accumulator's start.
+// Lane l of an MFMA supplies A[l%16][l/16] (the weight of output 16w + l%16
+// and input 4j + l/16, read from L2), B[l/16][l%16] (the input activation
+// of node l%16, from LDS) and receives D[4(l/16)+i][l%16]. Activations
+// ping-pong between two LDS arrays [128][16]; the output layer is a
+// sequential FMA chain per node. An FP32 MFMA adds its four products to the
+// accumulator one after the other with a rounding each, so a chain of
+// MFMAs over k = 0, 4, 8, ... from the bias is k_bnn's FMA chain: the
+// outputs are bit for bit k_bnn's, and the oracle's (asserted in the
+// tests). The block's type follows from its index, so its first weights
+// are requested before the neighbour gather; each wave holds its 16
+// outputs' weights and biases of the next layer in registers, loaded at the
+// start of the current layer (the scheduling fence keeps the compiler from
+// sinking those loads to the layer's end). One tile per block: the ~5,800
+// edge nodes of a 2401 x 501 lattice make 363 blocks for the 256 CUs
+// (64-node blocks left 165 idle); at 124 VGPRs two blocks share a CU, so
+// one block's gather and barriers overlap the other's MFMAs.
+constexpr int BNM_T = 512, BNM_NODES = BNN_PAD, BNM_LD = BNM_NODES + 1;  // activation rows padded: no bank conflicts
+typedef float f32x4b __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void bnm_ld4(const float* __restrict__ p, float* v) {  // one 16-byte load
+    const f32x4b q = *reinterpret_cast<const f32x4b*>(p);
+#pragma unroll
+    for (int e = 0; e < 4; e++) v[e] = q[e];
+}
+// One hidden layer: reads X[P ^ 1], writes X[P]; its A operands and biases
+// in av, bv, which then take the next layer's from Wn (if not null).
+template <int P>
+__device__ __forceinline__ void bnm_hidden(float (*X)[BNN_NH][BNM_LD], float (&av)[BNN_NH / 4], float (&bv)[4],
+                                           const float* __restrict__ Wn, int w, int l, int bofs) {
+    const int kq = l >> 4;
+    f32x4b c;
+#pragma unroll
+    for (int i = 0; i < 4; i++) c[i] = bv[i];
+    float avc[BNN_NH / 4];
+#pragma unroll
+    for (int j = 0; j < BNN_NH / 4; j++) avc[j] = av[j];
+    if (Wn) {  // block-uniform
+#pragma unroll
+        for (int q = 0; q < BNN_NH / 16; q++) bnm_ld4(Wn + 4 * ((8 * w + q) * 64 + l), av + 4 * q);
+        bnm_ld4(Wn + BNM_LAYER_H - BNN_NH + bofs, bv);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the loads stay here, a layer ahead of their MFMAs
+    // the B operands of 16 k-steps read together, then their MFMAs
+#pragma unroll
+    for (int j0 = 0; j0 < BNN_NH / 4; j0 += 16) {
+        float b[16];
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) b[jj] = X[P ^ 1][4 * (j0 + jj) + kq][l & 15];
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(avc[j0 + jj], b[jj], c, 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);  // LDS reads
+        __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);  // then the MFMAs
+    }
+    // ReLU: outputs 16 w + 4 kq + i of node l % 16
+#pragma unroll
+    for (int i = 0; i < 4; i++) X[P][bofs + i][l & 15] = fmaxf(c[i], 0.0f);
+    __syncthreads();  // this layer's outputs written and its inputs read by every wave
+}
+
+template <typename T, int BC>
+__global__ void __launch_bounds__(BNM_T) k_bnn_mfma(const T* __restrict__ src, const double* __restrict__ mac,
+                                                   const double* __restrict__ corner, int pull, StepArgs a,
+                                                   BnnDev d) {
+    __shared__ __attribute__((aligned(16))) float xin[16][BNM_LD];       // layer-0 inputs (row 15: zero)
+    __shared__ __attribute__((aligned(16))) float X[2][BNN_NH][BNM_LD];  // activations [k][node]
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int i0 = blockIdx.x * BNM_NODES;
+    const int nx = a.nx, ny = a.ny;
+    int tp = 0;  // the tile's type (block-uniform)
+    while (tp < 3 && i0 >= d.tend[tp]) tp++;
+    const float* W = d.wm[tp];
+    const int kq = l >> 4;                                  // this lane's A column / B row within a k-step
+    const int bofs = 16 * w + 4 * kq;                       // this lane's four outputs (biases)
+    const BnnNode b = d.nodes[i0 + (t & (BNM_NODES - 1))];  // ahead of the weights: the gather waits on it first
+    // layer 0 (K = 15, padded to 16 with a zero weight and a zero input) and the first hidden
+    // layer's weights and biases, requested before the gather
+    float a0[4], b0[4], av[BNN_NH / 4], bv[4];
+    bnm_ld4(W + 4 * (64 * w + l), a0);
+    bnm_ld4(W + BNM_OFF_B0 + bofs, b0);
+#pragma unroll
+    for (int q = 0; q < BNN_NH / 16; q++) bnm_ld4(W + BNM_OFF_H1 + 4 * ((8 * w + q) * 64 + l), av + 4 * q);
+    bnm_ld4(W + BNM_OFF_H1 + BNM_LAYER_H - BNN_NH + bofs, bv);
+    // inputs: (u.x, u.y, rho) of the 5 neighbours of each node, as float
+    if (t < 5 * BNM_NODES) {
+        const int j = t / BNM_NODES, q = t % BNM_NODES;
+        int xn = b.x + BNN_DX[b.t][j], yn = b.y + BNN_DY[b.t][j];
+        xn = xn < 0 ? xn + nx : (xn >= nx ? xn - nx : xn);  // the ring's ghost columns
+        double f[9], rho, ux, uy;
+        if (pull) {
+            macro_at<T, BC>(src, mac, corner, xn, yn, nx, ny, a.uinx, a.uiny, a.plate_x, a.ref, f, rho, ux, uy);
+        } else {  // first step: the macros of the initial state
+            const unsigned N = (unsigned)nx * ny, n = (unsigned)yn * nx + xn;
+            rho = mac[n];
+            ux = mac[N + n];
+            uy = mac[2 * N + n];
+        }
+        xin[3 * j][q] = (float)ux;
+        xin[3 * j + 1][q] = (float)uy;
+        xin[3 * j + 2][q] = (float)rho;
+    } else if (t < 6 * BNM_NODES) {
+        xin[BNN_NIN][t - 5 * BNM_NODES] = 0.0f;
+    }
+    __syncthreads();
+    {  // layer 0 -> X[0]
+        f32x4b c;
+#pragma unroll
+        for (int i = 0; i < 4; i++) c[i] = b0[i];
+#pragma unroll
+        for (int j = 0; j < 4; j++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[j], xin[4 * j + kq][l & 15], c, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) X[0][bofs + i][l & 15] = fmaxf(c[i], 0.0f);
+        __syncthreads();
+    }
+    // hidden layers in pairs (odd: X[0] -> X[1], even: X[1] -> X[0])
+#pragma unroll
+    for (int hl = 1; hl < BNN_NHID; hl += 2) {
+        const float* Wh = W + BNM_OFF_H1 + hl * BNM_LAYER_H;  // hidden layer hl + 1
+        bnm_hidden<1>(X, av, bv, hl + 1 <= BNN_NHID ? Wh : nullptr, w, l, bofs);
+        bnm_hidden<0>(X, av, bv, hl + 2 <= BNN_NHID ? Wh + BNM_LAYER_H : nullptr, w, l, bofs);
+    }
+    const int cur = 0;  // the last hidden layer's outputs
+    if (t < BNM_NODES) {
+        const int slot = b.slot;  // b is node i0 + t for t < 16
+        const float* Wl = W + BNM_OFF_OUT;
+        float r = Wl[BNN_NH];
+        for (int k = 0; k < BNN_NH; k++) r = __builtin_fmaf(Wl[k], X[cur][k][t], r);
+        const double tau = (double)r;
+        if (slot >= 0) d.tau[slot] = tau >= 0.501 ? tau : 0.0;
     }
 }
 
@@ -2781,6 +2932,8 @@ struct ippi_handle {
                                   // network (environment IPPI_NN_NPX=1 for one)
     bool adj_overlap = true;      // environment IPPI_ADJ_OVERLAP=0: the adjoint recomputes each segment on
                                   // the handle's stream, between the sweeps (to time the sweep's kernels alone)
+    bool bnn_mfma = true;         // environment IPPI_BNN_MFMA=0: the boundary networks as k_bnn's FMA
+                                  // chains instead of k_bnn_mfma (the same results)
     bool adj_fused = true;        // environment IPPI_ADJ_FUSED=0: the reverse step as three kernels
                                   // (k_adj_act, k_adj_mk, k_adj_step) instead of k_adj_fused
     bool round32 = false;         // environment IPPI_FP64_ROUND_STORE=1: the FP64 path rounds each stored
@@ -2799,8 +2952,10 @@ struct ippi_handle {
     bool mlp_exact = false;  // W1 and W2 hold bf16 values only (the FP32 path's exact-weight network)
     // boundary-row / column networks (types 1, 3, 5, 7 -> index 0..3)
     float* d_bnn_w[4] = {nullptr, nullptr, nullptr, nullptr};
+    float* d_bnn_wm[4] = {nullptr, nullptr, nullptr, nullptr};  // the MFMA kernel's layout
     BnnNode* d_bnn_nodes = nullptr;
     int bnn_nnodes = 0;
+    int bnn_tend[4] = {0, 0, 0, 0};
     double* d_bnn_tau = nullptr;  // [2 nx + 2 ny]
     int bnn_on = 0;
     // optional per-launch timing (HIP events on this handle's stream)
@@ -3246,10 +3401,25 @@ struct CallTimer {
 template <typename T>
 int launch_bnn(ippi_handle* h, const StepArgs& a, bool pull) {
     BnnDev d{h->d_bnn_nodes, h->bnn_nnodes, {h->d_bnn_w[0], h->d_bnn_w[1], h->d_bnn_w[2], h->d_bnn_w[3]},
-             h->d_bnn_tau};
+             h->d_bnn_tau, {h->bnn_tend[0], h->bnn_tend[1], h->bnn_tend[2], h->bnn_tend[3]},
+             {h->d_bnn_wm[0], h->d_bnn_wm[1], h->d_bnn_wm[2], h->d_bnn_wm[3]}};
     const dim3 g((unsigned)((h->bnn_nnodes + BNN_NODES - 1) / BNN_NODES));
     const T* src = (const T*)h->buf[h->cur];
     const int p = pull ? 1 : 0;
+    if (h->bnn_mfma) {  // the MFMA networks (bitwise k_bnn's results, see k_bnn_mfma)
+        const dim3 gm((unsigned)((h->bnn_nnodes + BNM_NODES - 1) / BNM_NODES));
+        if (h->c.bc == IPPI_BC_PERIODIC_BB)
+            hipLaunchKernelGGL((k_bnn_mfma<T, IPPI_BC_PERIODIC_BB>), gm, dim3(BNM_T), 0, h->s, src, h->mac, h->corner,
+                               p, a, d);
+        else if (h->c.bc == IPPI_BC_INFLOW)
+            hipLaunchKernelGGL((k_bnn_mfma<T, IPPI_BC_INFLOW>), gm, dim3(BNM_T), 0, h->s, src, h->mac, h->corner, p,
+                               a, d);
+        else
+            hipLaunchKernelGGL((k_bnn_mfma<T, IPPI_BC_FLATPLATE>), gm, dim3(BNM_T), 0, h->s, src, h->mac, h->corner,
+                               p, a, d);
+        HIPC(h, hipGetLastError());
+        return IPPI_OK;
+    }
     if (h->c.bc == IPPI_BC_PERIODIC_BB)
         hipLaunchKernelGGL((k_bnn<T, IPPI_BC_PERIODIC_BB>), g, dim3(BNN_T), 0, h->s, src, h->mac, h->corner, p, a, d);
     else if (h->c.bc == IPPI_BC_INFLOW)
@@ -3794,6 +3964,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (const char* e = std::getenv("IPPI_FP64_ROUND_STORE")) h->round32 = std::strcmp(e, "1") == 0;
     if (const char* e = std::getenv("IPPI_ADJ_OVERLAP")) h->adj_overlap = std::strcmp(e, "0") != 0;
     if (const char* e = std::getenv("IPPI_ADJ_FUSED")) h->adj_fused = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("IPPI_BNN_MFMA")) h->bnn_mfma = std::strcmp(e, "0") != 0;
     if (const char* e = std::getenv("IPPI_NN_NPX")) h->npx_nn = std::strcmp(e, "1") == 0 ? 1 : 2;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;
@@ -3840,6 +4011,7 @@ int ippi_destroy(ippi_handle* h) {
     dfree(h->d_frag);
     dfree(h->d_frage);
     for (auto& w : h->d_bnn_w) dfree(w);
+    for (auto& w : h->d_bnn_wm) dfree(w);
     dfree(h->d_bnn_nodes);
     dfree(h->d_bnn_tau);
     dfree(h->d_blob);
@@ -4232,16 +4404,45 @@ int ippi_set_boundary_mlp(ippi_handle* h, int type, const double* blob, int n) {
     HIPC(h, hipStreamSynchronize(h->s));
     if (!h->d_bnn_w[t]) HIPC(h, hipMalloc(&h->d_bnn_w[t], sizeof(float) * BNN_NPARAM));
     HIPC(h, hipMemcpy(h->d_bnn_w[t], w.data(), sizeof(float) * BNN_NPARAM, hipMemcpyHostToDevice));
+    // the MFMA kernel's copy (layout at BNM_OFF_*), from the input-major one
+    std::vector<float> wm(BNM_NPARAM, 0.0f);
+    const size_t L0 = (size_t)BNN_NIN * BNN_NH + BNN_NH, LH = (size_t)BNN_NH * BNN_NH + BNN_NH;
+    for (int wv = 0; wv < 8; wv++)
+        for (int ln = 0; ln < 64; ln++) {
+            const int kq = ln / 16, o = 16 * wv + ln % 16;
+            for (int j = 0; j < 4; j++) {
+                const int k = 4 * j + kq;
+                wm[4 * (64 * wv + ln) + j] = k < BNN_NIN ? w[(size_t)k * BNN_NH + o] : 0.0f;
+            }
+            for (int hl = 1; hl <= BNN_NHID; hl++)
+                for (int j = 0; j < 32; j++)
+                    wm[BNM_OFF_H1 + (size_t)(hl - 1) * BNM_LAYER_H + 4 * ((8 * wv + j / 4) * 64 + ln) + j % 4] =
+                        w[L0 + (size_t)(hl - 1) * LH + (size_t)(4 * j + kq) * BNN_NH + o];
+        }
+    for (int o = 0; o < BNN_NH; o++) {
+        wm[BNM_OFF_B0 + o] = w[(size_t)BNN_NIN * BNN_NH + o];
+        for (int hl = 1; hl <= BNN_NHID; hl++)
+            wm[BNM_OFF_H1 + (size_t)hl * BNM_LAYER_H - BNN_NH + o] = w[L0 + (size_t)(hl - 1) * LH + (size_t)BNN_NH * BNN_NH + o];
+    }
+    for (int k = 0; k <= BNN_NH; k++) wm[BNM_OFF_OUT + k] = w[L0 + (size_t)BNN_NHID * LH + k];
+    if (!h->d_bnn_wm[t]) HIPC(h, hipMalloc(&h->d_bnn_wm[t], sizeof(float) * BNM_NPARAM));
+    HIPC(h, hipMemcpy(h->d_bnn_wm[t], wm.data(), sizeof(float) * BNM_NPARAM, hipMemcpyHostToDevice));
     h->bnn_on |= 1 << t;
     // node list of every type set, grouped by type
     const int nx = h->c.nx, ny = h->c.ny;
     std::vector<BnnNode> nodes;
     for (int tt = 0; tt < 4; tt++) {
-        if (!(h->bnn_on & (1 << tt))) continue;
-        if (tt == 0 || tt == 2)
-            for (int y = 1; y <= ny - 2; y++) nodes.push_back(BnnNode{tt == 0 ? 0 : nx - 1, y, 2 * nx + (tt == 2 ? ny : 0) + y, tt});
-        else
-            for (int x = 0; x < nx; x++) nodes.push_back(BnnNode{x, tt == 3 ? 1 : ny - 2, (tt == 1 ? nx : 0) + x, tt});
+        if (h->bnn_on & (1 << tt)) {
+            if (tt == 0 || tt == 2)
+                for (int y = 1; y <= ny - 2; y++)
+                    nodes.push_back(BnnNode{tt == 0 ? 0 : nx - 1, y, 2 * nx + (tt == 2 ? ny : 0) + y, tt});
+            else
+                for (int x = 0; x < nx; x++) nodes.push_back(BnnNode{x, tt == 3 ? 1 : ny - 2, (tt == 1 ? nx : 0) + x, tt});
+            BnnNode last = nodes.back();
+            last.slot = -1;
+            while (nodes.size() % BNN_PAD) nodes.push_back(last);
+        }
+        h->bnn_tend[tt] = (int)nodes.size();
     }
     dfree(h->d_bnn_nodes);
     HIPC(h, hipMalloc(&h->d_bnn_nodes, sizeof(BnnNode) * nodes.size()));

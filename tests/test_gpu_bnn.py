@@ -112,3 +112,39 @@ def test_bnn_rejects_bad_input(ippi):
             s.set_boundary_mlp(3, p)
         with pytest.raises((ValueError, ippi.IppiError)):
             s.set_boundary_mlp(3, np.zeros(10))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_bnn_mfma_matches_fma_chains(ippi, monkeypatch, case, precision):
+    """The networks on FP32 MFMA tiles (k_bnn_mfma, the default) and as
+    sequential FMA chains (IPPI_BNN_MFMA=0, bit for bit the oracle's): an
+    MFMA adds its four products to the accumulator in order with a rounding
+    each, so the two give the same relaxation times and fields bit for bit
+    over the case's steps."""
+    nx, ny, coll, bc, les, tau0, cs, u0, types, body, steps = CASES[case]
+    kw = dict(les=les, tau0=tau0, cs=cs, u_in=u0, plate_x=40)
+
+    def make():
+        g = ippi.Solver(nx, ny, coll, precision, bc, **kw)
+        g.init_fields(3, 1.0, u0, 1e-3)
+        if body:
+            g.add_body(*body)
+        for t in types:
+            g.set_boundary_mlp(t, random_params(100 + t))
+        return g
+    mm = make()
+    monkeypatch.setenv("IPPI_BNN_MFMA", "0")
+    ch = make()
+    monkeypatch.delenv("IPPI_BNN_MFMA")
+    for g in (mm, ch):
+        g.step(steps)
+    edge = np.zeros((ny, nx), bool)
+    edge[1, :] = edge[ny - 2, :] = True
+    edge[1:ny - 1, 0] = edge[1:ny - 1, nx - 1] = True
+    t_m, t_c = mm.tau(), ch.tau()
+    print(f"{case} {precision}: edge nodes accepting the network: {np.mean(t_c[edge] >= 0.501):.2f}")
+    assert np.array_equal(t_m, t_c)
+    assert all(np.array_equal(a, b) for a, b in zip(mm.fields(), ch.fields()))
+    mm.close()
+    ch.close()
