@@ -37,11 +37,17 @@
  *                step for single nodes, random directions and the body
  *                centre, on horizons of 1 to 50 steps; exact zeros outside
  *                the domain of dependence; translation equivariance.
+ *   or_bnn_forward, or_set_bnn  the boundary-row / column networks
+ *                (mesh.cpp:2767-2898, 3128-3260): the forward pass against
+ *                torch.nn in FP64 and FP32; which node each type writes and
+ *                which five neighbours it reads, in the reference's input
+ *                order (pass-through networks); the >= 0.501 guard and the
+ *                row-over-column order where they meet.
  * The step as a whole (mesh.cpp) is not compiled (it needs MPI and
  * libtorch), so the composition is pinned by the checks above rather than
- * bit for bit. Not restated, so no pin applies: the reference's libtorch
- * boundary models (mesh.cpp:2767-3260); parity unpinned for those, and the
- * HIP library does not implement them.
+ * bit for bit. Parity unpinned: the outputs of the reference's TorchScript
+ * archives (AutoT0x.pt), which are not loaded; or_bnn_forward follows the
+ * layer shapes recorded from the reference, with ReLU.
  */
 #include "ippi_oracle.h"
 #include <math.h>

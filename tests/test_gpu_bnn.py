@@ -148,3 +148,39 @@ def test_bnn_mfma_matches_fma_chains(ippi, monkeypatch, case, precision):
     assert all(np.array_equal(a, b) for a, b in zip(mm.fields(), ch.fields()))
     mm.close()
     ch.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_bnn_mfma_full_size_flat_plate(ippi, monkeypatch, precision):
+    """The reference's own run at full size (2401 x 501 flat plate, BGK +
+    LES Cs = 0.173, Re = 5e5) with all four boundary networks: the MFMA
+    networks and the FMA chains give the same relaxation times and fields
+    bit for bit after 50 steps (366 tiles of 16 nodes, each type's list
+    padded to whole tiles)."""
+    from ippi_amd import workloads
+    nx, ny, u = 2401, 501, 0.1
+    tau = 0.5 + 6.0 * (ny * u) / 5e5
+
+    def make():
+        g = ippi.Solver(nx, ny, "bgk", precision, "flatplate", les=True, tau0=tau, cs=0.173, u_in=(u, 0.0),
+                        plate_x=400)
+        g.init_fields(1, 1.0, (u, 0.0), 1e-3)
+        for t in (1, 3, 5, 7):
+            g.set_boundary_mlp(t, workloads.boundary_blob(t))
+        return g
+    mm = make()
+    monkeypatch.setenv("IPPI_BNN_MFMA", "0")
+    ch = make()
+    monkeypatch.delenv("IPPI_BNN_MFMA")
+    print(f"flat plate 2401x501 {precision}: 50 steps, MFMA networks vs FMA chains", flush=True)
+    for g in (mm, ch):
+        g.step(50)
+    t_m, t_c = mm.tau(), ch.tau()
+    edge = np.zeros((ny, nx), bool)
+    edge[1, :] = edge[ny - 2, :] = True
+    edge[1:ny - 1, 0] = edge[1:ny - 1, nx - 1] = True
+    print(f"  edge nodes accepting the network: {np.mean(t_c[edge] >= 0.501):.3f}")
+    assert np.array_equal(t_m, t_c)
+    assert all(np.array_equal(a, b) for a, b in zip(mm.fields(), ch.fields()))
+    mm.close()
+    ch.close()
