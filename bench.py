@@ -633,9 +633,9 @@ def pattern_probe(ippi):
     """Bandwidth ceiling of the fused step's memory pattern: the same nine
     pulled loads and nine stores with no arithmetic, in the plain and the
     XCD-grouped block order, next to a plain nine-array copy, at the
-    headline and network-leg sizes."""
+    headline and network-leg sizes and the reference's flat plate."""
     out = {}
-    for (nx, ny) in ((2048, 1024), (4096, 2048), (8192, 4096)):
+    for (nx, ny) in ((2048, 1024), (4096, 2048), (8192, 4096), (2401, 501)):
         for prec in ("fp32", "fp64"):
             bpn = 72 if prec == "fp32" else 144
             for mode in ("pull", "pull_xcd", "copy", "copy16"):

@@ -590,6 +590,15 @@ template <typename T, bool PULL, bool NN>
 constexpr int step_npt() {
     return PULL && sizeof(T) == 8 && !NN ? 4 : 1;
 }
+// The same for boundary set BC. The flat-plate rules load their own
+// populations, so no row's loads go out ahead of the previous row's work
+// there and one row per thread is fastest in both precisions (2401 x 501
+// FP64: 30.6 us against 31.9 with two rows and 33.2 with four; FP32: 17.5
+// against 26.6 and 24.4).
+template <typename T, bool PULL, bool NN, int BC>
+constexpr int step_npt_bc() {
+    return BC == IPPI_BC_FLATPLATE ? 1 : step_npt<T, PULL, NN>();
+}
 
 // ------------------------------------------------------------- FP64 step
 // Raw pulled populations of node (x, y) (for IPPI_BC_INFLOW the NEE columns
@@ -3014,7 +3023,7 @@ template <typename T, bool PULL, int NN>
 StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
 #define IPPI_PICK(C, B)                                                                         \
     if (coll == C && bc == B) {                                                                 \
-        constexpr int NPT_ = step_npt<T, PULL, (NN != 0)>();                                  \
+        constexpr int NPT_ = step_npt_bc<T, PULL, (NN != 0), B>();                            \
         if constexpr (sizeof(T) == 8) {                                                         \
             if (PULL && nt) return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, (NN != 0), PULL>; \
             return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, (NN != 0), false>;               \
@@ -3435,7 +3444,10 @@ int run_steps(ippi_handle* h, int nsteps) {
     CallTimer call_timer(h);
     const int nx = h->c.nx, ny = h->c.ny;
     const bool nn_mode = nn_kind(h) != 0;
-    const int npt = nn_mode ? step_npt<T, true, true>() : step_npt<T, true, false>();
+    const int npt = h->c.bc == IPPI_BC_FLATPLATE
+                        ? (nn_mode ? step_npt_bc<T, true, true, IPPI_BC_FLATPLATE>()
+                                   : step_npt_bc<T, true, false, IPPI_BC_FLATPLATE>())
+                        : (nn_mode ? step_npt<T, true, true>() : step_npt<T, true, false>());
     const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const bool ib = !h->bodies.empty();
     if (ib) {
