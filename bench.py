@@ -32,7 +32,8 @@ Reported beside it:
                     WRITE_SIZE summary, scaled by the same counters on a plain
                     copy of known size;
   cpu_baseline      the FP64 CPU restatement (oracle/, OpenMP) on a bounded
-                    sample of the same workload on this host's cores;
+                    sample of the same workload on this host's cores, and
+                    (other_workloads) on C1, C2 and the flat plate;
   legs              the other workloads, each measured the same way:
                     accuracy_nn_4096x2048  the headline workload over 10,000
                                    steps: FP32 vs FP64 (both with the
@@ -57,6 +58,8 @@ Reported beside it:
                     flatplate_2401x501  the reference's own run (flat plate,
                                    BGK + LES, Re = 5e5), also with its
                                    boundary-row / column networks set;
+                    c1_bgk_channel_256x128  BGK channel with a cylinder,
+                                   FP64 and FP32, 10,000 steps;
                     tausearch      FindAndWriteTau's candidate search.
 
 The headline's W warm-up and K timed steps run after the legs, right after
@@ -202,9 +205,33 @@ def cpu_baseline(seconds_target=15.0):
     o.step(n)
     dt = time.perf_counter() - t0
     o.close()
-    return {"value": round(NX * NY * n / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"oracle/ FP64 MRT+LES+IB+network tau, {NX}x{NY}, {n} steps after 1 warm-up step, "
-                      f"{threads} OpenMP threads"}
+    out = {"value": round(NX * NY * n / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": f"oracle/ FP64 MRT+LES+IB+network tau, {NX}x{NY}, {n} steps after 1 warm-up step, "
+                     f"{threads} OpenMP threads"}
+    # the same oracle on the other CPU-sized workloads (a few seconds each)
+    others = {}
+    for name, (nx, ny, coll, bc, kw, body) in {
+        "bgk_channel_cyl_256x128": (256, 128, "bgk", "periodic_bb", dict(tau0=0.6, u_in=(0.05, 0.0)), (64.0, 64.0, 20.0, 63)),
+        "mrt_les_ib_2048x1024": (2048, 1024, "mrt", "inflow", dict(les=True, tau0=0.5 + 3 * 0.08 * 80 / 1e4, cs=0.1,
+                                                                    u_in=(0.08, 0.0)), (512.0, 512.0, 80.0, 251)),
+        "flatplate_2401x501": (2401, 501, "bgk", "flatplate", dict(les=True, tau0=0.5 + 6.0 * (501 * 0.1) / 5e5,
+                                                                    cs=0.173, u_in=(0.1, 0.0), plate_x=400), None),
+    }.items():
+        o = pyoracle.Sim(nx, ny, coll, bc, threads=threads, **kw)
+        o.init_fields(1, 1.0, kw["u_in"], 1e-3)
+        if body:
+            o.add_body(*body)
+        t0 = time.perf_counter()
+        o.step(1)
+        one = time.perf_counter() - t0
+        m = max(2, min(2000, int(4.0 / max(one, 1e-4))))
+        t0 = time.perf_counter()
+        o.step(m)
+        dt = time.perf_counter() - t0
+        o.close()
+        others[name] = {"mlups": round(nx * ny * m / dt / 1e6, 2), "steps": m}
+    out["other_workloads"] = others
+    return out
 
 
 # ---------------------------------------------------------------- other legs
@@ -568,6 +595,29 @@ def leg_tausearch(ippi, n=65536, n_cpu=48):
             "cpu_sample_bitwise_equal": bool(np.array_equal(bd[:n_cpu], bo) and np.array_equal(be[:n_cpu], eo))}
 
 
+def leg_c1(ippi, steps=10000):
+    """C1: BGK (tau = 0.6) channel, 256 x 128, periodic in x with halfway
+    bounce-back walls, cylinder D = 20 (63 markers) at (64, 64), U = 0.05,
+    10,000 steps: FP64 (the reference's precision) and FP32 MLUPS, and the
+    FP32 velocity against FP64 at the end."""
+    nx, ny = 256, 128
+    out = {"workload": "BGK tau=0.6 channel 256x128, periodic x + bounce-back walls, cylinder D=20 (63 markers) "
+                       "at (64, 64), U=0.05 +-1e-3", "steps": steps}
+    runs = {}
+    for prec in ("fp64", "fp32"):
+        s = ippi.Solver(nx, ny, "bgk", prec, "periodic_bb", tau0=0.6, u_in=(0.05, 0.0))
+        s.init_fields(SEED, 1.0, (0.05, 0.0), 1e-3)
+        s.add_body(64.0, 64.0, 20.0, 63)
+        log(f"C1 {prec}: {steps} steps")
+        dt = timed_run(s, steps, 0)
+        out[f"mlups_{prec}"] = round(nx * ny * steps / dt / 1e6, 1)
+        runs[prec] = s
+    out["rel_l2_u_fp32_vs_fp64"] = vel_err(runs["fp32"].fields(), runs["fp64"].fields())
+    for s in runs.values():
+        s.close()
+    return out
+
+
 def leg_flatplate(ippi, steps=5000, marks=(1000, 5000)):
     """The reference's own run (IPPI-LBM/main.cpp): flat plate on
     2401 x 501, BGK + hybrid LES (Cs = 0.173), U = 0.1, Re = 5e5
@@ -654,9 +704,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparisons of the headline")
-    ap.add_argument("--legs", default="accuracy,c3,mrtles,adjoint,multibody,flatplate,tausearch",
+    ap.add_argument("--legs", default="accuracy,c3,mrtles,adjoint,multibody,flatplate,c1,tausearch",
                     help="comma list of extra workloads (accuracy, c3, mrtles, adjoint, multibody, flatplate, "
-                         "tausearch) or 'none'")
+                         "c1, tausearch) or 'none'")
     ap.add_argument("--c3-steps", type=int, default=10000, help="steps of the C3 run (FP32 vs FP64)")
     ap.add_argument("--acc-steps", type=int, default=10000, help="steps of the headline accuracy leg")
     ap.add_argument("--mrtles-steps", type=int, default=2000)
@@ -704,6 +754,8 @@ def main():
     if "flatplate" in names:
         legs["flatplate_2401x501"] = leg_flatplate(ippi, args.plate_steps,
                                                    tuple(m for m in (1000, 5000) if m <= args.plate_steps))
+    if "c1" in names:
+        legs["c1_bgk_channel_256x128"] = leg_c1(ippi)
     if "tausearch" in names:
         legs["tausearch"] = leg_tausearch(ippi)
 
