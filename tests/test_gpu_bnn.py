@@ -37,6 +37,8 @@ CASES = {
     "flatplate_bgk_les_all": (160, 48, "bgk", "flatplate", True, 0.52, 0.173, (0.1, 0.0), (1, 3, 5, 7), None, 200),
     "inflow_mrt_les_rows_ib": (128, 64, "mrt", "inflow", True, 0.53, 0.1, (0.08, 0.0), (3, 7), (40, 32, 16, 51), 150),
     "channel_bgk_columns": (96, 40, "bgk", "periodic_bb", False, 0.6, 0.0, (0.05, 0.0), (1, 5), None, 150),
+    # every type's node list shorter than one 16-node tile (mostly padding)
+    "tiny_flatplate_all": (9, 6, "bgk", "flatplate", True, 0.52, 0.173, (0.1, 0.0), (1, 3, 5, 7), None, 60),
 }
 
 
