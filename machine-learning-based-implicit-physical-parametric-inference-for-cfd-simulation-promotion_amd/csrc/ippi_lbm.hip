@@ -1733,9 +1733,21 @@ __global__ void __launch_bounds__(BNM_T) k_bnn_mfma(const T* __restrict__ src, c
     int tp = 0;  // the tile's type (block-uniform)
     while (tp < 3 && i0 >= d.tend[tp]) tp++;
     const float* W = d.wm[tp];
-    const int kq = l >> 4;                                  // this lane's A column / B row within a k-step
-    const int bofs = 16 * w + 4 * kq;                       // this lane's four outputs (biases)
-    const BnnNode b = d.nodes[i0 + (t & (BNM_NODES - 1))];  // ahead of the weights: the gather waits on it first
+    const int kq = l >> 4;             // this lane's A column / B row within a k-step
+    const int bofs = 16 * w + 4 * kq;  // this lane's four outputs (biases)
+    // node i0 + t % 16 of the list, computed as the host lays it out (each
+    // type's row or column in order, padded with copies of its last node
+    // whose slot is -1), so the gather starts without a load
+    BnnNode b;
+    {
+        const int q0 = i0 + (t & (BNM_NODES - 1)) - (tp == 0 ? 0 : d.tend[tp - 1]);
+        const bool col = tp == 0 || tp == 2;
+        const int cnt = col ? ny - 2 : nx, q = min(q0, cnt - 1);
+        b.x = col ? (tp == 0 ? 0 : nx - 1) : q;
+        b.y = col ? 1 + q : (tp == 3 ? 1 : ny - 2);
+        b.slot = q0 >= cnt ? -1 : (col ? 2 * nx + (tp == 2 ? ny : 0) + b.y : (tp == 1 ? nx : 0) + b.x);
+        b.t = tp;
+    }
     // layer 0 (K = 15, padded to 16 with a zero weight and a zero input) and the first hidden
     // layer's weights and biases, requested before the gather
     float a0[4], b0[4], av[BNN_NH / 4], bv[4];
@@ -1784,7 +1796,7 @@ __global__ void __launch_bounds__(BNM_T) k_bnn_mfma(const T* __restrict__ src, c
     }
     const int cur = 0;  // the last hidden layer's outputs
     if (t < BNM_NODES) {
-        const int slot = b.slot;  // b is node i0 + t for t < 16
+        const int slot = b.slot;  // b is node i0 + t for t < 16 (-1: padding)
         const float* Wl = W + BNM_OFF_OUT;
         float r = Wl[BNN_NH];
         for (int k = 0; k < BNN_NH; k++) r = __builtin_fmaf(Wl[k], X[cur][k][t], r);
