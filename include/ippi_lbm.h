@@ -80,7 +80,7 @@ enum ippi_tau_mode { IPPI_TAU_CONST = 0, IPPI_TAU_FIELD = 1, IPPI_TAU_NN = 2, IP
 typedef struct ippi_config {
     int nx, ny;          /* whole domain (no ghost columns) */
     int collision;       /* ippi_collision */
-    int precision;       /* ippi_precision: FP64 populations, or FP32 f - w */
+    int precision;       /* ippi_precision: FP64 populations, or FP32 deviations f - feq(1, (u_in_x, 0)) */
     int bc;              /* ippi_bc */
     int les;             /* 1: hybrid Smagorinsky tau (mesh.cpp:877-910) */
     int tau_mode;        /* ippi_tau_mode */
