@@ -2404,6 +2404,7 @@ struct AdjIb {
     // active node (-1: none)
     double* lu;           // [marker][2]
     double* pc;           // [m][16][2]
+    const int* pnr;       // [m][16][2] the pair's support node (y * nx + x) and patch node
     const int* pslot;
 };
 
@@ -2495,12 +2496,11 @@ __device__ __forceinline__ A adj_tx(const AdjArgs& a, unsigned n) {
 // sum to zero, so the rounding of FP32 arithmetic on these quantities would
 // not cancel with the rest.
 template <typename S, typename A, bool PULL>
-__device__ __forceinline__ void adj_act_vals(int i, const A* __restrict__ lin, const S* __restrict__ src,
-                                             const double* __restrict__ mac, const AdjIb& d, const AdjArgs& a,
-                                             double& lfx, double& lfy, double& ux, double& uy) {
+__device__ __forceinline__ void adj_act_vals_at(int node, int r, const A* __restrict__ lin, const S* __restrict__ src,
+                                                const double* __restrict__ mac, const AdjArgs& a, double& lfx,
+                                                double& lfy, double& ux, double& uy) {
     const int nx = a.nx, ny = a.ny;
     const unsigned N = (unsigned)nx * ny;
-    const int node = d.act_node[i], r = d.act[i];
     const int x = node % nx, y = node / nx;
     A la[9];
     pull_adj(lin, x, y, nx, ny, N, la);
@@ -2526,6 +2526,12 @@ __device__ __forceinline__ void adj_act_vals(int i, const A* __restrict__ lin, c
     const adj::WMom<double> m = adj::wmoments(l);
     const adj::UsAdj<double> ua = adj::us_adjoint(m, s, true);
     adj::force_adjoint(ua, s, lfx, lfy);
+}
+template <typename S, typename A, bool PULL>
+__device__ __forceinline__ void adj_act_vals(int i, const A* __restrict__ lin, const S* __restrict__ src,
+                                             const double* __restrict__ mac, const AdjIb& d, const AdjArgs& a,
+                                             double& lfx, double& lfy, double& ux, double& uy) {
+    adj_act_vals_at<S, A, PULL>(d.act_node[i], d.act[i], lin, src, mac, a, lfx, lfy, ux, uy);
 }
 template <typename S, typename A, bool PULL>
 __device__ __forceinline__ void adj_act_node(int i, const A* __restrict__ lin, const S* __restrict__ src,
@@ -2771,11 +2777,15 @@ __device__ __forceinline__ void adj_body_block(const A* __restrict__ lin, const 
     const int item = 16 * mb0 + t;  // the pair in k_adj_mk's numbering within the body
     const bool ok = item < 16 * b.M;
     const int qq = 16 * b.m0 + (ok ? item : 0);
-    const int i = b.a0 + d.pidx[qq];
-    double lfx, lfy, ux, uy;
-    adj_act_vals<S, A, PULL>(i, lin, src, mac, d, a, lfx, lfy, ux, uy);
-    const double w = ok ? d.w[qq] : 0.0, wxs = ok ? d.wx[qq] : 0.0, wys = ok ? d.wy[qq] : 0.0;
+    // the pair's node and patch node from the table (no dependent index load), its weights and
+    // the marker's force requested with them
     const int ml = min(item, 16 * b.M - 1) >> 4;
+    const int m = b.m0 + ml;
+    const int pnode = d.pnr[2 * qq], pr = d.pnr[2 * qq + 1];
+    const double w = ok ? d.w[qq] : 0.0, wxs = ok ? d.wx[qq] : 0.0, wys = ok ? d.wy[qq] : 0.0;
+    const double Fx = d.F[2 * m], Fy = d.F[2 * m + 1];
+    double lfx, lfy, ux, uy;
+    adj_act_vals_at<S, A, PULL>(pnode, pr, lin, src, mac, a, lfx, lfy, ux, uy);
     double ax = w * lfx, ay = w * lfy;
 #pragma unroll
     for (int o = 8; o > 0; o >>= 1) {
@@ -2784,12 +2794,10 @@ __device__ __forceinline__ void adj_body_block(const A* __restrict__ lin, const 
     }
     // record = - sum of marker forces; F = 2 (U_d - U)
     const double lux = -2.0 * (ax - cj * b.wx), luy = -2.0 * (ay - cj * b.wy);
-    const int m = b.m0 + ml;
     if (ok && (t & 15) == 0) {
         st_coh(d.lu + 2 * m, lux);
         st_coh(d.lu + 2 * m + 1, luy);
     }
-    const double Fx = d.F[2 * m], Fy = d.F[2 * m + 1];
     const double cc = lux * ux + luy * uy + lfx * Fx + lfy * Fy;
     if (ok) {
         st_coh(d.pc + 2 * (size_t)qq, wxs * cc);
@@ -2960,6 +2968,7 @@ struct ippi_handle {
     bool round32 = false;         // environment IPPI_FP64_ROUND_STORE=1: the FP64 path rounds each stored
                                   // population as the FP32 path stores it (accuracy diagnostic)
     std::vector<int> h_act;      // active node -> patch node (host copy)
+    std::vector<int> h_pidx;     // [marker][16] support -> active index within its body (host copy)
     std::vector<int> body_act0;  // first active node of each body (+ total)
     double* hist = nullptr;
     int hist_cap = 0;
@@ -3233,6 +3242,7 @@ int upload_ib(ippi_handle* h) {
     h->nact = (int)act.size();
     h->nblk_ib = (int)blk.size();
     h->h_act = act;
+    h->h_pidx = pidx;
     h->body_act0 = act0s;
     free_ib(h);
     HIPC(h, hipMalloc(&h->d_pidx, sizeof(int) * 16 * nm));
@@ -3797,7 +3807,20 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     unsigned seq = 0;
     int nab = 0;  // row-0 blocks: ceil(markers / ADJ_MPB) per body
     for (int b = 0; b < nb; b++) nab += (tab[b].M + ADJ_MPB - 1) / ADJ_MPB;
+    std::vector<int> pnr;  // the body blocks' pairs: support node and patch node
     if (fused) {
+        pnr.resize(32 * (size_t)nm);
+        for (int b = 0; b < nb; b++)
+            for (int mk = tab[b].m0; mk < tab[b].m0 + tab[b].M; mk++)
+                for (int sp = 0; sp < 16; sp++) {
+                    const int i = tab[b].a0 + h->h_pidx[16 * (size_t)mk + sp];
+                    pnr[2 * (16 * (size_t)mk + sp)] = act_node[i];
+                    pnr[2 * (16 * (size_t)mk + sp) + 1] = h->h_act[i];
+                }
+        int* dpnr = nullptr;
+        HIPC(h, mem.alloc(&dpnr, sizeof(int) * 32 * (size_t)nm));
+        HIPC(h, hipMemcpyAsync(dpnr, pnr.data(), sizeof(int) * 32 * (size_t)nm, hipMemcpyHostToDevice, h->s));
+        ad.pnr = dpnr;
         HIPC(h, mem.alloc(&flags, sizeof(unsigned) * 2 * nb));
         HIPC(h, hipMemsetAsync(flags, 0, sizeof(unsigned) * 2 * nb, h->s));
         HIPC(h, mem.alloc(&ad.lu, sizeof(double) * 2 * (size_t)nm));
