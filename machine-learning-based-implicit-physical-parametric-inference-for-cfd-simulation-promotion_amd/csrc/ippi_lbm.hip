@@ -2849,9 +2849,16 @@ __device__ __forceinline__ void adj_body_block(const A* __restrict__ lin, const 
     }
 }
 
+// Position of the patch rows in the adjoint's row order, as a fraction of
+// the rows. Timed on the 1024 x 512 cylinder (D = 40): FP64 25.9-26.0 us per
+// reverse step with the patch rows last, 25.0-25.2 us at 0.7-0.8; FP32
+// 18.6 us last, 18.4-18.5 us at 0.7-0.8 (the marker blocks' flags are set
+// by then, and the rows after the band overlap the patch rows' gathers).
+constexpr double ADJ_BAND_AT = 0.75;
+
 // One reverse step in one launch: the first grid rows hold the bodies'
 // blocks (adj_body_block), the lattice rows follow in rotated order with the
-// bodies' patch rows last; a patch row waits for the flags of the bodies
+// bodies' patch rows at ADJ_BAND_AT of the order; a patch row waits for the flags of the bodies
 // covering it (bounded: a time-out sets *err = 2 and the sweep reports the
 // failure, never hangs) and its forced nodes gather their velocity adjoint
 // from the published marker weights. Ordering as in the forward step's
@@ -3832,7 +3839,9 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
         aa.err = h->flag;
         aa.band0 = h->band0;
         aa.band1 = h->band1;
-        aa.rot = h->band1 % ny;  // the patch rows last in the order
+        // the band wholly inside the order (a long band ends the order)
+        const int at = std::min((int)(ADJ_BAND_AT * ny), ny - (h->band1 - h->band0));
+        aa.rot = ((h->band0 - at) % ny + ny) % ny;  // lattice row of the first lattice grid row
     }
     const int gxa = (nx + BX - 1) / BX;
     aa.brows = (nab + gxa - 1) / gxa;
