@@ -585,11 +585,18 @@ __device__ __forceinline__ void ib_slots(const StepArgs& a, const int* sl, const
 // of one column and issues the loads of row j+1 before the arithmetic of
 // row j, so every wave keeps two rows of populations in flight. Measured on
 // the headline lattice: FP32 is fastest with one row (two or four cost
-// 3-15%), FP64 without the network with four (5% faster than one).
+// 3-15%), FP64 without the network with four (5% faster than one). The FP64
+// value is the most rows a kernel is built for; rows64() picks per lattice.
 template <typename T, bool PULL, bool NN>
 constexpr int step_npt() {
     return PULL && sizeof(T) == 8 && !NN ? 4 : 1;
 }
+// Four rows per thread leave small lattices with too few blocks for the
+// GPU: 1024 x 512 (512 blocks of four rows) takes 27.3 us per FP64 step with
+// the IB inside against 19.5 us with one row, 256 x 128 18.5 against 10.4;
+// 2048 x 1024 (2048 blocks) 66.9 against 69.0. Four rows from 2^20 nodes on.
+constexpr size_t F64_ROWS4_MIN_NODES = (size_t)1 << 20;
+inline int rows64_for(int nx, int ny) { return (size_t)nx * ny >= F64_ROWS4_MIN_NODES ? 4 : 1; }
 // The same for boundary set BC. The flat-plate rules load their own
 // populations, so no row's loads go out ahead of the previous row's work
 // there and one row per thread is fastest in both precisions (2401 x 501
@@ -2263,11 +2270,10 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     }
 }
 
-// FP64 variant: four rows per thread as k_step_f64, the rotation and the
+// FP64 variant: NPT rows per thread as k_step_f64, the rotation and the
 // band in units of those row groups.
-template <int COLL, int BC, bool NT>
+template <int COLL, int BC, bool NT, int NPT>
 __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ src, double* __restrict__ dst, const double* __restrict__ mac, StepArgs a) {
-    constexpr int NPT = step_npt<double, true, false>();
     __shared__ double U[2 * INK_MAXACT64];
     const int nx = a.nx, ny = a.ny;
     if ((int)blockIdx.y < a.brows) {  // the body blocks' rows (block-uniform)
@@ -3047,12 +3053,17 @@ using StepKernel = void (*)(const T*, T*, const double*, StepArgs);
 // nt: non-temporal stores (pull steps of lattices whose two population
 // buffers exceed 512 MB)
 // NN: network mode (0 none, 1 compensated, 2 plain bf16; FP64 has one network)
+// rows64: rows per thread of the FP64 kernels built for four (rows64_for)
 template <typename T, bool PULL, int NN>
-StepKernel<T> pick_step(int coll, int bc, bool nt = false) {
+StepKernel<T> pick_step(int coll, int bc, bool nt = false, int rows64 = 4) {
 #define IPPI_PICK(C, B)                                                                         \
     if (coll == C && bc == B) {                                                                 \
         constexpr int NPT_ = step_npt_bc<T, PULL, (NN != 0), B>();                            \
         if constexpr (sizeof(T) == 8) {                                                         \
+            if constexpr (NPT_ == 4)                                                            \
+                if (rows64 == 1)                                                                \
+                    return PULL && nt ? (StepKernel<T>)k_step_f64<C, B, PULL, 1, (NN != 0), PULL> \
+                                      : (StepKernel<T>)k_step_f64<C, B, PULL, 1, (NN != 0), false>; \
             if (PULL && nt) return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, (NN != 0), PULL>; \
             return (StepKernel<T>)k_step_f64<C, B, PULL, NPT_, (NN != 0), false>;               \
         } else {                                                                                \
@@ -3082,14 +3093,16 @@ StepKernel<float> pick_step_ib_bc(int nn, bool nt) {
 }
 
 template <int COLL, int BC>
-StepKernel<double> pick_step_ib64_bc(bool nt) {
-    return nt ? k_step_f64_ib<COLL, BC, true> : k_step_f64_ib<COLL, BC, false>;
+StepKernel<double> pick_step_ib64_bc(bool nt, int rows) {
+    if (rows == 1) return nt ? k_step_f64_ib<COLL, BC, true, 1> : k_step_f64_ib<COLL, BC, false, 1>;
+    return nt ? k_step_f64_ib<COLL, BC, true, 4> : k_step_f64_ib<COLL, BC, false, 4>;
 }
 
-// FP64 pull step with the IB inside (no network; periodic and inflow boundaries)
-StepKernel<double> pick_step_ib64(int coll, int bc, bool nt) {
+// FP64 pull step with the IB inside (no network; periodic and inflow
+// boundaries), rows (1 or 4) rows per thread
+StepKernel<double> pick_step_ib64(int coll, int bc, bool nt, int rows) {
 #define IPPI_PICK_IB(C, B) \
-    if (coll == C && bc == B) return pick_step_ib64_bc<C, B>(nt);
+    if (coll == C && bc == B) return pick_step_ib64_bc<C, B>(nt, rows);
     IPPI_PICK_IB(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB)
     IPPI_PICK_IB(IPPI_COLL_MRT, IPPI_BC_PERIODIC_BB)
     IPPI_PICK_IB(IPPI_COLL_RR, IPPI_BC_PERIODIC_BB)
@@ -3473,10 +3486,12 @@ int run_steps(ippi_handle* h, int nsteps) {
     CallTimer call_timer(h);
     const int nx = h->c.nx, ny = h->c.ny;
     const bool nn_mode = nn_kind(h) != 0;
-    const int npt = h->c.bc == IPPI_BC_FLATPLATE
-                        ? (nn_mode ? step_npt_bc<T, true, true, IPPI_BC_FLATPLATE>()
-                                   : step_npt_bc<T, true, false, IPPI_BC_FLATPLATE>())
-                        : (nn_mode ? step_npt<T, true, true>() : step_npt<T, true, false>());
+    int npt = h->c.bc == IPPI_BC_FLATPLATE
+                  ? (nn_mode ? step_npt_bc<T, true, true, IPPI_BC_FLATPLATE>()
+                             : step_npt_bc<T, true, false, IPPI_BC_FLATPLATE>())
+                  : (nn_mode ? step_npt<T, true, true>() : step_npt<T, true, false>());
+    const int rows64 = rows64_for(nx, ny);
+    if (npt == 4) npt = rows64;  // FP64 kernels built for four rows: per lattice
     const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const bool ib = !h->bodies.empty();
     if (ib) {
@@ -3546,9 +3561,9 @@ int run_steps(ippi_handle* h, int nsteps) {
                 hipLaunchKernelGGL(k, dim3(gx, ny + ai.brows), dim3(BX), 0, h->s,
                                    (const float*)h->buf[h->cur], (float*)h->buf[h->cur ^ 1], h->mac, ai);
             } else {
-                StepKernel<double> k = pick_step_ib64(h->c.collision, h->c.bc, a.nt);
+                const int NPT = rows64_for(nx, ny);
+                StepKernel<double> k = pick_step_ib64(h->c.collision, h->c.bc, a.nt, NPT);
                 if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
-                constexpr int NPT = step_npt<double, true, false>();
                 const int G = (ny + NPT - 1) / NPT;
                 ai.rot = ((h->band0 / NPT - at / NPT) % G + G) % G;
                 ai.xcd = !a.nt;
@@ -3604,7 +3619,7 @@ int run_steps(ippi_handle* h, int nsteps) {
         StepKernel<T> k = pull ? (nk == 3   ? pick_step<T, true, 3>(h->c.collision, h->c.bc, a.nt)
                                   : nk == 2 ? pick_step<T, true, 2>(h->c.collision, h->c.bc, a.nt)
                                   : nk == 1 ? pick_step<T, true, 1>(h->c.collision, h->c.bc, a.nt)
-                                            : pick_step<T, true, 0>(h->c.collision, h->c.bc, a.nt))
+                                            : pick_step<T, true, 0>(h->c.collision, h->c.bc, a.nt, rows64))
                                : (nk == 3   ? pick_step<T, false, 3>(h->c.collision, h->c.bc)
                                   : nk == 2 ? pick_step<T, false, 2>(h->c.collision, h->c.bc)
                                   : nk == 1 ? pick_step<T, false, 1>(h->c.collision, h->c.bc)
@@ -3846,10 +3861,11 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     const int gxa = (nx + BX - 1) / BX;
     aa.brows = (nab + gxa - 1) / gxa;
     const dim3 grid_fused(gxa, ny + aa.brows);
-    constexpr int npt = step_npt<T, true, false>();
+    const int rows64 = rows64_for(nx, ny);
+    const int npt = step_npt<T, true, false>() == 4 ? rows64 : step_npt<T, true, false>();
     const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const dim3 grid_adj((nx + BX - 1) / BX, ny), grid_act((h->nact + 255) / 256);
-    const StepKernel<T> kpull = pick_step<T, true, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB);
+    const StepKernel<T> kpull = pick_step<T, true, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB, false, rows64);
     const StepKernel<T> kfirst = pick_step<T, false, false>(IPPI_COLL_BGK, IPPI_BC_PERIODIC_BB);
     // Two buffer sets for the recomputed segments (states, patch forces,
     // marker forces): while the reverse sweep of segment sg runs on the
