@@ -2970,6 +2970,8 @@ struct ippi_handle {
     bool ink_enabled = true;      // environment IPPI_IB_IN_STEP=0 turns it off
     bool ink_wait_skew = false;   // environment IPPI_IB_WAIT_SKEW=1: the band waits for a value never
                                   // stored, so its bounded wait times out (tests of that path only)
+    int rows64 = 0;               // environment IPPI_F64_ROWS=1 or 4: rows per thread of the FP64 pull
+                                  // steps without the network (0: by lattice size, rows64_for)
     int npx_nn = 2;               // nodes per thread of the in-step kernel with the exact-weight
                                   // network (environment IPPI_NN_NPX=1 for one)
     bool adj_overlap = true;      // environment IPPI_ADJ_OVERLAP=0: the adjoint recomputes each segment on
@@ -3490,7 +3492,7 @@ int run_steps(ippi_handle* h, int nsteps) {
                   ? (nn_mode ? step_npt_bc<T, true, true, IPPI_BC_FLATPLATE>()
                              : step_npt_bc<T, true, false, IPPI_BC_FLATPLATE>())
                   : (nn_mode ? step_npt<T, true, true>() : step_npt<T, true, false>());
-    const int rows64 = rows64_for(nx, ny);
+    const int rows64 = h->rows64 ? h->rows64 : rows64_for(nx, ny);
     if (npt == 4) npt = rows64;  // FP64 kernels built for four rows: per lattice
     const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const bool ib = !h->bodies.empty();
@@ -3561,7 +3563,7 @@ int run_steps(ippi_handle* h, int nsteps) {
                 hipLaunchKernelGGL(k, dim3(gx, ny + ai.brows), dim3(BX), 0, h->s,
                                    (const float*)h->buf[h->cur], (float*)h->buf[h->cur ^ 1], h->mac, ai);
             } else {
-                const int NPT = rows64_for(nx, ny);
+                const int NPT = h->rows64 ? h->rows64 : rows64_for(nx, ny);
                 StepKernel<double> k = pick_step_ib64(h->c.collision, h->c.bc, a.nt, NPT);
                 if (!k) return fail(h, IPPI_ERR_ARG, "unsupported collision / boundary combination");
                 const int G = (ny + NPT - 1) / NPT;
@@ -3861,7 +3863,7 @@ int adjoint_run(ippi_handle* h, int nsteps, int avg_from, int ckpt, const double
     const int gxa = (nx + BX - 1) / BX;
     aa.brows = (nab + gxa - 1) / gxa;
     const dim3 grid_fused(gxa, ny + aa.brows);
-    const int rows64 = rows64_for(nx, ny);
+    const int rows64 = h->rows64 ? h->rows64 : rows64_for(nx, ny);
     const int npt = step_npt<T, true, false>() == 4 ? rows64 : step_npt<T, true, false>();
     const dim3 grid_pull((nx + BX - 1) / BX, (ny + npt - 1) / npt), grid_first((nx + BX - 1) / BX, ny);
     const dim3 grid_adj((nx + BX - 1) / BX, ny), grid_act((h->nact + 255) / 256);
@@ -4038,6 +4040,7 @@ int ippi_create(const ippi_config* cfg, ippi_handle** out) {
     if (const char* e = std::getenv("IPPI_ADJ_FUSED")) h->adj_fused = std::strcmp(e, "0") != 0;
     if (const char* e = std::getenv("IPPI_BNN_MFMA")) h->bnn_mfma = std::strcmp(e, "0") != 0;
     if (const char* e = std::getenv("IPPI_NN_NPX")) h->npx_nn = std::strcmp(e, "1") == 0 ? 1 : 2;
+    if (const char* e = std::getenv("IPPI_F64_ROWS")) h->rows64 = std::strcmp(e, "1") == 0 ? 1 : std::strcmp(e, "4") == 0 ? 4 : 0;
     h->N = (size_t)cfg->nx * cfg->ny;
     h->esz = cfg->precision == IPPI_FP64 ? 8 : 4;
     // reference state of the FP32 storage: the free stream (u_in_x, 0)
