@@ -2098,7 +2098,7 @@ __device__ void ib_in_step(const StepArgs& a, int body, const T* __restrict__ sr
     // A. velocity of every active node of the body (k_ib, pulling), NA nodes
     // in flight per lane (one in FP64, where the lattice path's registers
     // already set the kernel's budget)
-    constexpr int NA = sizeof(T) == 8 ? 1 : 2;
+    constexpr int NA = sizeof(T) == 8 ? 1 : 3;
     for (int c0 = 0; c0 < bk.nact; c0 += NA * BX) {
         int nd[NA];
 #pragma unroll
@@ -2249,7 +2249,6 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
     const int xs = bxi * BX * NPX;  // first column of this block
     int y = g + a.rot;  // rot < ny
     if (y >= ny) y -= ny;
-    if (y >= a.band0 && y < a.band1) wait_ib_forces(a);  // block-uniform
     const unsigned N = (unsigned)nx * ny;
     u32x4* lds_frag = reinterpret_cast<u32x4*>(U);
     if (NN == 3) mlp_frage_to_lds<BX>(a.mlp, lds_frag);  // in flight with the population loads
@@ -2259,6 +2258,9 @@ k_step_f32_ib(const float* __restrict__ src, float* __restrict__ dst, const doub
         const int x = min(xs + p * BX + (int)threadIdx.x, nx - 1);
         load_raw<float, BC, true>(src, x, y, nx, ny, N, cur[p], a.ref.bb);
     }
+    // the band waits with its populations in flight (the body blocks only
+    // read src, so they are the same either way)
+    if (y >= a.band0 && y < a.band1) wait_ib_forces(a);  // block-uniform
     if (NN == 3) __syncthreads();
     __bf16* lds = lds_mlp + (NN ? (threadIdx.x & ~63) * 16 : 0);
 #pragma unroll
@@ -2287,12 +2289,12 @@ __global__ void __launch_bounds__(BX) k_step_f64_ib(const double* __restrict__ s
     int gi = g + a.rot;  // rot < G
     if (gi >= G) gi -= G;
     const int y0 = gi * NPT;
-    if (y0 < a.band1 && y0 + NPT > a.band0) wait_ib_forces(a);  // block-uniform
     const bool active = (int)(bxi * BX + threadIdx.x) < nx;
     const int x = min((int)(bxi * BX + threadIdx.x), nx - 1);
     const unsigned N = (unsigned)nx * ny;
     double cur[9];
     load_raw<double, BC, true>(src, x, y0, nx, ny, N, cur);
+    if (y0 < a.band1 && y0 + NPT > a.band0) wait_ib_forces(a);  // block-uniform, first row's loads in flight
 #pragma unroll
     for (int j = 0; j < NPT; j++) {
         const int y = y0 + j;
